@@ -1,0 +1,106 @@
+/*
+ * wn.h -- C ABI of the MI355X (gfx950) generalized-winding-number engine,
+ * library mojo-libigl_amd/lib/libmojo_libigl_amd.so.
+ *
+ * Plain pointers and sizes only. Host pointers are marked (host), device
+ * (HBM) pointers (device). Every int64_t-returning wn_* call returns 0 (or a
+ * non-negative value where stated) on success and a negative status on
+ * failure, with a message in wn_last_error():
+ *   -1 bad argument, -2 no HIP device, -3 HIP runtime error,
+ *   -4 workspace too small, -5 failure injected through WN_FAULT.
+ *
+ * Face layout ("tri", the reference's): 9 x nf float64 SoA, row k in
+ * tri[k*nf : (k+1)*nf], rows ax,ay,az,bx,by,bz,cx,cy,cz
+ * (reference python/igl/__init__.py:248-250). Queries: nq x 3 float64,
+ * row-major. Output: nq float64 winding numbers.
+ *
+ * precision: 0 = FP64 plain (the reference formula in FP64 for every pair);
+ *            1 = FP32-compensated (FP32 pairs, FP64 accumulation, FP64
+ *                re-evaluation of ill-conditioned pairs).
+ * Tuning arguments (0 or -1 = library default): qpl = queries per lane
+ * (1, 2, 4), splits = face splits (split-K, 1..65535), sort = Morton-order
+ * the queries (FP32-compensated only; -1 auto), kappa = conditioning
+ * threshold of the FP64 re-evaluation (< 0: default 1.0).
+ */
+#ifndef MOJO_LIBIGL_AMD_WN_H
+#define MOJO_LIBIGL_AMD_WN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Reference-compatible entry points --------------------------------- */
+
+/* Replaces mli_winding, src/capi.mojo:938-942 (bound in the reference at
+ * python/igl/_lib.py:22 as 7 x c_int64). CPU, FP64, writes dst[q0:q1] only.
+ * (host) tri, q, dst. Reentrant; spans are fanned out by the caller. */
+void mli_winding(const double *tri, const double *q, double *dst, int64_t nf,
+                 int64_t nq, int64_t q0, int64_t q1);
+
+/* Replaces mli_winding_gpu, src/capi.mojo:945-947 (python/igl/_lib.py:23,
+ * 5 x c_int64 -> c_int64). GPU, FP64 plain. (host) tri, q, dst. Returns 1 on
+ * success, 0 on any failure (reason in wn_last_error()), as the reference's
+ * try_winding_gpu does (src/capi.mojo:366-398). Thread-safe. */
+int64_t mli_winding_gpu(const double *tri, const double *q, double *dst,
+                        int64_t nf, int64_t nq);
+
+/* ---- Status ------------------------------------------------------------- */
+
+/* No reference counterpart (the reference probes nvidia-smi from Python,
+ * python/igl/__init__.py:213-229). Number of visible HIP devices (0 if none). */
+int64_t wn_device_count(void);
+
+/* Free HBM bytes on the current device, or a negative status. Replaces the
+ * memory.free probe of python/igl/__init__.py:215-228. */
+int64_t wn_device_free_bytes(void);
+
+/* Message of the calling thread's last failure. Replaces the reference's
+ * swallow-all except (src/capi.mojo:397-398), which keeps no reason. */
+const char *wn_last_error(void);
+
+/* ABI version (1). */
+int64_t wn_version(void);
+
+/* ---- Extended forward --------------------------------------------------- */
+
+/* Extends mli_winding_gpu (src/capi.mojo:945-947) with precision and tuning.
+ * (host) tri, q, dst; synchronous. */
+int64_t wn_winding_gpu_ex(const double *tri, const double *q, double *dst,
+                          int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
+                          int64_t splits, int64_t sort, double kappa);
+
+/* Resolved launch configuration for the arguments: out4 = {qpl, splits, sort,
+ * precision}. */
+int64_t wn_resolve_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
+                          int64_t splits, int64_t sort, double kappa, int64_t *out4);
+
+/* Bytes of device workspace wn_forward needs (>= 0), or a negative status. */
+int64_t wn_forward_workspace_bytes(int64_t nf, int64_t nq, int64_t precision,
+                                   int64_t qpl, int64_t splits, int64_t sort);
+
+/* Device-resident forward pass, the path of the reference's GPU kernels
+ * winding_angles_gpu + winding_reduce_gpu (src/capi.mojo:297-363) without the
+ * nf x nq angle buffer. (device) tri, q, dst, workspace. Asynchronous on
+ * `stream` (a hipStream_t; NULL = default stream). Allocates nothing. */
+int64_t wn_forward(const double *tri, const double *q, double *dst, int64_t nf,
+                   int64_t nq, int64_t precision, int64_t qpl, int64_t splits,
+                   int64_t sort, double kappa, void *workspace,
+                   int64_t workspace_bytes, void *stream);
+
+/* Device-side ingest replacing the host SoA pack of
+ * python/igl/__init__.py:248-250: tri = V[F] as 9 x nf float64. (device) V
+ * (nv x 3; vdtype 0 = float64, 1 = float32), F (nf x 3; fdtype 0 = int64,
+ * 1 = int32), tri, bad_index (optional int32 the caller initialises to
+ * INT32_MAX; receives the smallest face index with a corner outside [0, nv)).
+ * Asynchronous on `stream`. */
+int64_t wn_pack_soa(const void *V, const void *F, int64_t nv, int64_t nf,
+                    int64_t vdtype, int64_t fdtype, double *tri,
+                    int32_t *bad_index, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOJO_LIBIGL_AMD_WN_H */

@@ -1,0 +1,257 @@
+// extern "C" boundary of the MI355X winding-number engine. Declarations and
+// the reference interface each entry point replaces: include/wn.h.
+//
+// Two families:
+//  * host-buffer calls (mli_winding_gpu, wn_winding_gpu_ex): the reference's
+//    contract -- the caller owns every host buffer, the library never retains
+//    a pointer (src/capi.mojo:1-5). Unlike the reference, which creates a
+//    DeviceContext and four device buffers per call (src/capi.mojo:366-398),
+//    one stream and grow-only device buffers persist per process, guarded by a
+//    mutex so concurrent callers are safe.
+//  * device-buffer calls (wn_forward, wn_pack_soa, wn_batch_*, wn_backward):
+//    inputs already in HBM, caller-provided workspace, caller's stream,
+//    asynchronous. No allocation inside.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "wn_internal.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int64_t fail(int64_t code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+constexpr int64_t kOk = 0;
+constexpr int64_t kErrArg = -1;
+constexpr int64_t kErrNoDevice = -2;
+constexpr int64_t kErrHip = -3;
+constexpr int64_t kErrWorkspace = -4;
+constexpr int64_t kErrInjected = -5;
+
+// Test hook: WN_FAULT=alloc|launch|sync makes the host-buffer path fail at that
+// step, from the host, without any device-side fault.
+bool injected(const char* step) {
+  const char* v = std::getenv("WN_FAULT");
+  return v && std::strcmp(v, step) == 0;
+}
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+struct HostPathContext {
+  std::mutex mu;
+  bool ready = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  void* buf = nullptr;  // tri | q | dst | workspace, grow-only
+  size_t cap = 0;
+};
+
+HostPathContext& host_ctx() {
+  static HostPathContext ctx;
+  return ctx;
+}
+
+std::string hip_msg(const char* what, hipError_t e) {
+  return std::string(what) + ": " + hipGetErrorString(e);
+}
+
+wn::FwdConfig make_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
+                          int64_t splits, int64_t sort, double kappa) {
+  wn::FwdConfig c;
+  c.precision = (int)precision;
+  c.qpl = (int)qpl;
+  c.splits = (int)splits;
+  c.sort = (int)sort;
+  c.kappa = (float)kappa;
+  if (c.qpl <= 0) {
+    c.qpl = c.precision == wn::kFP64 ? 1 : 2;
+    // Keep at least ~2 blocks per CU before widening lanes.
+    while (c.qpl > 1 && (nq + 256 * c.qpl - 1) / (256 * c.qpl) < 512) c.qpl /= 2;
+  }
+  if (c.splits <= 0) {
+    // Fill 256 CUs x 8 blocks, but keep >= 2 face tiles per split.
+    const int64_t qblocks = (nq + 256 * c.qpl - 1) / (256 * c.qpl);
+    int64_t s = (2048 + qblocks - 1) / (qblocks > 0 ? qblocks : 1);
+    const int64_t max_s = nf / (2 * wn::kFaceTile);
+    if (s > max_s) s = max_s;
+    if (s < 1) s = 1;
+    if (s > 4096) s = 4096;
+    c.splits = (int)s;
+  }
+  if (c.sort < 0) c.sort = (c.precision == wn::kFP32C && nq >= 4096) ? 1 : 0;
+  if (kappa < 0.0) c.kappa = 1.0f;
+  return c;
+}
+
+int64_t host_path(const double* tri, const double* q, double* dst, int64_t nf, int64_t nq,
+                  const wn::FwdConfig& cfg) {
+  if (nq < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (nq == 0) return kOk;
+  if (!tri && nf > 0) return fail(kErrArg, "null tri");
+  if (!q || !dst) return fail(kErrArg, "null query or output buffer");
+  if (const char* why = wn::fwd_check(nf, nq, cfg)) return fail(kErrArg, why);
+  HostPathContext& ctx = host_ctx();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hipError_t e;
+  if (!ctx.ready) {
+    if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+    e = hipGetDevice(&ctx.device);
+    if (e != hipSuccess) return fail(kErrHip, hip_msg("hipGetDevice", e));
+    e = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(kErrHip, hip_msg("hipStreamCreate", e));
+    ctx.ready = true;
+  }
+  e = hipSetDevice(ctx.device);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipSetDevice", e));
+
+  const size_t tri_b = wn::align_up((size_t)nf * 9 * sizeof(double));
+  const size_t q_b = wn::align_up((size_t)nq * 3 * sizeof(double));
+  const size_t dst_b = wn::align_up((size_t)nq * sizeof(double));
+  const size_t ws_b = wn::fwd_workspace_bytes(nf, nq, cfg);
+  const size_t need = tri_b + q_b + dst_b + ws_b;
+  if (injected("alloc")) return fail(kErrInjected, "injected allocation failure (WN_FAULT=alloc)");
+  if (need > ctx.cap) {
+    if (ctx.buf) {
+      (void)hipStreamSynchronize(ctx.stream);
+      (void)hipFree(ctx.buf);
+      ctx.buf = nullptr;
+      ctx.cap = 0;
+    }
+    e = hipMalloc(&ctx.buf, need);
+    if (e != hipSuccess) {
+      ctx.buf = nullptr;
+      return fail(kErrHip, hip_msg("hipMalloc", e));
+    }
+    ctx.cap = need;
+  }
+  char* base = (char*)ctx.buf;
+  double* d_tri = (double*)base;
+  double* d_q = (double*)(base + tri_b);
+  double* d_dst = (double*)(base + tri_b + q_b);
+  void* d_ws = base + tri_b + q_b + dst_b;
+
+  if (nf > 0) {
+    e = hipMemcpyAsync(d_tri, tri, (size_t)nf * 9 * sizeof(double), hipMemcpyHostToDevice,
+                       ctx.stream);
+    if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy tri", e));
+  }
+  e = hipMemcpyAsync(d_q, q, (size_t)nq * 3 * sizeof(double), hipMemcpyHostToDevice, ctx.stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy q", e));
+  if (injected("launch")) {
+    (void)hipStreamSynchronize(ctx.stream);
+    return fail(kErrInjected, "injected launch failure (WN_FAULT=launch)");
+  }
+  e = wn::fwd_launch(d_tri, d_q, d_dst, nf, nq, cfg, d_ws, ws_b, ctx.stream);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(ctx.stream);
+    return fail(kErrHip, hip_msg("kernel launch", e));
+  }
+  e = hipMemcpyAsync(dst, d_dst, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost, ctx.stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy dst", e));
+  e = hipStreamSynchronize(ctx.stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipStreamSynchronize", e));
+  if (injected("sync")) return fail(kErrInjected, "injected synchronisation failure (WN_FAULT=sync)");
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t wn_version(void) { return 1; }
+
+const char* wn_last_error(void) { return g_error.c_str(); }
+
+int64_t wn_device_count(void) { return device_count(); }
+
+int64_t wn_device_free_bytes(void) {
+  if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  size_t free_b = 0, total_b = 0;
+  hipError_t e = hipMemGetInfo(&free_b, &total_b);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemGetInfo", e));
+  return (int64_t)free_b;
+}
+
+int64_t mli_winding_gpu(const double* tri, const double* q, double* dst, int64_t nf,
+                        int64_t nq) {
+  const wn::FwdConfig cfg = make_config(nf, nq, wn::kFP64, 0, 0, 0, 1.0);
+  return host_path(tri, q, dst, nf, nq, cfg) == kOk ? 1 : 0;
+}
+
+int64_t wn_winding_gpu_ex(const double* tri, const double* q, double* dst, int64_t nf,
+                          int64_t nq, int64_t precision, int64_t qpl, int64_t splits,
+                          int64_t sort, double kappa) {
+  const wn::FwdConfig cfg = make_config(nf, nq, precision, qpl, splits, sort, kappa);
+  return host_path(tri, q, dst, nf, nq, cfg);
+}
+
+int64_t wn_resolve_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
+                          int64_t splits, int64_t sort, double kappa, int64_t* out4) {
+  const wn::FwdConfig c = make_config(nf, nq, precision, qpl, splits, sort, kappa);
+  if (const char* why = wn::fwd_check(nf, nq, c)) return fail(kErrArg, why);
+  if (out4) {
+    out4[0] = c.qpl;
+    out4[1] = c.splits;
+    out4[2] = c.sort;
+    out4[3] = c.precision;
+  }
+  return kOk;
+}
+
+int64_t wn_forward_workspace_bytes(int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
+                                   int64_t splits, int64_t sort) {
+  const wn::FwdConfig c = make_config(nf, nq, precision, qpl, splits, sort, 1.0);
+  if (const char* why = wn::fwd_check(nf, nq, c)) return fail(kErrArg, why);
+  if (c.precision == wn::kFP32C && c.sort && device_count() < 1)
+    return fail(kErrNoDevice, "no HIP device available");
+  return (int64_t)wn::fwd_workspace_bytes(nf, nq, c);
+}
+
+int64_t wn_forward(const double* tri, const double* q, double* dst, int64_t nf, int64_t nq,
+                   int64_t precision, int64_t qpl, int64_t splits, int64_t sort, double kappa,
+                   void* workspace, int64_t workspace_bytes, void* stream) {
+  const wn::FwdConfig c = make_config(nf, nq, precision, qpl, splits, sort, kappa);
+  if (const char* why = wn::fwd_check(nf, nq, c)) return fail(kErrArg, why);
+  if (nq == 0) return kOk;
+  if ((!tri && nf > 0) || !q || !dst) return fail(kErrArg, "null device buffer");
+  const size_t need = wn::fwd_workspace_bytes(nf, nq, c);
+  if (workspace_bytes < 0 || (size_t)workspace_bytes < need || (need > 0 && !workspace))
+    return fail(kErrWorkspace, "workspace smaller than wn_forward_workspace_bytes()");
+  hipError_t e = wn::fwd_launch(tri, q, dst, nf, nq, c, workspace, (size_t)workspace_bytes,
+                                (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_forward", e));
+  return kOk;
+}
+
+int64_t wn_pack_soa(const void* V, const void* F, int64_t nv, int64_t nf, int64_t vdtype,
+                    int64_t fdtype, double* tri, int32_t* bad_index, void* stream) {
+  if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if ((vdtype != 0 && vdtype != 1) || (fdtype != 0 && fdtype != 1))
+    return fail(kErrArg, "vdtype/fdtype must be 0 (64-bit) or 1 (32-bit)");
+  if (nf > 0 && (!V || !F || !tri)) return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::pack_soa_launch(V, F, nv, nf, (int)vdtype, (int)fdtype, tri, bad_index,
+                                     (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_pack_soa", e));
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+// Device-side building blocks of the winding-number kernels (gfx950, wave64).
+//
+// Per query-face pair the reference evaluates (src/capi.mojo:218-235, 309-325)
+//   a = A - O, b = B - O, c = C - O, la = |a|, lb = |b|, lc = |c|
+//   det = a . (b x c)
+//   den = la lb lc + (a.b) lc + (b.c) la + (c.a) lb
+//   angle = atan2(det, den)            (half the signed solid angle)
+// and w(O) = sum_f angle_f / (2 pi)  (src/capi.mojo:286).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wn {
+
+constexpr int kWave = 64;
+constexpr double kInv2Pi = 0.159154943091895335768883763372514362;
+
+// ---------------------------------------------------------------------------
+// FP32 atan2 with our own minimax polynomial.
+//
+// atan(r) ~= r * P(r^2) on r in [0, 1], P of degree 8 in r^2, coefficients
+// from a Remez exchange minimising the RELATIVE error (max 1.52e-8, below
+// half an FP32 ulp) computed at 40 digits in mpmath by gen_atan_coeffs.py in
+// this directory. Octant reduction: r = min(|y|,|x|) / max(|y|,|x|), then
+// pi/2 - t if |y| > |x|, pi - t if x < 0, sign of y. The division is
+// v_rcp_f32 (1 ulp), the relative accuracy of the result is ~2 ulp.
+// `mx` returns max(|y|, |x|, 1e-30) for the conditioning test of the caller.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float atan2_f32(float y, float x, float& mx) {
+  const float ay = __builtin_fabsf(y);
+  const float ax = __builtin_fabsf(x);
+  mx = __builtin_fmaxf(__builtin_fmaxf(ay, ax), 1e-30f);
+  const float mn = __builtin_fminf(ay, ax);
+  const float r = mn * __builtin_amdgcn_rcpf(mx);
+  const float t = r * r;
+  float p = 0.0028498926605998343f;
+  p = __builtin_fmaf(p, t, -0.016068640521845983f);
+  p = __builtin_fmaf(p, t, 0.04269153699916155f);
+  p = __builtin_fmaf(p, t, -0.075042959409187448f);
+  p = __builtin_fmaf(p, t, 0.10640934631555586f);
+  p = __builtin_fmaf(p, t, -0.14203644605898132f);
+  p = __builtin_fmaf(p, t, 0.19992619406714787f);
+  p = __builtin_fmaf(p, t, -0.33333073345588504f);
+  p = __builtin_fmaf(p, t, 0.99999998476576588f);
+  float a = r * p;
+  a = (ay > ax) ? (1.57079632679489662f - a) : a;
+  a = (x < 0.0f) ? (3.14159265358979324f - a) : a;
+  return __builtin_copysignf(a, y);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 pair in the reference's exact arithmetic form (cross-product det,
+// dot-product den), with OCML's correctly-rounded sqrt and atan2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double pair_f64(double ax, double ay, double az,
+                                           double bx, double by, double bz,
+                                           double cx, double cy, double cz) {
+  const double la = sqrt(ax * ax + ay * ay + az * az);
+  const double lb = sqrt(bx * bx + by * by + bz * bz);
+  const double lc = sqrt(cx * cx + cy * cy + cz * cz);
+  const double det = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) +
+                     az * (bx * cy - by * cx);
+  double den = la * lb * lc;
+  den += (ax * bx + ay * by + az * bz) * lc;
+  den += (bx * cx + by * cy + bz * cz) * la;
+  den += (cx * ax + cy * ay + cz * az) * lb;
+  return atan2(det, den);
+}
+
+// ---------------------------------------------------------------------------
+// FP32-compensated face record, 64 bytes, staged in LDS per block.
+// Coordinates are relative to the block's local origin (its query bbox
+// centre), rounded once to FP32; N2 = 2 (B'-A') x (C'-A') and the squared edge
+// lengths are computed in FP64 from those same rounded corners, so every
+// quantity the pair uses describes one consistent (rounded) triangle.
+//   r0 = (A.x, A.y, A.z, B.x)   r1 = (B.y, B.z, C.x, C.y)
+//   r2 = (C.z, N2.x, N2.y, N2.z) r3 = (|B-A|^2, |C-B|^2, |A-C|^2, 0)
+// ---------------------------------------------------------------------------
+struct FaceRec32 {
+  float4 r0, r1, r2, r3;
+};
+
+// Build the record from FP64 corners and the FP64 local origin (gx,gy,gz).
+__device__ __forceinline__ FaceRec32 make_face_rec32(double Ax, double Ay, double Az,
+                                                     double Bx, double By, double Bz,
+                                                     double Cx, double Cy, double Cz,
+                                                     double gx, double gy, double gz) {
+  const float ax = (float)(Ax - gx), ay = (float)(Ay - gy), az = (float)(Az - gz);
+  const float bx = (float)(Bx - gx), by = (float)(By - gy), bz = (float)(Bz - gz);
+  const float cx = (float)(Cx - gx), cy = (float)(Cy - gy), cz = (float)(Cz - gz);
+  const double e1x = (double)bx - (double)ax, e1y = (double)by - (double)ay,
+               e1z = (double)bz - (double)az;
+  const double e2x = (double)cx - (double)ax, e2y = (double)cy - (double)ay,
+               e2z = (double)cz - (double)az;
+  const double e3x = (double)cx - (double)bx, e3y = (double)cy - (double)by,
+               e3z = (double)cz - (double)bz;
+  const double nx = 2.0 * (e1y * e2z - e1z * e2y);
+  const double ny = 2.0 * (e1z * e2x - e1x * e2z);
+  const double nz = 2.0 * (e1x * e2y - e1y * e2x);
+  FaceRec32 r;
+  r.r0 = make_float4(ax, ay, az, bx);
+  r.r1 = make_float4(by, bz, cx, cy);
+  r.r2 = make_float4(cz, (float)nx, (float)ny, (float)nz);
+  r.r3 = make_float4((float)(e1x * e1x + e1y * e1y + e1z * e1z),
+                     (float)(e3x * e3x + e3y * e3y + e3z * e3z),
+                     (float)(e2x * e2x + e2y * e2y + e2z * e2z), 0.0f);
+  return r;
+}
+
+// FP64 re-evaluation of one pair from the same FP32 record and FP32 query
+// offset. Differences of FP32 values are exact in FP64, so this is the
+// reference formula on the rounded triangle at ~1e-16 relative accuracy.
+__device__ __noinline__ double pair_rec32_f64(float4 r0, float4 r1, float4 r2,
+                                              float ox, float oy, float oz) {
+  const double qx = ox, qy = oy, qz = oz;
+  return pair_f64((double)r0.x - qx, (double)r0.y - qy, (double)r0.z - qz,
+                  (double)r0.w - qx, (double)r1.x - qy, (double)r1.y - qz,
+                  (double)r1.z - qx, (double)r1.w - qy, (double)r2.x - qz);
+}
+
+// FP32 pair from a record. Algebra (exact identities, cheaper and better
+// conditioned than the direct form):
+//   det = N . a              with N = (B-A) x (C-A)  (det(a,b,c) = det(a,b-a,c-a))
+//   2 a.b = la^2 + lb^2 - |B-A|^2   (law of cosines), likewise b.c, c.a
+// so with det2 = 2 det and den2 = 2 den (atan2 is invariant under the common
+// positive factor):
+//   den2 = lc (2 la lb + s_ab) + la s_bc + lb s_ca
+// `risky` is set when max(|det2|, |den2|) < kappa * 2 la lb lc: the pair's
+// atan2 is ill-conditioned there (query near an edge line, or close to the
+// face inside its outline), and the caller re-evaluates it in FP64.
+// For a far face max(|det2|,|den2|) -> 4 * 2 la lb lc.
+__device__ __forceinline__ float pair_rec32(const float4 r0, const float4 r1,
+                                            const float4 r2, const float4 r3,
+                                            float ox, float oy, float oz,
+                                            float kappa, bool& risky) {
+  const float ax = r0.x - ox, ay = r0.y - oy, az = r0.z - oz;
+  const float bx = r0.w - ox, by = r1.x - oy, bz = r1.y - oz;
+  const float cx = r1.z - ox, cy = r1.w - oy, cz = r2.x - oz;
+  const float la2 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
+  const float lb2 = __builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx));
+  const float lc2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
+  const float la = __builtin_amdgcn_sqrtf(la2);
+  const float lb = __builtin_amdgcn_sqrtf(lb2);
+  const float lc = __builtin_amdgcn_sqrtf(lc2);
+  const float det2 = __builtin_fmaf(r2.w, az, __builtin_fmaf(r2.z, ay, r2.y * ax));
+  const float sab = (la2 + lb2) - r3.x;
+  const float sbc = (lb2 + lc2) - r3.y;
+  const float sca = (lc2 + la2) - r3.z;
+  const float u2 = 2.0f * (la * lb);
+  const float s2 = u2 * lc;
+  const float den2 = __builtin_fmaf(lc, u2 + sab, __builtin_fmaf(la, sbc, lb * sca));
+  float mx;
+  const float ang = atan2_f32(det2, den2, mx);
+  risky = mx < kappa * s2;
+  return ang;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront reductions.
+// ---------------------------------------------------------------------------
+// Sum over the 64 lanes of a wave; the total is valid in lane 0 (and, with a
+// lane-dependent rounding order, in the others). Steps: DPP quad_perm xor 1,
+// xor 2, row_ror 4, row_ror 8 inside each 16-lane row, then ds_swizzle xor 16
+// inside each half, then a cross-half permute. Fixed order => deterministic.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move_f64(double v) {
+  const uint64_t bits = __double_as_longlong(v);
+  const int lo = (int)(uint32_t)bits;
+  const int hi = (int)(uint32_t)(bits >> 32);
+  const int lo2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, true);
+  const int hi2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi2 << 32) | (uint32_t)lo2));
+}
+
+__device__ __forceinline__ double swizzle_xor16_f64(double v) {
+  const uint64_t bits = __double_as_longlong(v);
+  // ds_swizzle bit mode: and_mask = 0x1F, or_mask = 0, xor_mask = 0x10.
+  const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)bits, 0x401F);
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(bits >> 32), 0x401F);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  v += dpp_move_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move_f64<0x124>(v);  // row_ror:4
+  v += dpp_move_f64<0x128>(v);  // row_ror:8
+  v += swizzle_xor16_f64(v);
+  v += __shfl_xor(v, 32, kWave);
+  return v;
+}
+
+__device__ __forceinline__ double wave_min_f64(double v) {
+  for (int m = 1; m < kWave; m <<= 1) v = fmin(v, __shfl_xor(v, m, kWave));
+  return v;
+}
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+  for (int m = 1; m < kWave; m <<= 1) v = fmax(v, __shfl_xor(v, m, kWave));
+  return v;
+}
+
+}  // namespace wn

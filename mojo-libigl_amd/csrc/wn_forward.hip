@@ -1,0 +1,340 @@
+// Forward winding-number kernels for gfx950.
+//
+// Both kernels are "one query per lane, faces broadcast from LDS": a block of
+// 256 threads owns 256*QPL queries, streams the face list through LDS tiles of
+// 256 faces (one face staged per thread, coalesced loads from the 9 x nf SoA),
+// and every lane sweeps each tile with all its queries. Accumulation stays in
+// registers in fixed face order, so results are bitwise reproducible.
+//
+// Split-K: grid.y splits the face range. Each (block, split) writes an
+// unscaled partial to partial[q * splits + s]; reduce_splits sums the splits
+// of one query in one wave (DPP/ds_swizzle tree, fixed order) and applies
+// 1/(2 pi). With splits == 1 the kernel writes dst directly.
+//
+// The reference instead stores an nf x nq FP64 angle buffer and reduces it
+// serially per query (src/capi.mojo:297-363, 366-398); nothing of nf*nq size
+// is materialised here.
+
+#include "wn_device.h"
+#include "wn_internal.h"
+
+namespace wn {
+
+// --------------------------------------------------------------------------
+// FP64 plain: the reference formula in FP64 for every pair.
+// --------------------------------------------------------------------------
+template <int QPL>
+__global__ __launch_bounds__(kBlockThreads) void fwd_f64_kernel(
+    const double* __restrict__ tri, const double* __restrict__ q,
+    double* __restrict__ out, int64_t nf, int64_t nq, int64_t faces_per_split,
+    int splits) {
+  __shared__ double2 sface[kFaceTile][5];
+  const int tid = threadIdx.x;
+  const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
+
+  double ox[QPL], oy[QPL], oz[QPL], acc[QPL];
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    const bool ok = qi < nq;
+    ox[k] = ok ? q[3 * qi] : 0.0;
+    oy[k] = ok ? q[3 * qi + 1] : 0.0;
+    oz[k] = ok ? q[3 * qi + 2] : 0.0;
+    acc[k] = 0.0;
+  }
+
+  const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
+  const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
+  for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
+    const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t f = t0 + tid;
+      sface[tid][0] = make_double2(tri[f], tri[nf + f]);
+      sface[tid][1] = make_double2(tri[2 * nf + f], tri[3 * nf + f]);
+      sface[tid][2] = make_double2(tri[4 * nf + f], tri[5 * nf + f]);
+      sface[tid][3] = make_double2(tri[6 * nf + f], tri[7 * nf + f]);
+      sface[tid][4] = make_double2(tri[8 * nf + f], 0.0);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const double2 p0 = sface[j][0], p1 = sface[j][1], p2 = sface[j][2],
+                    p3 = sface[j][3], p4 = sface[j][4];
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        acc[k] += pair_f64(p0.x - ox[k], p0.y - oy[k], p1.x - oz[k],
+                           p1.y - ox[k], p2.x - oy[k], p2.y - oz[k],
+                           p3.x - ox[k], p3.y - oy[k], p4.x - oz[k]);
+      }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    if (qi < nq) {
+      if (splits == 1)
+        out[qi] = acc[k] * kInv2Pi;
+      else
+        out[qi * splits + blockIdx.y] = acc[k];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// FP32-compensated.
+//  * local origin: the block's query bbox centre g (FP64). Query offsets
+//    o = O - g and face corners A - g are rounded to FP32 once, so the whole
+//    block works on one consistently rounded copy of the mesh whose rounding
+//    error is eps * |A - g|: small exactly where it matters (faces near the
+//    block's queries). With Morton-ordered queries (perm) blocks are compact.
+//  * per pair: FP32 arithmetic (pair_rec32), angle accumulated into an FP64
+//    running sum (an FP32 value converts to FP64 exactly; the FP64 sum of the
+//    per-face terms keeps ~1e-16 relative accumulation error, which is the
+//    compensation a Kahan/TwoSum FP32 sum would provide, for 2 VALU ops).
+//  * ill-conditioned pairs (risky) are re-evaluated in FP64 from the same
+//    FP32 record (pair_rec32_f64).
+// --------------------------------------------------------------------------
+template <int QPL>
+__global__ __launch_bounds__(kBlockThreads) void fwd_f32c_kernel(
+    const double* __restrict__ tri, const double* __restrict__ q,
+    const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nf,
+    int64_t nq, int64_t faces_per_split, int splits, float kappa) {
+  __shared__ FaceRec32 sface[kFaceTile];
+  __shared__ double sred[6][kBlockThreads / kWave];
+  const int tid = threadIdx.x;
+  const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
+
+  int64_t qidx[QPL];
+  double Ox[QPL], Oy[QPL], Oz[QPL];
+  double lo0 = __builtin_inf(), lo1 = __builtin_inf(), lo2 = __builtin_inf();
+  double hi0 = -__builtin_inf(), hi1 = -__builtin_inf(), hi2 = -__builtin_inf();
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t slot = qbase + k * kBlockThreads + tid;
+    const bool ok = slot < nq;
+    qidx[k] = ok ? (perm ? (int64_t)perm[slot] : slot) : -1;
+    if (ok) {
+      Ox[k] = q[3 * qidx[k]];
+      Oy[k] = q[3 * qidx[k] + 1];
+      Oz[k] = q[3 * qidx[k] + 2];
+      lo0 = fmin(lo0, Ox[k]); hi0 = fmax(hi0, Ox[k]);
+      lo1 = fmin(lo1, Oy[k]); hi1 = fmax(hi1, Oy[k]);
+      lo2 = fmin(lo2, Oz[k]); hi2 = fmax(hi2, Oz[k]);
+    }
+  }
+  lo0 = wave_min_f64(lo0); lo1 = wave_min_f64(lo1); lo2 = wave_min_f64(lo2);
+  hi0 = wave_max_f64(hi0); hi1 = wave_max_f64(hi1); hi2 = wave_max_f64(hi2);
+  const int wave = tid / kWave;
+  if ((tid & (kWave - 1)) == 0) {
+    sred[0][wave] = lo0; sred[1][wave] = lo1; sred[2][wave] = lo2;
+    sred[3][wave] = hi0; sred[4][wave] = hi1; sred[5][wave] = hi2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kBlockThreads / kWave; ++w) {
+    lo0 = fmin(lo0, sred[0][w]); lo1 = fmin(lo1, sred[1][w]); lo2 = fmin(lo2, sred[2][w]);
+    hi0 = fmax(hi0, sred[3][w]); hi1 = fmax(hi1, sred[4][w]); hi2 = fmax(hi2, sred[5][w]);
+  }
+  const double gx = 0.5 * (lo0 + hi0), gy = 0.5 * (lo1 + hi1), gz = 0.5 * (lo2 + hi2);
+
+  float ox[QPL], oy[QPL], oz[QPL];
+  double acc[QPL];
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const bool ok = qidx[k] >= 0;
+    ox[k] = ok ? (float)(Ox[k] - gx) : 0.0f;
+    oy[k] = ok ? (float)(Oy[k] - gy) : 0.0f;
+    oz[k] = ok ? (float)(Oz[k] - gz) : 0.0f;
+    acc[k] = 0.0;
+  }
+
+  const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
+  const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
+  for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
+    const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t f = t0 + tid;
+      sface[tid] = make_face_rec32(tri[f], tri[nf + f], tri[2 * nf + f],
+                                   tri[3 * nf + f], tri[4 * nf + f], tri[5 * nf + f],
+                                   tri[6 * nf + f], tri[7 * nf + f], tri[8 * nf + f],
+                                   gx, gy, gz);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const float4 r0 = sface[j].r0, r1 = sface[j].r1, r2 = sface[j].r2,
+                   r3 = sface[j].r3;
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        bool risky;
+        const float ang = pair_rec32(r0, r1, r2, r3, ox[k], oy[k], oz[k], kappa, risky);
+        double a64 = (double)ang;
+        if (risky) a64 = pair_rec32_f64(r0, r1, r2, ox[k], oy[k], oz[k]);
+        acc[k] += a64;
+      }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    if (qidx[k] >= 0) {
+      if (splits == 1)
+        out[qidx[k]] = acc[k] * kInv2Pi;
+      else
+        out[qidx[k] * splits + blockIdx.y] = acc[k];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void fill_f64_kernel(double* __restrict__ p,
+                                                                 int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// One wave per query: lanes stride over the splits, then a fixed DPP tree.
+__global__ __launch_bounds__(kBlockThreads) void reduce_splits_kernel(
+    const double* __restrict__ partial, double* __restrict__ dst, int64_t nq,
+    int splits) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t qi = (int64_t)blockIdx.x * (kBlockThreads / kWave) + threadIdx.x / kWave;
+  if (qi >= nq) return;  // wave-uniform
+  double s = 0.0;
+  for (int j = lane; j < splits; j += kWave) s += partial[qi * splits + j];
+  s = wave_sum_f64(s);
+  if (lane == 0) dst[qi] = s * kInv2Pi;
+}
+
+// --------------------------------------------------------------------------
+// Device ingest: tri[k*nf + f] = V[F[f, k/3], k%3]. Out-of-range indices are
+// reported through bad_index (first offending face, via atomicMin) and the
+// corner is zeroed.
+// --------------------------------------------------------------------------
+template <typename VT, typename FT>
+__global__ __launch_bounds__(kBlockThreads) void pack_soa_kernel(
+    const VT* __restrict__ V, const FT* __restrict__ F, int64_t nv, int64_t nf,
+    double* __restrict__ tri, int32_t* __restrict__ bad_index) {
+  const int64_t f = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (f >= nf) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t vi = (int64_t)F[3 * f + c];
+    const bool ok = vi >= 0 && vi < nv;
+    if (!ok && bad_index) atomicMin(bad_index, (int32_t)(f < 0x7fffffff ? f : 0x7fffffff));
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      tri[(3 * c + d) * nf + f] = ok ? (double)V[3 * vi + d] : 0.0;
+  }
+}
+
+hipError_t pack_soa_launch(const void* V, const void* F, int64_t nv, int64_t nf,
+                           int vdtype, int fdtype, double* tri, int32_t* bad_index,
+                           hipStream_t stream) {
+  if (nf <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((nf + kBlockThreads - 1) / kBlockThreads));
+  if (vdtype == 0 && fdtype == 0)
+    pack_soa_kernel<double, int64_t><<<grid, kBlockThreads, 0, stream>>>(
+        (const double*)V, (const int64_t*)F, nv, nf, tri, bad_index);
+  else if (vdtype == 0 && fdtype == 1)
+    pack_soa_kernel<double, int32_t><<<grid, kBlockThreads, 0, stream>>>(
+        (const double*)V, (const int32_t*)F, nv, nf, tri, bad_index);
+  else if (vdtype == 1 && fdtype == 0)
+    pack_soa_kernel<float, int64_t><<<grid, kBlockThreads, 0, stream>>>(
+        (const float*)V, (const int64_t*)F, nv, nf, tri, bad_index);
+  else
+    pack_soa_kernel<float, int32_t><<<grid, kBlockThreads, 0, stream>>>(
+        (const float*)V, (const int32_t*)F, nv, nf, tri, bad_index);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Host launchers.
+// --------------------------------------------------------------------------
+static int64_t faces_per_split_of(int64_t nf, int splits) {
+  int64_t per = (nf + splits - 1) / splits;
+  per = (per + kFaceTile - 1) / kFaceTile * kFaceTile;
+  return per > 0 ? per : kFaceTile;
+}
+
+const char* fwd_check(int64_t nf, int64_t nq, const FwdConfig& cfg) {
+  if (nf < 0 || nq < 0) return "negative size";
+  if (cfg.precision != kFP64 && cfg.precision != kFP32C) return "unknown precision";
+  if (cfg.qpl != 1 && cfg.qpl != 2 && cfg.qpl != 4) return "queries_per_lane must be 1, 2 or 4";
+  if (cfg.precision == kFP64 && cfg.qpl == 4) return "fp64 supports queries_per_lane 1 or 2";
+  if (cfg.splits < 1 || cfg.splits > 65535) return "splits must be in [1, 65535]";
+  if (nq > (int64_t)0x7fffffff) return "more than 2^31-1 queries per call";
+  const int64_t qblock = (int64_t)kBlockThreads * cfg.qpl;
+  if ((nq + qblock - 1) / qblock > (int64_t)0x7fffffff) return "too many query blocks";
+  if (!(cfg.kappa >= 0.0f)) return "kappa must be >= 0";
+  return nullptr;
+}
+
+size_t fwd_workspace_bytes(int64_t nf, int64_t nq, const FwdConfig& cfg) {
+  (void)nf;
+  size_t bytes = 0;
+  if (cfg.splits > 1) bytes += align_up((size_t)nq * (size_t)cfg.splits * sizeof(double));
+  if (cfg.precision == kFP32C && cfg.sort) {
+    bytes += align_up((size_t)nq * sizeof(int32_t));
+    bytes += morton_workspace_bytes(nq);
+  }
+  return bytes;
+}
+
+hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t nf,
+                      int64_t nq, const FwdConfig& cfg, void* ws, size_t ws_bytes,
+                      hipStream_t stream) {
+  if (nq == 0) return hipSuccess;
+  if (ws_bytes < fwd_workspace_bytes(nf, nq, cfg)) return hipErrorInvalidValue;
+  char* cursor = (char*)ws;
+  double* partial = nullptr;
+  if (cfg.splits > 1) {
+    partial = (double*)cursor;
+    cursor += align_up((size_t)nq * (size_t)cfg.splits * sizeof(double));
+  }
+  int32_t* perm = nullptr;
+  if (cfg.precision == kFP32C && cfg.sort) {
+    perm = (int32_t*)cursor;
+    cursor += align_up((size_t)nq * sizeof(int32_t));
+    hipError_t e = morton_order(q, nq, perm, cursor, morton_workspace_bytes(nq), stream);
+    if (e != hipSuccess) return e;
+  }
+  double* out = cfg.splits > 1 ? partial : dst;
+  const int64_t fps = faces_per_split_of(nf, cfg.splits);
+  const int64_t qblock = (int64_t)kBlockThreads * cfg.qpl;
+  const dim3 grid((unsigned)((nq + qblock - 1) / qblock), (unsigned)cfg.splits);
+  if (nf == 0) {
+    // Empty mesh: w = 0 everywhere (a fill kernel, not a memset node, so the
+    // launch sequence stays safe to capture in a graph).
+    fill_f64_kernel<<<dim3((unsigned)((nq + kBlockThreads - 1) / kBlockThreads)),
+                      kBlockThreads, 0, stream>>>(dst, nq, 0.0);
+    return hipGetLastError();
+  }
+  if (cfg.precision == kFP64) {
+    if (cfg.qpl == 1)
+      fwd_f64_kernel<1><<<grid, kBlockThreads, 0, stream>>>(tri, q, out, nf, nq, fps, cfg.splits);
+    else
+      fwd_f64_kernel<2><<<grid, kBlockThreads, 0, stream>>>(tri, q, out, nf, nq, fps, cfg.splits);
+  } else {
+    if (cfg.qpl == 1)
+      fwd_f32c_kernel<1><<<grid, kBlockThreads, 0, stream>>>(tri, q, perm, out, nf, nq, fps,
+                                                             cfg.splits, cfg.kappa);
+    else if (cfg.qpl == 2)
+      fwd_f32c_kernel<2><<<grid, kBlockThreads, 0, stream>>>(tri, q, perm, out, nf, nq, fps,
+                                                             cfg.splits, cfg.kappa);
+    else
+      fwd_f32c_kernel<4><<<grid, kBlockThreads, 0, stream>>>(tri, q, perm, out, nf, nq, fps,
+                                                             cfg.splits, cfg.kappa);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (cfg.splits > 1) {
+    const int per_block = kBlockThreads / kWave;
+    const dim3 rgrid((unsigned)((nq + per_block - 1) / per_block));
+    reduce_splits_kernel<<<rgrid, kBlockThreads, 0, stream>>>(partial, dst, nq, cfg.splits);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+}  // namespace wn

@@ -1,0 +1,44 @@
+// Host-side interfaces between the C-ABI layer and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace wn {
+
+enum Precision : int { kFP64 = 0, kFP32C = 1 };
+
+constexpr int kBlockThreads = 256;  // 4 waves
+constexpr int kFaceTile = 256;      // faces staged in LDS per tile (one per thread)
+
+struct FwdConfig {
+  int precision = kFP64;  // kFP64: plain FP64; kFP32C: FP32 + FP64 accumulation/fallback
+  int qpl = 1;            // queries per lane (1, 2 or 4)
+  int splits = 1;         // face splits (split-K over faces), >= 1
+  int sort = 0;           // kFP32C: process queries in Morton order
+  float kappa = 1.0f;     // kFP32C: conditioning threshold for the FP64 re-evaluation
+};
+
+// Rounds to 256 bytes so sub-buffers stay aligned.
+inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
+
+// Forward pass (wn_forward.hip).
+size_t fwd_workspace_bytes(int64_t nf, int64_t nq, const FwdConfig& cfg);
+const char* fwd_check(int64_t nf, int64_t nq, const FwdConfig& cfg);  // nullptr if ok
+hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t nf,
+                      int64_t nq, const FwdConfig& cfg, void* ws, size_t ws_bytes,
+                      hipStream_t stream);
+
+// Morton ordering of query points (wn_sort.hip).
+size_t morton_workspace_bytes(int64_t nq);
+hipError_t morton_order(const double* q, int64_t nq, int32_t* perm, void* ws,
+                        size_t ws_bytes, hipStream_t stream);
+
+// Device-side gather of V[F] into the 9 x nf FP64 face SoA (wn_forward.hip).
+// vdtype: 0 = float64, 1 = float32; fdtype: 0 = int64, 1 = int32.
+hipError_t pack_soa_launch(const void* V, const void* F, int64_t nv, int64_t nf,
+                           int vdtype, int fdtype, double* tri, int32_t* bad_index,
+                           hipStream_t stream);
+
+}  // namespace wn

@@ -1,0 +1,107 @@
+"""ctypes bridge to the MI355X winding-number library.
+
+Mirrors the reference's shim (python/igl/_lib.py): integer addresses cross the
+ABI as ``c_int64`` (``:16-32``), the CPU kernel is fanned out over query spans
+on a thread pool (``:37-95``), and ``lib()`` binds every declared symbol
+(``:122-130``). Differences: the library is HIP + C++
+(``mojo-libigl_amd/lib/libmojo_libigl_amd.so``, override with ``$WN_LIB``) and
+is never rebuilt implicitly -- a missing library raises, so nothing silently
+runs a different code path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+PACKAGE_ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB = os.environ.get("WN_LIB") or os.path.join(PACKAGE_ROOT, "lib", "libmojo_libigl_amd.so")
+
+I = ctypes.c_int64
+D = ctypes.c_double
+S = ctypes.c_char_p
+
+# name -> (argument types, result type). Pointers are passed as c_int64
+# addresses, as in the reference (python/igl/_lib.py:16-32).
+_SIGNATURES = {
+    "mli_winding": ([I] * 7, None),
+    "mli_winding_gpu": ([I] * 5, I),
+    "wn_device_count": ([], I),
+    "wn_device_free_bytes": ([], I),
+    "wn_last_error": ([], S),
+    "wn_version": ([], I),
+    "wn_winding_gpu_ex": ([I] * 9 + [D], I),
+    "wn_resolve_config": ([I] * 6 + [D, I], I),
+    "wn_forward_workspace_bytes": ([I] * 6, I),
+    "wn_forward": ([I] * 9 + [D] + [I] * 3, I),
+    "wn_pack_soa": ([I] * 9, I),
+}
+
+# Same policy as the reference (python/igl/_lib.py:37-50): at most four
+# workers, and no split below the measured crossover of the query-face sweep.
+MAX_WORKERS = 4
+PARALLEL_MIN_WORK = {"winding": 120_000}
+
+
+def worker_count(items: int, work: int, kernel: str) -> int:
+    """Span count for a kernel with ``items`` independent outputs
+    (python/igl/_lib.py:53-61)."""
+    if items < 2 or work < PARALLEL_MIN_WORK[kernel]:
+        return 1
+    try:
+        available = len(os.sched_getaffinity(0))
+    except AttributeError:
+        available = os.cpu_count() or 1
+    return max(1, min(items, available, MAX_WORKERS))
+
+
+def spans(total: int, parts: int) -> list[tuple[int, int]]:
+    """``total`` items in ``parts`` contiguous near-equal spans
+    (python/igl/_lib.py:64-69)."""
+    step = -(-total // parts)
+    return [(lo, min(lo + step, total)) for lo in range(0, total, step) if lo < total]
+
+
+def fan_out(function, arguments, total: int, workers: int) -> int:
+    """Call ``function(*arguments, lo, hi, span=index)`` once per span
+    (python/igl/_lib.py:72-95); returns the number of spans used."""
+    if workers == 1:
+        function(*arguments, 0, total, span=0)
+        return 1
+    parts = spans(total, workers)
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        list(pool.map(lambda item: function(*arguments, item[1][0], item[1][1], span=item[0]),
+                      enumerate(parts)))
+    return len(parts)
+
+
+_handle: ctypes.CDLL | None = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the library and bind every symbol of ``_SIGNATURES``."""
+    global _handle
+    if _handle is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(
+                f"winding-number library not found at {LIB}; build it with "
+                "`python mojo-libigl_amd/build.py` (or set WN_LIB)")
+        handle = ctypes.CDLL(LIB)
+        for name, (args, result) in _SIGNATURES.items():
+            function = getattr(handle, name)
+            function.argtypes = args
+            function.restype = result
+        _handle = handle
+    return _handle
+
+
+def last_error() -> str:
+    message = lib().wn_last_error()
+    return message.decode(errors="replace") if message else ""
+
+
+def addr(array: np.ndarray) -> int:
+    return array.ctypes.data

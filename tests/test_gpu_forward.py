@@ -1,0 +1,222 @@
+"""GPU parity of the forward kernels against the FP64 oracle (restatement of
+the reference CPU path, src/capi.mojo:205-294).
+
+Bars: FP64 plain within 1e-10 of the oracle (the contract of the reference's
+FP64 mode); FP32-compensated within 1e-6 of the oracle on every query of the
+sizes run here. All calls go through the C ABI (include/wn.h).
+"""
+
+import numpy as np
+import pytest
+
+import igl
+import oracle
+import wn_workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32C_TOL = 1e-6
+
+
+def host_call(tri, Q, precision, qpl=0, splits=0, sort=-1, kappa=-1.0):
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(-1, 3)
+    out = np.full(len(Q), np.nan)
+    h = igl.lib()
+    status = h.wn_winding_gpu_ex(tri.ctypes.data, Q.ctypes.data, out.ctypes.data,
+                                 tri.shape[1], len(Q), precision, qpl, splits, sort, kappa)
+    assert status == 0, igl._lib.last_error()
+    return out
+
+
+@pytest.fixture(scope="module")
+def c1():
+    V, F, Q = W.config1()
+    tri = oracle.pack_soa(V, F)
+    return V, F, Q, tri, oracle.winding_soa(tri, Q)
+
+
+@pytest.fixture(scope="module")
+def c2():
+    V, F, Q = W.config2(1024)
+    tri = oracle.pack_soa(V, F)
+    return V, F, Q, tri, oracle.winding_soa(tri, Q)
+
+
+def test_tetrahedron_gpu_matches_cpu(require_gpu):
+    """Reference tests/test_parity.py:108-114: GPU == CPU within 1e-14."""
+    V, F = W.tetrahedron()
+    Q = np.array([[0.1, 0.1, 0.1], [2.0, 2.0, 2.0], [-1.0, 0.2, 0.2]])
+    cpu = igl.winding_number(V, F, Q)
+    gpu = igl.winding_number(V, F, Q, device="gpu")
+    assert np.allclose(gpu, cpu, atol=1e-14, rtol=0)
+    assert gpu[0] == pytest.approx(1.0, abs=1e-14)
+    assert igl.winding_number(V, F, Q[0], device="gpu") == pytest.approx(1.0, abs=1e-14)
+
+
+@pytest.mark.parametrize("qpl,splits", [(1, 1), (2, 1), (1, 7), (2, 20), (0, 0)])
+def test_fp64_config1(require_gpu, c1, qpl, splits):
+    V, F, Q, tri, ref = c1
+    out = host_call(tri, Q, 0, qpl, splits)
+    print(f"fp64 c1 qpl={qpl} splits={splits} max|dw|={np.abs(out - ref).max():.3e}", flush=True)
+    assert np.abs(out - ref).max() <= FP64_TOL
+
+
+def test_fp64_config2_1024(require_gpu, c2):
+    V, F, Q, tri, ref = c2
+    out = host_call(tri, Q, 0)
+    assert np.abs(out - ref).max() <= FP64_TOL
+    # closed torus: integer winding numbers
+    assert np.abs(out - np.round(out)).max() <= FP64_TOL
+
+
+@pytest.mark.parametrize("qpl,splits,sort", [(1, 1, 0), (2, 1, 0), (4, 1, 0), (2, 9, 1), (0, 0, -1)])
+def test_fp32c_config1(require_gpu, c1, qpl, splits, sort):
+    V, F, Q, tri, ref = c1
+    out = host_call(tri, Q, 1, qpl, splits, sort)
+    err = np.abs(out - ref).max()
+    print(f"fp32c c1 qpl={qpl} splits={splits} sort={sort} max|dw|={err:.3e}", flush=True)
+    assert err <= FP32C_TOL
+
+
+def test_fp32c_config2_1024(require_gpu, c2):
+    V, F, Q, tri, ref = c2
+    for sort in (0, 1):
+        out = host_call(tri, Q, 1, 0, 0, sort)
+        err = np.abs(out - ref).max()
+        print(f"fp32c c2@1024 sort={sort} max|dw|={err:.3e} mean={np.abs(out - ref).mean():.3e}", flush=True)
+        assert err <= FP32C_TOL
+
+
+def test_fp32c_all_pairs_fp64_fallback(require_gpu, c1):
+    """kappa = 1e30 sends every pair through the FP64 re-evaluation; the
+    result is then the FP64 formula on the block-rounded mesh."""
+    V, F, Q, tri, ref = c1
+    out = host_call(tri, Q, 1, 2, 1, 0, 1e30)
+    assert np.abs(out - ref).max() <= FP32C_TOL
+
+
+def test_fp32c_pure_fp32_is_worse_near_surface(require_gpu):
+    """kappa = 0 disables the FP64 re-evaluation. On queries 1e-6 above the
+    faces of a grid the pure-FP32 result must be no better than the default."""
+    V, F = W.grid(12)
+    rng = np.random.default_rng(5)
+    f = rng.integers(0, len(F), 512)
+    bary = rng.dirichlet(np.ones(3), 512)
+    T = V[F[f]]
+    p = np.einsum("nk,nkd->nd", bary, T)
+    n = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    Q = p + 1e-6 * n
+    tri = oracle.pack_soa(V, F)
+    ref = oracle.winding_soa(tri, Q)
+    err_default = np.abs(host_call(tri, Q, 1, 1, 1, 0) - ref).max()
+    err_fp32 = np.abs(host_call(tri, Q, 1, 1, 1, 0, 0.0) - ref).max()
+    print(f"near-surface: default {err_default:.3e}, pure fp32 {err_fp32:.3e}", flush=True)
+    assert err_default <= 1e-4
+    assert err_fp32 >= err_default
+
+
+@pytest.mark.parametrize("nf_take,nq", [(1, 1), (5, 7), (257, 255), (300, 513), (5120, 1)])
+def test_ragged_sizes(require_gpu, c1, nf_take, nq):
+    V, F, Q, tri, ref = c1
+    Fs = F[:nf_take]
+    Qs = Q[:nq]
+    t = oracle.pack_soa(V, Fs)
+    r = oracle.winding_soa(t, Qs)
+    for precision, tol in ((0, FP64_TOL), (1, FP32C_TOL)):
+        for splits in (1, 3):
+            out = host_call(t, Qs, precision, 1, splits, 0)
+            assert np.abs(out - r).max() <= tol, (precision, splits)
+
+
+def test_empty_mesh_and_queries(require_gpu):
+    tri = np.zeros((9, 0))
+    Q = np.random.default_rng(0).normal(size=(10, 3))
+    for precision in (0, 1):
+        assert np.array_equal(host_call(tri, Q, precision), np.zeros(10))
+    V, F = W.tetrahedron()
+    assert igl.winding_number(V, F, np.zeros((0, 3)), device="gpu").shape == (0,)
+
+
+def test_deterministic_run_to_run(require_gpu, c2):
+    V, F, Q, tri, ref = c2
+    for precision in (0, 1):
+        a = host_call(tri, Q, precision, 0, 16)
+        b = host_call(tri, Q, precision, 0, 16)
+        assert np.array_equal(a, b)
+
+
+def test_device_api_matches_host_api(require_gpu, c1):
+    """wn_pack_soa + wn_forward on device tensors (the path torch callers and
+    the bench use) equals the host-buffer path."""
+    import torch
+
+    V, F, Q, tri, ref = c1
+    h = igl.lib()
+    dev = torch.device("cuda")
+    Vd = torch.tensor(V, device=dev)
+    Fd = torch.tensor(F.astype(np.int32), device=dev)
+    Qd = torch.tensor(Q, device=dev)
+    trid = torch.empty((9, len(F)), dtype=torch.float64, device=dev)
+    bad = torch.full((1,), 2**31 - 1, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    assert h.wn_pack_soa(Vd.data_ptr(), Fd.data_ptr(), len(V), len(F), 0, 1, trid.data_ptr(),
+                         bad.data_ptr(), stream) == 0
+    torch.cuda.synchronize()
+    assert int(bad.item()) == 2**31 - 1
+    assert np.array_equal(trid.cpu().numpy(), tri)
+    for precision in (0, 1):
+        nbytes = h.wn_forward_workspace_bytes(len(F), len(Q), precision, 0, 0, -1)
+        assert nbytes >= 0
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+        out = torch.empty(len(Q), dtype=torch.float64, device=dev)
+        st = h.wn_forward(trid.data_ptr(), Qd.data_ptr(), out.data_ptr(), len(F), len(Q),
+                          precision, 0, 0, -1, -1.0, ws.data_ptr(), nbytes, stream)
+        assert st == 0, igl._lib.last_error()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), host_call(tri, Q, precision))
+
+
+def test_pack_soa_reports_bad_index(require_gpu):
+    import torch
+
+    h = igl.lib()
+    dev = torch.device("cuda")
+    V = torch.zeros((4, 3), dtype=torch.float32, device=dev)
+    F = torch.tensor([[0, 1, 2], [0, 1, 9], [0, 7, 1]], dtype=torch.int64, device=dev)
+    tri = torch.empty((9, 3), dtype=torch.float64, device=dev)
+    bad = torch.full((1,), 2**31 - 1, dtype=torch.int32, device=dev)
+    assert h.wn_pack_soa(V.data_ptr(), F.data_ptr(), 4, 3, 1, 0, tri.data_ptr(), bad.data_ptr(),
+                         torch.cuda.current_stream().cuda_stream) == 0
+    torch.cuda.synchronize()
+    assert int(bad.item()) == 1
+
+
+def test_injected_fault_raises(require_gpu, monkeypatch):
+    """Reference tests/test_parity.py:117-128: a failed GPU kernel raises
+    RuntimeError; here the failure is injected in the host code (WN_FAULT)."""
+    V, F = W.tetrahedron()
+    monkeypatch.setenv("WN_FAULT", "launch")
+    with pytest.raises(RuntimeError, match="GPU"):
+        igl.winding_number(V, F, [[0.1, 0.1, 0.1]], device="gpu")
+    monkeypatch.delenv("WN_FAULT")
+    assert igl.winding_number(V, F, [[0.1, 0.1, 0.1]], device="gpu")[0] == pytest.approx(1.0)
+
+
+def test_config2_full_size_properties(require_gpu):
+    """28,320 faces x 1,048,576 queries: FP32-compensated against FP64 plain
+    on the GPU (size-independent properties: agreement <= 1e-6 and, on the
+    closed torus, integer values), plus the oracle on a strided subsample."""
+    V, F, Q = W.config2(1_048_576)
+    tri = oracle.pack_soa(V, F)
+    w64 = host_call(tri, Q, 0)
+    w32 = host_call(tri, Q, 1)
+    d = np.abs(w32 - w64)
+    print(f"c2@1M fp32c vs fp64: max {d.max():.3e} mean {d.mean():.3e}", flush=True)
+    assert d.max() <= FP32C_TOL
+    assert np.abs(w64 - np.round(w64)).max() <= 1e-9
+    sub = np.arange(0, len(Q), 4099)
+    ref = oracle.winding_soa(tri, Q[sub])
+    assert np.abs(w64[sub] - ref).max() <= FP64_TOL
