@@ -1,0 +1,200 @@
+"""The C-ABI boundary and the Python drop-in, on the CPU.
+
+Mirrors the reference's winding tests (tests/test_parity.py:74-128, 270-291,
+294-340, 402-427) with the upstream-libigl comparisons replaced by the oracle.
+No GPU compute is called here.
+"""
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+import igl
+import oracle
+import wn_workloads as W
+from igl import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wn.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(\w+)\s*\(([^)]*)\)\s*;", text, flags=re.M):
+        name, args = m.group(1), m.group(2).strip()
+        out[name] = 0 if args in ("", "void") else len(args.split(","))
+    return out
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+
+    handle = ctypes.CDLL(_lib.LIB)
+    declared = header_functions()
+    assert {"mli_winding", "mli_winding_gpu", "wn_forward"} <= set(declared)
+    for name in declared:
+        assert hasattr(handle, name), name
+
+
+def test_signatures_match_header_arity():
+    """Reference tests/test_parity.py:294-340: every declared signature takes
+    exactly the arguments callers pass."""
+    declared = header_functions()
+    assert set(_lib._SIGNATURES) == set(declared)
+    for name, (args, _) in _lib._SIGNATURES.items():
+        assert len(args) == declared[name], name
+    # the two reference entry points keep the reference's arity
+    assert len(_lib._SIGNATURES["mli_winding"][0]) == 3 + 4
+    assert len(_lib._SIGNATURES["mli_winding_gpu"][0]) == 3 + 2
+
+
+def test_cpu_path_is_bitwise_the_oracle():
+    V, F, Q = W.config1()
+    assert np.array_equal(igl.winding_number(V, F, Q), oracle.winding(V, F, Q))
+
+
+def test_tetrahedron_published_invariant():
+    """Reference tests/test_parity.py:74-85."""
+    V, F = W.tetrahedron()
+    Q = np.array([[0.1, 0.1, 0.1], [2.0, 2.0, 2.0], [-1.0, 0.2, 0.2]])
+    w = igl.winding_number(V, F, Q)
+    assert w[0] == pytest.approx(1.0, abs=1e-14)
+    assert np.allclose(w, oracle.winding(V, F, Q), atol=1e-14)
+    assert igl.winding_number(V, F, Q[0]) == pytest.approx(1.0, abs=1e-14)
+    assert isinstance(igl.winding_number(V, F, Q[0]), float)
+    assert igl.winding_number_for_point(V, F, Q[0]) == pytest.approx(1.0, abs=1e-14)
+
+
+def test_simd_tail_and_parallel_threshold():
+    """Reference tests/test_parity.py:88-105 on the committed mpmath values."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "winding_golden.npz"))
+    w = igl.winding_number(g["tail_V"], g["tail_F"], g["tail_Q"])
+    assert np.abs(w - g["tail_w"]).max() <= 1e-14
+    w = igl.winding_number(g["par_V"], g["par_F"], g["par_Q"])
+    assert len(w) == 1619
+    assert np.abs(w[g["par_pick"]] - g["par_w"]).max() <= 1e-12
+
+
+def _force(monkeypatch, workers):
+    monkeypatch.setattr(igl, "worker_count", lambda items, work, kernel: max(1, min(items, workers)))
+
+
+def test_serial_and_parallel_output_is_identical(monkeypatch):
+    """Reference tests/test_parity.py:418-427 (grid(20) = 722 faces)."""
+    V, F = W.grid(20)
+    rng = np.random.default_rng(77)
+    Q = rng.uniform([0.0, 0.0, -2.0], [19.0, 19.0, 2.0], size=(512, 3))
+    _force(monkeypatch, 1)
+    serial = igl.winding_number(V, F, Q)
+    for workers in (1, 2, 4, 8):
+        _force(monkeypatch, workers)
+        assert np.array_equal(igl.winding_number(V, F, Q), serial)
+
+
+def test_worker_policy():
+    assert _lib.worker_count(1, 10**9, "winding") == 1
+    assert _lib.worker_count(1000, 1000, "winding") == 1
+    assert 1 <= _lib.worker_count(1000, 10**7, "winding") <= _lib.MAX_WORKERS
+    assert _lib.spans(10, 4) == [(0, 3), (3, 6), (6, 9), (9, 10)]
+
+
+def test_input_contracts():
+    """Reference tests/test_parity.py:270-291 (winding rows) and the
+    validation helpers of python/igl/__init__.py:72-124."""
+    V, F = W.grid(3)
+    with pytest.raises(ValueError):
+        igl.winding_number(V, F, [[np.nan, 0.0, 0.0]])
+    with pytest.raises(ValueError):
+        igl.winding_number(V, F, [[np.inf, 0.0, 0.0]])
+    with pytest.raises(IndexError):
+        igl.winding_number(V, np.array([[0, 1, len(V)]]), [[0.0, 0.0, 0.0]])
+    with pytest.raises(IndexError):
+        igl.winding_number(V, np.array([[0, -1, 2]]), [[0.0, 0.0, 0.0]])
+    with pytest.raises(TypeError):
+        igl.winding_number(V, F.astype(np.float64), [[0.0, 0.0, 0.0]])
+    with pytest.raises(TypeError):
+        igl.winding_number(V.astype(complex), F, [[0.0, 0.0, 0.0]])
+    with pytest.raises(ValueError):
+        igl.winding_number(V[:, :2], F, [[0.0, 0.0, 0.0]])
+    with pytest.raises(ValueError):
+        igl.winding_number(V, F, [0.0, 0.0])
+    with pytest.raises(ValueError):
+        igl.winding_number(V, F, np.zeros((2, 2)))
+    with pytest.raises(ValueError):
+        igl.winding_number(V, F, [[0.0, 0.0, 0.0]], device="tpu")
+    with pytest.raises(ValueError):
+        igl.winding_number(V, F, [[0.0, 0.0, 0.0]], precision="fp16")
+    with pytest.raises(ValueError):
+        igl.winding_number(V, F, [[0.0, 0.0, 0.0]], precision="fp32c")  # GPU-only mode
+    with pytest.raises(OverflowError):
+        igl.winding_number(V, np.array([[0, 1, 2**63]], dtype=np.uint64), [[0.0, 0.0, 0.0]])
+    with pytest.raises(ValueError):
+        igl.winding_number(np.array([[2**60, 0, 0], [0, 1, 0], [0, 0, 1]]), [[0, 1, 2]], [[0, 0, 0]])
+    # accepted: integer coordinates, int32 faces, empty queries, empty faces
+    assert igl.winding_number(V.astype(np.int64), F.astype(np.int32), [[0.5, 0.5, 1.0]]).shape == (1,)
+    assert igl.winding_number(V, F, np.zeros((0, 3))).shape == (0,)
+    assert np.array_equal(igl.winding_number(V, np.zeros((0, 3), dtype=np.int64), [[0.0, 0, 0]]), [0.0])
+
+
+def test_gpu_request_without_device_raises(monkeypatch):
+    """The reference silently used the CPU here (python/igl/__init__.py:251,
+    264, tests/test_parity.py:108-114); this build refuses instead."""
+    V, F = W.tetrahedron()
+    monkeypatch.setattr(igl, "_gpu_memory_available", lambda: False)
+    with pytest.raises(RuntimeError, match="GPU"):
+        igl.winding_number(V, F, [[0.1, 0.1, 0.1]], device="gpu")
+
+
+def test_gpu_kernel_failure_is_not_swallowed(monkeypatch):
+    """Reference tests/test_parity.py:117-128."""
+    V, F = W.tetrahedron()
+
+    class FailedGpu:
+        @staticmethod
+        def mli_winding_gpu(*args):
+            return 0
+
+        @staticmethod
+        def wn_winding_gpu_ex(*args):
+            return -3
+
+    monkeypatch.setattr(igl, "_gpu_memory_available", lambda: True)
+    monkeypatch.setattr(igl, "lib", lambda: FailedGpu())
+    with pytest.raises(RuntimeError, match="GPU"):
+        igl.winding_number(V, F, [[0.1, 0.1, 0.1]], device="gpu")
+    with pytest.raises(RuntimeError, match="GPU"):
+        igl.winding_number(V, F, [[0.1, 0.1, 0.1]], device="gpu", precision="fp32c")
+
+
+def test_no_device_here_reports_through_the_abi():
+    h = igl.lib()
+    if h.wn_device_count() > 0:
+        pytest.skip("a HIP device is visible; this checks the no-device path")
+    V, F = W.tetrahedron()
+    tri = igl.pack_faces(V, F)
+    Q = np.zeros((1, 3))
+    out = np.zeros(1)
+    assert h.mli_winding_gpu(tri.ctypes.data, Q.ctypes.data, out.ctypes.data, 4, 1) == 0
+    assert "no HIP device" in _lib.last_error()
+    assert h.wn_device_free_bytes() < 0
+    with pytest.raises(RuntimeError, match="GPU"):
+        igl.winding_number(V, F, Q, device="gpu")
+
+
+def test_config_validation_through_the_abi():
+    h = igl.lib()
+    cfg = np.zeros(4, dtype=np.int64)
+    assert h.wn_resolve_config(1000, 10**6, 1, 0, 0, -1, -1.0, cfg.ctypes.data) == 0
+    assert cfg[0] in (1, 2, 4) and cfg[1] >= 1 and cfg[2] == 1 and cfg[3] == 1
+    assert h.wn_resolve_config(28320, 1024, 0, 0, 0, -1, -1.0, cfg.ctypes.data) == 0
+    assert cfg[0] == 1 and cfg[1] > 1  # split-K for few queries
+    assert h.wn_resolve_config(10, 10, 7, 0, 0, 0, 1.0, cfg.ctypes.data) == -1
+    assert h.wn_resolve_config(10, 10, 0, 3, 0, 0, 1.0, cfg.ctypes.data) == -1
+    assert h.wn_resolve_config(10, 10, 0, 1, 70000, 0, 1.0, cfg.ctypes.data) == -1
+    assert h.wn_resolve_config(10, 2**31, 0, 1, 1, 0, 1.0, cfg.ctypes.data) == -1
+    assert h.wn_forward(0, 0, 0, 10, 10, 9, 1, 1, 0, 1.0, 0, 0, 0) == -1
+    assert "precision" in _lib.last_error()
