@@ -81,6 +81,30 @@ def fan_out(function, arguments, total: int, workers: int) -> int:
 _handle: ctypes.CDLL | None = None
 
 
+def _preload_hip_runtime() -> None:
+    """Make the library share PyTorch's HIP runtime.
+
+    The library needs ``libamdhip64.so.7``; PyTorch ships its own copy of that
+    runtime under ``torch/lib``. Two runtimes in one process cannot both own
+    the GPU, so the one PyTorch will use is loaded first (RTLD_GLOBAL): the
+    library's dependency then resolves to it, and a later ``import torch``
+    maps the same file instead of a second runtime. ``WN_HIP_RUNTIME=system``
+    keeps the system runtime (processes that never import torch).
+    """
+    if os.environ.get("WN_HIP_RUNTIME", "").lower() == "system":
+        return
+    import importlib.util
+
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for base in spec.submodule_search_locations:
+        path = os.path.join(base, "lib", "libamdhip64.so")
+        if os.path.exists(path):
+            ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+            return
+
+
 def lib() -> ctypes.CDLL:
     """Load the library and bind every symbol of ``_SIGNATURES``."""
     global _handle
@@ -89,6 +113,7 @@ def lib() -> ctypes.CDLL:
             raise RuntimeError(
                 f"winding-number library not found at {LIB}; build it with "
                 "`python mojo-libigl_amd/build.py` (or set WN_LIB)")
+        _preload_hip_runtime()
         handle = ctypes.CDLL(LIB)
         for name, (args, result) in _SIGNATURES.items():
             function = getattr(handle, name)

@@ -120,12 +120,19 @@ def test_fp32c_pure_fp32_is_worse_near_surface(require_gpu):
 
 @pytest.mark.parametrize("nf_take,nq", [(1, 1), (5, 7), (257, 255), (300, 513), (5120, 1)])
 def test_ragged_sizes(require_gpu, c1, nf_take, nq):
+    """Odd face and query counts (tails of the LDS tile and of the query
+    block). Prefixes of the face list are OPEN patches: there w is not
+    locally constant and the FP32 rounding of the block-local coordinates
+    (~6e-8 * block extent, here the whole [-1.5, 1.5]^3 box) moves w by
+    |grad w| times that, ~1e-6 within 1e-2 of a boundary edge; the
+    FP32-compensated bar for open patches is therefore 1e-5."""
     V, F, Q, tri, ref = c1
     Fs = F[:nf_take]
     Qs = Q[:nq]
     t = oracle.pack_soa(V, Fs)
     r = oracle.winding_soa(t, Qs)
-    for precision, tol in ((0, FP64_TOL), (1, FP32C_TOL)):
+    tol32 = FP32C_TOL if nf_take == len(F) else 1e-5
+    for precision, tol in ((0, FP64_TOL), (1, tol32)):
         for splits in (1, 3):
             out = host_call(t, Qs, precision, 1, splits, 0)
             assert np.abs(out - r).max() <= tol, (precision, splits)
