@@ -99,6 +99,25 @@ int64_t wn_pack_soa(const void *V, const void *F, int64_t nv, int64_t nf,
                     int64_t vdtype, int64_t fdtype, double *tri,
                     int32_t *bad_index, void *stream);
 
+/* ---- Adjoint ------------------------------------------------------------ */
+
+/* No reference counterpart (the reference has no derivative of
+ * src/capi.mojo:218-235,286). Given grad_w = dL/dw (nq), writes
+ * grad_q = dL/dO (nq x 3) and grad_tri = dL/d(face corners) (9 x nf, the tri
+ * layout). FP64 (precision must be 0). (device) all buffers; asynchronous on
+ * `stream`; allocates nothing. grad_tri is accumulated with FP64 atomics, so
+ * its rounding depends on scheduling; grad_q is exact-order when splits == 1.
+ * qpl 0 = default (1), splits 0 = default. */
+int64_t wn_backward(const double *tri, const double *q, const double *grad_w,
+                    double *grad_q, double *grad_tri, int64_t nf, int64_t nq,
+                    int64_t precision, int64_t qpl, int64_t splits, void *stream);
+
+/* grad_V (nv x 3, overwritten) = per-vertex sum of the corner gradients of
+ * grad_tri through F (nf x 3; fdtype 0 = int64, 1 = int32). (device);
+ * asynchronous on `stream`. */
+int64_t wn_scatter_corners(const double *grad_tri, const void *F, int64_t fdtype,
+                           int64_t nf, int64_t nv, double *grad_V, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

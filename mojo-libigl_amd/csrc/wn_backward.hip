@@ -1,0 +1,234 @@
+// Adjoint of the winding number (no counterpart in the reference, which has
+// no derivative; the forward it differentiates is src/capi.mojo:218-235,286).
+//
+// For one pair, theta = atan2(det, den) with a = A - O, b = B - O, c = C - O:
+//   d theta = (den d(det) - det d(den)) / (det^2 + den^2)
+//   d det / da = b x c,  d det / db = c x a,  d det / dc = a x b
+//   d den / da = a^ (lb lc + b.c) + b lc + c lb      (a^ = a / la)
+//   d den / db = b^ (lc la + c.a) + c la + a lc
+//   d den / dc = c^ (la lb + a.b) + a lb + b la
+// and d theta / dO = -(d/da + d/db + d/dc). With w = sum theta / (2 pi) and
+// upstream gradient g_q = dL/dw_q:
+//   dL/dO_q = g_q / (2 pi) sum_f d theta_qf / dO
+//   dL/dA_f = sum_q g_q / (2 pi) d theta_qf / da      (likewise B, C)
+// Pairs with det^2 + den^2 == 0 or a zero corner distance (query on an edge
+// line or at a corner, where theta is not differentiable) contribute 0.
+//
+// FP64 throughout. Face tiles are restaged from the 9 x nf SoA per block (no
+// per-pair quantity is stored). dL/dO stays in registers (atomics only when
+// the face range is split); the face gradient is summed over the wave (DPP),
+// over the block (LDS ds_add_f64), then added to grad_tri with one FP64
+// atomic per face-corner component per block -- so grad_tri and grad_V are
+// sums in an unspecified order (not bitwise reproducible), grad_q with
+// splits == 1 is.
+
+#include "wn_device.h"
+#include "wn_internal.h"
+
+namespace wn {
+
+namespace {
+
+struct PairGrad {
+  double ax, ay, az, bx, by, bz, cx, cy, cz;  // d theta / d(a, b, c)
+};
+
+__device__ __forceinline__ PairGrad pair_grad_f64(double ax, double ay, double az, double bx,
+                                                  double by, double bz, double cx, double cy,
+                                                  double cz) {
+  PairGrad g;
+  const double la = sqrt(ax * ax + ay * ay + az * az);
+  const double lb = sqrt(bx * bx + by * by + bz * bz);
+  const double lc = sqrt(cx * cx + cy * cy + cz * cz);
+  // cross products
+  const double bcx = by * cz - bz * cy, bcy = bz * cx - bx * cz, bcz = bx * cy - by * cx;
+  const double cax = cy * az - cz * ay, cay = cz * ax - cx * az, caz = cx * ay - cy * ax;
+  const double abx = ay * bz - az * by, aby = az * bx - ax * bz, abz = ax * by - ay * bx;
+  const double det = ax * bcx + ay * bcy + az * bcz;
+  const double ab = ax * bx + ay * by + az * bz;
+  const double bc = bx * cx + by * cy + bz * cz;
+  const double ca = cx * ax + cy * ay + cz * az;
+  const double den = la * lb * lc + ab * lc + bc * la + ca * lb;
+  const double r2 = det * det + den * den;
+  if (!(r2 > 0.0) || !(la > 0.0) || !(lb > 0.0) || !(lc > 0.0)) {
+    g.ax = g.ay = g.az = g.bx = g.by = g.bz = g.cx = g.cy = g.cz = 0.0;
+    return g;
+  }
+  const double inv = 1.0 / r2;
+  const double p = den * inv;   // weight of d det
+  const double s = -det * inv;  // weight of d den
+  const double ka = s * (lb * lc + bc) / la;
+  const double kb = s * (lc * la + ca) / lb;
+  const double kc = s * (la * lb + ab) / lc;
+  g.ax = p * bcx + ka * ax + s * (bx * lc + cx * lb);
+  g.ay = p * bcy + ka * ay + s * (by * lc + cy * lb);
+  g.az = p * bcz + ka * az + s * (bz * lc + cz * lb);
+  g.bx = p * cax + kb * bx + s * (cx * la + ax * lc);
+  g.by = p * cay + kb * by + s * (cy * la + ay * lc);
+  g.bz = p * caz + kb * bz + s * (cz * la + az * lc);
+  g.cx = p * abx + kc * cx + s * (ax * lb + bx * la);
+  g.cy = p * aby + kc * cy + s * (ay * lb + by * la);
+  g.cz = p * abz + kc * cz + s * (az * lb + bz * la);
+  return g;
+}
+
+template <int QPL>
+__global__ __launch_bounds__(kBlockThreads) void bwd_f64_kernel(
+    const double* __restrict__ tri, const double* __restrict__ q,
+    const double* __restrict__ gw, double* __restrict__ gq, double* __restrict__ gtri,
+    int64_t nf, int64_t nq, int64_t faces_per_split, int splits) {
+  __shared__ double2 sface[kFaceTile][5];
+  __shared__ double sacc[kFaceTile][9];
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
+
+  double ox[QPL], oy[QPL], oz[QPL], g[QPL], dx[QPL], dy[QPL], dz[QPL];
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    const bool ok = qi < nq;
+    ox[k] = ok ? q[3 * qi] : 0.0;
+    oy[k] = ok ? q[3 * qi + 1] : 0.0;
+    oz[k] = ok ? q[3 * qi + 2] : 0.0;
+    g[k] = ok ? gw[qi] * kInv2Pi : 0.0;
+    dx[k] = dy[k] = dz[k] = 0.0;
+  }
+
+  const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
+  const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
+  for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
+    const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t f = t0 + tid;
+      sface[tid][0] = make_double2(tri[f], tri[nf + f]);
+      sface[tid][1] = make_double2(tri[2 * nf + f], tri[3 * nf + f]);
+      sface[tid][2] = make_double2(tri[4 * nf + f], tri[5 * nf + f]);
+      sface[tid][3] = make_double2(tri[6 * nf + f], tri[7 * nf + f]);
+      sface[tid][4] = make_double2(tri[8 * nf + f], 0.0);
+    }
+    for (int i = tid; i < kFaceTile * 9; i += kBlockThreads) (&sacc[0][0])[i] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const double2 p0 = sface[j][0], p1 = sface[j][1], p2 = sface[j][2], p3 = sface[j][3],
+                    p4 = sface[j][4];
+      double fa[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) fa[i] = 0.0;
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        const PairGrad pg = pair_grad_f64(p0.x - ox[k], p0.y - oy[k], p1.x - oz[k],
+                                          p1.y - ox[k], p2.x - oy[k], p2.y - oz[k],
+                                          p3.x - ox[k], p3.y - oy[k], p4.x - oz[k]);
+        const double gk = g[k];
+        dx[k] -= gk * (pg.ax + pg.bx + pg.cx);
+        dy[k] -= gk * (pg.ay + pg.by + pg.cy);
+        dz[k] -= gk * (pg.az + pg.bz + pg.cz);
+        fa[0] += gk * pg.ax; fa[1] += gk * pg.ay; fa[2] += gk * pg.az;
+        fa[3] += gk * pg.bx; fa[4] += gk * pg.by; fa[5] += gk * pg.bz;
+        fa[6] += gk * pg.cx; fa[7] += gk * pg.cy; fa[8] += gk * pg.cz;
+      }
+      double mine = 0.0;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const double s = wave_sum_f64(fa[i]);
+        const double s0 = __longlong_as_double(
+            ((long long)(uint32_t)__builtin_amdgcn_readfirstlane(
+                 (int)(uint32_t)((uint64_t)__double_as_longlong(s) >> 32))
+             << 32) |
+            (long long)(uint32_t)__builtin_amdgcn_readfirstlane(
+                (int)(uint32_t)__double_as_longlong(s)));
+        if (lane == i) mine = s0;
+      }
+      if (lane < 9) atomicAdd(&sacc[j][lane], mine);
+    }
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t f = t0 + tid;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) unsafeAtomicAdd(&gtri[i * nf + f], sacc[tid][i]);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    if (qi < nq) {
+      if (splits == 1) {
+        gq[3 * qi] = dx[k];
+        gq[3 * qi + 1] = dy[k];
+        gq[3 * qi + 2] = dz[k];
+      } else {
+        unsafeAtomicAdd(&gq[3 * qi], dx[k]);
+        unsafeAtomicAdd(&gq[3 * qi + 1], dy[k]);
+        unsafeAtomicAdd(&gq[3 * qi + 2], dz[k]);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void zero_kernel(double* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+template <typename FT>
+__global__ __launch_bounds__(kBlockThreads) void scatter_corners_kernel(
+    const double* __restrict__ gtri, const FT* __restrict__ F, int64_t nf, int64_t nv,
+    double* __restrict__ gv) {
+  const int64_t f = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (f >= nf) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int64_t v = (int64_t)F[3 * f + c];
+    if (v < 0 || v >= nv) continue;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) unsafeAtomicAdd(&gv[3 * v + d], gtri[(3 * c + d) * nf + f]);
+  }
+}
+
+inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlockThreads - 1) / kBlockThreads)); }
+
+}  // namespace
+
+const char* bwd_check(int64_t nf, int64_t nq, int qpl, int splits) {
+  if (nf < 0 || nq < 0) return "negative size";
+  if (qpl != 1 && qpl != 2) return "backward queries_per_lane must be 1 or 2";
+  if (splits < 1 || splits > 65535) return "splits must be in [1, 65535]";
+  if (nq > (int64_t)0x7fffffff) return "more than 2^31-1 queries per call";
+  return nullptr;
+}
+
+hipError_t bwd_launch(const double* tri, const double* q, const double* gw, double* gq,
+                      double* gtri, int64_t nf, int64_t nq, int qpl, int splits,
+                      hipStream_t stream) {
+  if (nq > 0) zero_kernel<<<grid1(3 * nq), kBlockThreads, 0, stream>>>(gq, 3 * nq);
+  if (nf > 0) zero_kernel<<<grid1(9 * nf), kBlockThreads, 0, stream>>>(gtri, 9 * nf);
+  if (nq == 0 || nf == 0) return hipGetLastError();
+  int64_t per = (nf + splits - 1) / splits;
+  per = (per + kFaceTile - 1) / kFaceTile * kFaceTile;
+  const int64_t qblock = (int64_t)kBlockThreads * qpl;
+  const dim3 grid((unsigned)((nq + qblock - 1) / qblock), (unsigned)splits);
+  if (qpl == 1)
+    bwd_f64_kernel<1><<<grid, kBlockThreads, 0, stream>>>(tri, q, gw, gq, gtri, nf, nq, per, splits);
+  else
+    bwd_f64_kernel<2><<<grid, kBlockThreads, 0, stream>>>(tri, q, gw, gq, gtri, nf, nq, per, splits);
+  return hipGetLastError();
+}
+
+hipError_t scatter_corners_launch(const double* gtri, const void* F, int fdtype, int64_t nf,
+                                  int64_t nv, double* gv, hipStream_t stream) {
+  if (nv > 0) zero_kernel<<<grid1(3 * nv), kBlockThreads, 0, stream>>>(gv, 3 * nv);
+  if (nf > 0) {
+    if (fdtype == 0)
+      scatter_corners_kernel<int64_t><<<grid1(nf), kBlockThreads, 0, stream>>>(
+          gtri, (const int64_t*)F, nf, nv, gv);
+    else
+      scatter_corners_kernel<int32_t><<<grid1(nf), kBlockThreads, 0, stream>>>(
+          gtri, (const int32_t*)F, nf, nv, gv);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace wn

@@ -254,4 +254,39 @@ int64_t wn_pack_soa(const void* V, const void* F, int64_t nv, int64_t nf, int64_
   return kOk;
 }
 
+int64_t wn_backward(const double* tri, const double* q, const double* grad_w, double* grad_q,
+                    double* grad_tri, int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
+                    int64_t splits, void* stream) {
+  if (precision != wn::kFP64) return fail(kErrArg, "backward supports precision 0 (FP64) only");
+  const int lanes = (int)(qpl <= 0 ? 1 : qpl);
+  int s = (int)splits;
+  if (s <= 0) {
+    const int64_t qblocks = (nq + 256 * lanes - 1) / (256 * lanes);
+    int64_t t = (2048 + qblocks - 1) / (qblocks > 0 ? qblocks : 1);
+    const int64_t max_s = nf / (2 * wn::kFaceTile);
+    if (t > max_s) t = max_s;
+    if (t < 1) t = 1;
+    if (t > 4096) t = 4096;
+    s = (int)t;
+  }
+  if (const char* why = wn::bwd_check(nf, nq, lanes, s)) return fail(kErrArg, why);
+  if ((nq > 0 && (!q || !grad_w || !grad_q)) || (nf > 0 && (!tri || !grad_tri)))
+    return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::bwd_launch(tri, q, grad_w, grad_q, grad_tri, nf, nq, lanes, s,
+                                (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_backward", e));
+  return kOk;
+}
+
+int64_t wn_scatter_corners(const double* grad_tri, const void* F, int64_t fdtype, int64_t nf,
+                           int64_t nv, double* grad_V, void* stream) {
+  if (nf < 0 || nv < 0) return fail(kErrArg, "negative size");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if ((nf > 0 && (!grad_tri || !F)) || (nv > 0 && !grad_V)) return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::scatter_corners_launch(grad_tri, F, (int)fdtype, nf, nv, grad_V,
+                                            (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_scatter_corners", e));
+  return kOk;
+}
+
 }  // extern "C"

@@ -42,3 +42,17 @@ hipError_t pack_soa_launch(const void* V, const void* F, int64_t nv, int64_t nf,
                            hipStream_t stream);
 
 }  // namespace wn
+
+namespace wn {
+
+// Adjoint (wn_backward.hip). grad_q (nq x 3) and grad_tri (9 x nf) are
+// overwritten.
+const char* bwd_check(int64_t nf, int64_t nq, int qpl, int splits);
+hipError_t bwd_launch(const double* tri, const double* q, const double* gw, double* gq,
+                      double* gtri, int64_t nf, int64_t nq, int qpl, int splits,
+                      hipStream_t stream);
+// grad_V[F[f, c]] += grad_tri corner c of face f; grad_V (nv x 3) overwritten.
+hipError_t scatter_corners_launch(const double* gtri, const void* F, int fdtype, int64_t nf,
+                                  int64_t nv, double* gv, hipStream_t stream);
+
+}  // namespace wn

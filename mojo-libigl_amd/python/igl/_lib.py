@@ -38,6 +38,8 @@ _SIGNATURES = {
     "wn_forward_workspace_bytes": ([I] * 6, I),
     "wn_forward": ([I] * 9 + [D] + [I] * 3, I),
     "wn_pack_soa": ([I] * 9, I),
+    "wn_backward": ([I] * 11, I),
+    "wn_scatter_corners": ([I] * 7, I),
 }
 
 # Same policy as the reference (python/igl/_lib.py:37-50): at most four

@@ -1,0 +1,139 @@
+"""PyTorch autograd layer over the HIP winding-number kernels.
+
+Tensors stay on the GPU; every computation is a C-ABI call (include/wn.h) on
+PyTorch's current stream, with PyTorch only allocating device memory:
+
+* ``pack_faces(V, F)``      -> wn_pack_soa (device gather of V[F], 9 x m f64)
+* ``forward(tri, O)``       -> wn_forward (FP64 plain or FP32-compensated)
+* ``WindingNumber.apply``   -> forward, then wn_backward + wn_scatter_corners
+  for dL/dO and dL/dV (FP64 adjoint, face tiles recomputed).
+
+``winding_number(V, F, O, precision="fp64")`` is the differentiable entry
+point. V may be float32 or float64, F int32 or int64, O float32 or float64;
+w is float64 and gradients come back in the dtype of their input.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import PRECISIONS
+from ._lib import last_error, lib
+
+_INT32_MAX = 2**31 - 1
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise RuntimeError(f"{what} failed on the GPU: {last_error()}")
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _require_cuda(*tensors) -> None:
+    for t in tensors:
+        if not t.is_cuda:
+            raise ValueError("igl.autograd expects CUDA/HIP tensors")
+
+
+def pack_faces(V: torch.Tensor, F: torch.Tensor, validate: bool = True) -> torch.Tensor:
+    """9 x m float64 face SoA (rows ax..cz) gathered on the device."""
+    _require_cuda(V, F)
+    if V.dim() != 2 or V.shape[1] != 3:
+        raise ValueError("V must have shape (n, 3)")
+    if F.dim() != 2 or F.shape[1] != 3:
+        raise ValueError("F must have shape (m, 3)")
+    if V.dtype not in (torch.float32, torch.float64):
+        raise TypeError("V must be float32 or float64")
+    if F.dtype not in (torch.int32, torch.int64):
+        raise TypeError("F must be int32 or int64")
+    V = V.contiguous()
+    F = F.contiguous()
+    tri = torch.empty((9, F.shape[0]), dtype=torch.float64, device=V.device)
+    bad = torch.full((1,), _INT32_MAX, dtype=torch.int32, device=V.device)
+    _check(lib().wn_pack_soa(V.data_ptr(), F.data_ptr(), V.shape[0], F.shape[0],
+                             0 if V.dtype == torch.float64 else 1,
+                             0 if F.dtype == torch.int64 else 1,
+                             tri.data_ptr(), bad.data_ptr(), _stream()), "wn_pack_soa")
+    if validate and F.shape[0] and int(bad.item()) != _INT32_MAX:
+        raise IndexError("face index is outside V")
+    return tri
+
+
+def resolve_config(nf: int, nq: int, precision: int, qpl=0, splits=0, sort=-1, kappa=-1.0):
+    out = np.zeros(4, dtype=np.int64)
+    _check(lib().wn_resolve_config(nf, nq, precision, qpl, splits, sort, float(kappa),
+                                   out.ctypes.data), "wn_resolve_config")
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def forward(tri: torch.Tensor, O: torch.Tensor, precision="fp64", qpl=0, splits=0, sort=-1,
+            kappa=-1.0) -> torch.Tensor:
+    """w (nq,) float64 for the 9 x m SoA ``tri`` and queries O (nq, 3)."""
+    mode = PRECISIONS[str(precision).lower()] if isinstance(precision, str) else int(precision)
+    _require_cuda(tri, O)
+    Q = O.to(torch.float64).contiguous()
+    nf, nq = tri.shape[1], Q.shape[0]
+    qpl, splits, sort = resolve_config(nf, nq, mode, qpl, splits, sort, kappa)
+    nbytes = lib().wn_forward_workspace_bytes(nf, nq, mode, qpl, splits, sort)
+    if nbytes < 0:
+        raise RuntimeError(f"wn_forward_workspace_bytes failed: {last_error()}")
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=Q.device)
+    w = torch.empty(nq, dtype=torch.float64, device=Q.device)
+    _check(lib().wn_forward(tri.data_ptr(), Q.data_ptr(), w.data_ptr(), nf, nq, mode, qpl,
+                            splits, sort, float(kappa), ws.data_ptr(), nbytes, _stream()),
+           "wn_forward")
+    return w
+
+
+def backward(tri: torch.Tensor, Q: torch.Tensor, grad_w: torch.Tensor, qpl=0, splits=0):
+    """(dL/dO (nq, 3), dL/dtri (9, m)) in float64 for upstream dL/dw."""
+    Q = Q.to(torch.float64).contiguous()
+    gw = grad_w.to(torch.float64).contiguous()
+    nf, nq = tri.shape[1], Q.shape[0]
+    gq = torch.empty((nq, 3), dtype=torch.float64, device=Q.device)
+    gtri = torch.empty((9, nf), dtype=torch.float64, device=Q.device)
+    _check(lib().wn_backward(tri.data_ptr(), Q.data_ptr(), gw.data_ptr(), gq.data_ptr(),
+                             gtri.data_ptr(), nf, nq, 0, qpl, splits, _stream()), "wn_backward")
+    return gq, gtri
+
+
+def scatter_corners(grad_tri: torch.Tensor, F: torch.Tensor, nv: int) -> torch.Tensor:
+    F = F.contiguous()
+    gv = torch.empty((nv, 3), dtype=torch.float64, device=grad_tri.device)
+    _check(lib().wn_scatter_corners(grad_tri.data_ptr(), F.data_ptr(),
+                                    0 if F.dtype == torch.int64 else 1, F.shape[0], nv,
+                                    gv.data_ptr(), _stream()), "wn_scatter_corners")
+    return gv
+
+
+class WindingNumber(torch.autograd.Function):
+    """w = winding_number(V, F, O); backward gives dL/dV and dL/dO."""
+
+    @staticmethod
+    def forward(ctx, V, F, O, precision="fp64"):
+        tri = pack_faces(V, F)
+        Q = O.to(torch.float64).contiguous()
+        w = forward(tri, Q, precision)
+        ctx.save_for_backward(tri, Q, F)
+        ctx.nv = V.shape[0]
+        ctx.vdtype = V.dtype
+        ctx.odtype = O.dtype
+        return w
+
+    @staticmethod
+    def backward(ctx, grad_w):
+        tri, Q, F = ctx.saved_tensors
+        need_v, _, need_o = ctx.needs_input_grad[:3]
+        gq, gtri = backward(tri, Q, grad_w)
+        gv = scatter_corners(gtri, F, ctx.nv).to(ctx.vdtype) if need_v else None
+        go = gq.to(ctx.odtype) if need_o else None
+        return gv, None, go, None
+
+
+def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision="fp64"):
+    """Differentiable generalized winding number of (V, F) at the rows of O."""
+    return WindingNumber.apply(V, F, O, precision)
