@@ -99,6 +99,36 @@ int64_t wn_pack_soa(const void *V, const void *F, int64_t nv, int64_t nf,
                     int64_t vdtype, int64_t fdtype, double *tri,
                     int32_t *bad_index, void *stream);
 
+/* ---- Batched forward ------------------------------------------------------ */
+
+/* No reference counterpart (the reference evaluates one mesh per call,
+ * src/capi.mojo:366-398). Plans a batch of n_meshes meshes: all faces in one
+ * 9 x F_total tri SoA, mesh m owning faces [face_off[m], face_off[m+1]) and
+ * queries [query_off[m], query_off[m+1]) of one Q_total x 3 array. (host)
+ * face_off, query_off (n_meshes + 1 entries each, starting at 0). Builds the
+ * longest-first work list of (mesh, query tile, face chunk) items, uploads it
+ * and allocates the plan's device workspace (the only allocation of the
+ * batch path). precision 0/1; qpl 1 or 2 (0 = 1); chunk_faces a multiple of
+ * 256 (0 = 2048); sort -1 = Morton order for FP32-compensated. Returns a
+ * handle (> 0) or a negative status. */
+int64_t wn_batch_create(const int64_t *face_off, const int64_t *query_off,
+                        int64_t n_meshes, int64_t precision, int64_t qpl,
+                        int64_t chunk_faces, int64_t sort, double kappa);
+
+/* Runs a plan: (device) tri (9 x F_total), q (Q_total x 3), dst (Q_total).
+ * use_graph != 0 replays a hipGraph of the persistent launch, captured on
+ * first use for these buffer addresses. Asynchronous on `stream`. Calls on
+ * one handle must not overlap. */
+int64_t wn_batch_run(int64_t handle, const double *tri, const double *q,
+                     double *dst, int64_t use_graph, void *stream);
+
+/* out6 = {work items, partial slots, workspace bytes, persistent grid,
+ * F_total, Q_total}. */
+int64_t wn_batch_info(int64_t handle, int64_t *out6);
+
+/* Frees a plan (synchronises the device first). */
+int64_t wn_batch_destroy(int64_t handle);
+
 /* ---- Adjoint ------------------------------------------------------------ */
 
 /* No reference counterpart (the reference has no derivative of

@@ -1,0 +1,377 @@
+// Batched forward over many meshes of uneven sizes in one persistent launch.
+//
+// No counterpart in the reference (one mesh per call, src/capi.mojo:366-398).
+//
+// Layout: all meshes' faces in one 9 x F_total SoA (mesh m owns faces
+// [face_off[m], face_off[m+1])), all queries in one Q_total x 3 array (mesh m
+// owns [query_off[m], query_off[m+1])), one Q_total output.
+//
+// Plan (host, once per batch shape): work items (mesh, query tile of
+// 256*QPL queries, face chunk of `chunk_faces` faces), sorted by decreasing
+// pair count (longest-processing-time first) so the tail of the persistent
+// loop holds only small items. Persistent kernel: CUs x occupancy blocks,
+// each pulls the next item index from an atomic counter until the list is
+// exhausted (every block reaches the exit test each iteration). Items of a
+// single-chunk mesh write w directly; multi-chunk meshes write unscaled
+// partials to fixed slots, summed in chunk order by a reduce kernel, so the
+// result does not depend on which block took which item.
+//
+// The fill-counter / persistent / reduce sequence can be captured once into
+// a hipGraph and replayed (fill kernel, no memset node). The optional Morton
+// ordering of queries (segmented by mesh) runs eagerly before the graph.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+#include "wn_internal.h"
+#include "wn_sweep.h"
+
+namespace wn {
+
+struct BatchMesh {
+  int64_t face_off;
+  int64_t query_off;
+  int64_t pbase;  // first partial slot (chunks > 1)
+  int32_t nf;
+  int32_t nq;
+  int32_t chunks;
+  int32_t pad;
+};
+
+struct BatchItem {
+  int32_t mesh;
+  int32_t qtile;
+  int32_t chunk;
+  int32_t pad;
+};
+
+namespace {
+
+__global__ void counter_reset_kernel(int32_t* counter) {
+  if (threadIdx.x == 0) counter[0] = 0;
+}
+
+template <int PREC, int QPL>
+__global__ __launch_bounds__(kBlockThreads) void batch_kernel(
+    const double* __restrict__ tri, int64_t ftot, const double* __restrict__ q,
+    const int32_t* __restrict__ perm, double* __restrict__ dst, double* __restrict__ partial,
+    const BatchMesh* __restrict__ meshes, const BatchItem* __restrict__ items, int64_t n_items,
+    int32_t* __restrict__ counter, int64_t chunk_faces, float kappa) {
+  __shared__ typename std::conditional<PREC == kFP64, SweepF64Smem, SweepF32Smem>::type sm;
+  __shared__ int32_t s_item;
+  const int tid = threadIdx.x;
+  constexpr int QB = kBlockThreads * QPL;
+  for (;;) {
+    __syncthreads();  // every thread has read the previous s_item
+    if (tid == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    if (it >= n_items) break;  // block-uniform exit
+    const BatchItem item = items[it];
+    const BatchMesh m = meshes[item.mesh];
+    const int64_t slot0 = m.query_off + (int64_t)item.qtile * QB;
+    const int64_t slot_end = m.query_off + m.nq;
+    const int64_t f0 = m.face_off + (int64_t)item.chunk * chunk_faces;
+    int64_t f1 = f0 + chunk_faces;
+    if (f1 > m.face_off + m.nf) f1 = m.face_off + m.nf;
+
+    int64_t qidx[QPL];
+    bool valid[QPL];
+    double Ox[QPL], Oy[QPL], Oz[QPL], acc[QPL];
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      const int64_t slot = slot0 + k * kBlockThreads + tid;
+      valid[k] = slot < slot_end;
+      qidx[k] = valid[k] ? (perm ? (int64_t)perm[slot] : slot) : 0;
+      Ox[k] = valid[k] ? q[3 * qidx[k]] : 0.0;
+      Oy[k] = valid[k] ? q[3 * qidx[k] + 1] : 0.0;
+      Oz[k] = valid[k] ? q[3 * qidx[k] + 2] : 0.0;
+      acc[k] = 0.0;
+    }
+    if constexpr (PREC == kFP64) {
+      sweep_f64<QPL>(tri, ftot, f0, f1, Ox, Oy, Oz, acc, sm);
+    } else {
+      double gx, gy, gz;
+      block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz);
+      float ox[QPL], oy[QPL], oz[QPL];
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        ox[k] = valid[k] ? (float)(Ox[k] - gx) : 0.0f;
+        oy[k] = valid[k] ? (float)(Oy[k] - gy) : 0.0f;
+        oz[k] = valid[k] ? (float)(Oz[k] - gz) : 0.0f;
+      }
+      sweep_f32c<QPL>(tri, ftot, f0, f1, gx, gy, gz, ox, oy, oz, kappa, acc, sm);
+    }
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      if (valid[k]) {
+        if (m.chunks == 1)
+          dst[qidx[k]] = acc[k] * kInv2Pi;
+        else
+          partial[m.pbase + (qidx[k] - m.query_off) * m.chunks + item.chunk] = acc[k];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlockThreads) void batch_reduce_kernel(
+    const double* __restrict__ partial, double* __restrict__ dst,
+    const BatchMesh* __restrict__ meshes, const int64_t* __restrict__ query_off, int n_meshes,
+    int64_t qtot) {
+  const int64_t qi = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (qi >= qtot) return;
+  int lo = 0, hi = n_meshes;  // query_off[lo] <= qi < query_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (query_off[mid] <= qi) lo = mid; else hi = mid;
+  }
+  const BatchMesh m = meshes[lo];
+  if (m.chunks <= 1) return;
+  const double* p = partial + m.pbase + (qi - m.query_off) * m.chunks;
+  double s = 0.0;
+  for (int k = 0; k < m.chunks; ++k) s += p[k];
+  dst[qi] = s * kInv2Pi;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct BatchPlan {
+  int precision = kFP32C;
+  int qpl = 1;
+  int sort = 0;
+  float kappa = 1.0f;
+  int64_t chunk_faces = 2048;
+  int n_meshes = 0;
+  int64_t ftot = 0, qtot = 0, n_items = 0, n_partials = 0;
+  int grid = 0;
+  std::vector<int64_t> query_off;
+  // device
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  int32_t* d_counter = nullptr;
+  BatchMesh* d_meshes = nullptr;
+  BatchItem* d_items = nullptr;
+  int64_t* d_query_off = nullptr;
+  double* d_partial = nullptr;
+  int32_t* d_perm = nullptr;
+  void* d_sort_ws = nullptr;
+  size_t sort_bytes = 0;
+  // graph cache
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* g_tri = nullptr;
+  const void* g_q = nullptr;
+  void* g_dst = nullptr;
+};
+
+static const void* batch_kernel_ptr(int precision, int qpl) {
+  if (precision == kFP64)
+    return qpl == 1 ? (const void*)batch_kernel<kFP64, 1> : (const void*)batch_kernel<kFP64, 2>;
+  return qpl == 1 ? (const void*)batch_kernel<kFP32C, 1> : (const void*)batch_kernel<kFP32C, 2>;
+}
+
+const char* batch_check(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
+                        int precision, int qpl, int64_t chunk_faces) {
+  if (n_meshes < 1 || n_meshes > (1 << 24)) return "n_meshes must be in [1, 2^24]";
+  if (!face_off || !query_off) return "null offsets";
+  if (precision != kFP64 && precision != kFP32C) return "unknown precision";
+  if (qpl != 1 && qpl != 2) return "batch queries_per_lane must be 1 or 2";
+  if (chunk_faces < kFaceTile || chunk_faces % kFaceTile) return "chunk_faces must be a positive multiple of 256";
+  if (face_off[0] != 0 || query_off[0] != 0) return "offsets must start at 0";
+  for (int64_t m = 0; m < n_meshes; ++m) {
+    if (face_off[m + 1] < face_off[m] || query_off[m + 1] < query_off[m]) return "offsets must be non-decreasing";
+    if (face_off[m + 1] - face_off[m] > 0x7fffffff || query_off[m + 1] - query_off[m] > 0x7fffffff)
+      return "a mesh has more than 2^31-1 faces or queries";
+  }
+  if (query_off[n_meshes] > 0x7fffffff) return "more than 2^31-1 queries in the batch";
+  return nullptr;
+}
+
+hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
+                        int precision, int qpl, int64_t chunk_faces, int sort, float kappa,
+                        BatchPlan** out) {
+  BatchPlan* p = new BatchPlan();
+  p->precision = precision;
+  p->qpl = qpl;
+  p->sort = sort;
+  p->kappa = kappa;
+  p->chunk_faces = chunk_faces;
+  p->n_meshes = (int)n_meshes;
+  p->ftot = face_off[n_meshes];
+  p->qtot = query_off[n_meshes];
+  p->query_off.assign(query_off, query_off + n_meshes + 1);
+
+  const int64_t QB = (int64_t)kBlockThreads * qpl;
+  std::vector<BatchMesh> meshes((size_t)n_meshes);
+  std::vector<std::pair<int64_t, BatchItem>> items;
+  int64_t pbase = 0;
+  for (int64_t m = 0; m < n_meshes; ++m) {
+    BatchMesh bm{};
+    bm.face_off = face_off[m];
+    bm.query_off = query_off[m];
+    bm.nf = (int32_t)(face_off[m + 1] - face_off[m]);
+    bm.nq = (int32_t)(query_off[m + 1] - query_off[m]);
+    bm.chunks = (int32_t)std::max<int64_t>(1, (bm.nf + chunk_faces - 1) / chunk_faces);
+    bm.pbase = pbase;
+    if (bm.chunks > 1) pbase += (int64_t)bm.nq * bm.chunks;
+    meshes[(size_t)m] = bm;
+    const int64_t tiles = (bm.nq + QB - 1) / QB;
+    for (int64_t j = 0; j < tiles; ++j) {
+      const int64_t qn = std::min<int64_t>(QB, bm.nq - j * QB);
+      for (int32_t k = 0; k < bm.chunks; ++k) {
+        const int64_t fn = std::min<int64_t>(chunk_faces, bm.nf - (int64_t)k * chunk_faces);
+        // cost in pairs, plus a fixed per-item overhead of one face tile
+        const int64_t cost = qn * std::max<int64_t>(fn, 0) + QB * kFaceTile;
+        items.push_back({cost, BatchItem{(int32_t)m, (int32_t)j, k, 0}});
+      }
+    }
+  }
+  std::stable_sort(items.begin(), items.end(),
+                   [](const auto& a, const auto& b) { return a.first > b.first; });
+  p->n_items = (int64_t)items.size();
+  p->n_partials = pbase;
+  std::vector<BatchItem> flat(items.size());
+  for (size_t i = 0; i < items.size(); ++i) flat[i] = items[i].second;
+
+  // persistent grid: CUs x resident blocks of this kernel instance
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_ptr(precision, qpl),
+                                                     kBlockThreads, 0);
+  if (e != hipSuccess) {
+    delete p;
+    return e;
+  }
+  int64_t grid = (int64_t)std::max(1, cus) * std::max(1, per_cu);
+  p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(1, p->n_items)));
+
+  const size_t b_counter = align_up(sizeof(int32_t));
+  const size_t b_meshes = align_up(sizeof(BatchMesh) * (size_t)n_meshes);
+  const size_t b_items = align_up(sizeof(BatchItem) * std::max<size_t>(1, flat.size()));
+  const size_t b_qoff = align_up(sizeof(int64_t) * (size_t)(n_meshes + 1));
+  const size_t b_partial = align_up(sizeof(double) * (size_t)std::max<int64_t>(1, pbase));
+  const size_t b_perm = sort ? align_up(sizeof(int32_t) * (size_t)std::max<int64_t>(1, p->qtot)) : 0;
+  p->sort_bytes = sort ? morton_segmented_workspace_bytes(p->qtot) : 0;
+  p->ws_bytes = b_counter + b_meshes + b_items + b_qoff + b_partial + b_perm + p->sort_bytes;
+  e = hipMalloc(&p->ws, p->ws_bytes);
+  if (e != hipSuccess) {
+    delete p;
+    return e;
+  }
+  char* c = (char*)p->ws;
+  p->d_counter = (int32_t*)c; c += b_counter;
+  p->d_meshes = (BatchMesh*)c; c += b_meshes;
+  p->d_items = (BatchItem*)c; c += b_items;
+  p->d_query_off = (int64_t*)c; c += b_qoff;
+  p->d_partial = (double*)c; c += b_partial;
+  if (sort) {
+    p->d_perm = (int32_t*)c; c += b_perm;
+    p->d_sort_ws = c;
+  }
+  e = hipMemcpy(p->d_meshes, meshes.data(), sizeof(BatchMesh) * meshes.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !flat.empty())
+    e = hipMemcpy(p->d_items, flat.data(), sizeof(BatchItem) * flat.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_query_off, query_off, sizeof(int64_t) * (size_t)(n_meshes + 1),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)hipFree(p->ws);
+    delete p;
+    return e;
+  }
+  *out = p;
+  return hipSuccess;
+}
+
+static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q, double* dst,
+                                hipStream_t s) {
+  counter_reset_kernel<<<1, 64, 0, s>>>(p->d_counter);
+  const int32_t* perm = p->sort ? p->d_perm : nullptr;
+  const dim3 grid((unsigned)p->grid);
+#define WN_BATCH_LAUNCH(PREC, QPL)                                                             \
+  batch_kernel<PREC, QPL><<<grid, kBlockThreads, 0, s>>>(tri, p->ftot, q, perm, dst,          \
+                                                          p->d_partial, p->d_meshes,          \
+                                                          p->d_items, p->n_items,             \
+                                                          p->d_counter, p->chunk_faces,       \
+                                                          p->kappa)
+  if (p->precision == kFP64) {
+    if (p->qpl == 1) WN_BATCH_LAUNCH(kFP64, 1); else WN_BATCH_LAUNCH(kFP64, 2);
+  } else {
+    if (p->qpl == 1) WN_BATCH_LAUNCH(kFP32C, 1); else WN_BATCH_LAUNCH(kFP32C, 2);
+  }
+#undef WN_BATCH_LAUNCH
+  if (p->n_partials > 0 && p->qtot > 0)
+    batch_reduce_kernel<<<dim3((unsigned)((p->qtot + kBlockThreads - 1) / kBlockThreads)),
+                          kBlockThreads, 0, s>>>(p->d_partial, dst, p->d_meshes, p->d_query_off,
+                                                 p->n_meshes, p->qtot);
+  return hipGetLastError();
+}
+
+hipError_t batch_run(BatchPlan* p, const double* tri, const double* q, double* dst,
+                     bool use_graph, hipStream_t stream) {
+  if (p->qtot == 0) return hipSuccess;
+  hipError_t e;
+  if (p->sort) {
+    e = morton_order_segmented(q, p->qtot, p->d_query_off, p->n_meshes, p->d_perm,
+                               p->d_sort_ws, p->sort_bytes, stream);
+    if (e != hipSuccess) return e;
+  }
+  if (!use_graph) return batch_enqueue(p, tri, q, dst, stream);
+  if (!p->exec || p->g_tri != tri || p->g_q != q || p->g_dst != dst) {
+    if (p->exec) {
+      (void)hipGraphExecDestroy(p->exec);
+      p->exec = nullptr;
+    }
+    e = hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = batch_enqueue(p, tri, q, dst, p->cap_stream);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(p->cap_stream, &graph);
+    if (le != hipSuccess) e = le;
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      p->exec = nullptr;
+      return e;
+    }
+    p->g_tri = tri;
+    p->g_q = q;
+    p->g_dst = dst;
+  }
+  return hipGraphLaunch(p->exec, stream);
+}
+
+void batch_destroy(BatchPlan* p) {
+  if (!p) return;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->cap_stream) {
+    (void)hipStreamSynchronize(p->cap_stream);
+    (void)hipStreamDestroy(p->cap_stream);
+  }
+  if (p->ws) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p->ws);
+  }
+  delete p;
+}
+
+void batch_info(const BatchPlan* p, int64_t* out) {
+  out[0] = p->n_items;
+  out[1] = p->n_partials;
+  out[2] = (int64_t)p->ws_bytes;
+  out[3] = p->grid;
+  out[4] = p->ftot;
+  out[5] = p->qtot;
+}
+
+}  // namespace wn

@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <string>
 
 #include "wn_internal.h"
@@ -173,6 +174,23 @@ int64_t host_path(const double* tri, const double* q, double* dst, int64_t nf, i
   return kOk;
 }
 
+// Live batch plans; handles are checked against this set before use.
+std::mutex& batch_registry_mu() {
+  static std::mutex mu;
+  return mu;
+}
+
+std::set<wn::BatchPlan*>& batch_registry() {
+  static std::set<wn::BatchPlan*> live;
+  return live;
+}
+
+wn::BatchPlan* lookup_batch(int64_t handle) {
+  std::lock_guard<std::mutex> lock(batch_registry_mu());
+  auto it = batch_registry().find((wn::BatchPlan*)(intptr_t)handle);
+  return it == batch_registry().end() ? nullptr : *it;
+}
+
 }  // namespace
 
 extern "C" {
@@ -275,6 +293,57 @@ int64_t wn_backward(const double* tri, const double* q, const double* grad_w, do
   hipError_t e = wn::bwd_launch(tri, q, grad_w, grad_q, grad_tri, nf, nq, lanes, s,
                                 (hipStream_t)stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_backward", e));
+  return kOk;
+}
+
+int64_t wn_batch_create(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
+                        int64_t precision, int64_t qpl, int64_t chunk_faces, int64_t sort,
+                        double kappa) {
+  const int lanes = (int)(qpl <= 0 ? 1 : qpl);
+  const int64_t chunk = chunk_faces <= 0 ? 2048 : chunk_faces;
+  if (const char* why = wn::batch_check(face_off, query_off, n_meshes, (int)precision, lanes, chunk))
+    return fail(kErrArg, why);
+  if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  const int sorted = sort < 0 ? (precision == wn::kFP32C ? 1 : 0) : (sort ? 1 : 0);
+  wn::BatchPlan* plan = nullptr;
+  hipError_t e = wn::batch_create(face_off, query_off, n_meshes, (int)precision, lanes, chunk,
+                                  sorted, kappa < 0.0 ? 1.0f : (float)kappa, &plan);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_batch_create", e));
+  std::lock_guard<std::mutex> lock(batch_registry_mu());
+  batch_registry().insert(plan);
+  return (int64_t)(intptr_t)plan;
+}
+
+int64_t wn_batch_run(int64_t handle, const double* tri, const double* q, double* dst,
+                     int64_t use_graph, void* stream) {
+  wn::BatchPlan* plan = lookup_batch(handle);
+  if (!plan) return fail(kErrArg, "unknown batch handle");
+  int64_t info[6];
+  wn::batch_info(plan, info);
+  if ((info[4] > 0 && !tri) || (info[5] > 0 && (!q || !dst))) return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::batch_run(plan, tri, q, dst, use_graph != 0, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_batch_run", e));
+  return kOk;
+}
+
+int64_t wn_batch_info(int64_t handle, int64_t* out6) {
+  wn::BatchPlan* plan = lookup_batch(handle);
+  if (!plan) return fail(kErrArg, "unknown batch handle");
+  if (!out6) return fail(kErrArg, "null output");
+  wn::batch_info(plan, out6);
+  return kOk;
+}
+
+int64_t wn_batch_destroy(int64_t handle) {
+  wn::BatchPlan* plan = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(batch_registry_mu());
+    auto it = batch_registry().find((wn::BatchPlan*)(intptr_t)handle);
+    if (it == batch_registry().end()) return fail(kErrArg, "unknown batch handle");
+    plan = *it;
+    batch_registry().erase(it);
+  }
+  wn::batch_destroy(plan);
   return kOk;
 }
 

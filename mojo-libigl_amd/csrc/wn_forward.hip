@@ -17,6 +17,7 @@
 
 #include "wn_device.h"
 #include "wn_internal.h"
+#include "wn_sweep.h"
 
 namespace wn {
 
@@ -28,7 +29,7 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f64_kernel(
     const double* __restrict__ tri, const double* __restrict__ q,
     double* __restrict__ out, int64_t nf, int64_t nq, int64_t faces_per_split,
     int splits) {
-  __shared__ double2 sface[kFaceTile][5];
+  __shared__ SweepF64Smem sm;
   const int tid = threadIdx.x;
   const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
 
@@ -42,32 +43,9 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f64_kernel(
     oz[k] = ok ? q[3 * qi + 2] : 0.0;
     acc[k] = 0.0;
   }
-
   const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
   const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
-  for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
-    const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
-    __syncthreads();
-    if (tid < cnt) {
-      const int64_t f = t0 + tid;
-      sface[tid][0] = make_double2(tri[f], tri[nf + f]);
-      sface[tid][1] = make_double2(tri[2 * nf + f], tri[3 * nf + f]);
-      sface[tid][2] = make_double2(tri[4 * nf + f], tri[5 * nf + f]);
-      sface[tid][3] = make_double2(tri[6 * nf + f], tri[7 * nf + f]);
-      sface[tid][4] = make_double2(tri[8 * nf + f], 0.0);
-    }
-    __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
-      const double2 p0 = sface[j][0], p1 = sface[j][1], p2 = sface[j][2],
-                    p3 = sface[j][3], p4 = sface[j][4];
-#pragma unroll
-      for (int k = 0; k < QPL; ++k) {
-        acc[k] += pair_f64(p0.x - ox[k], p0.y - oy[k], p1.x - oz[k],
-                           p1.y - ox[k], p2.x - oy[k], p2.y - oz[k],
-                           p3.x - ox[k], p3.y - oy[k], p4.x - oz[k]);
-      }
-    }
-  }
+  sweep_f64<QPL>(tri, nf, f0, f1, ox, oy, oz, acc, sm);
 
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
@@ -100,85 +78,41 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f32c_kernel(
     const double* __restrict__ tri, const double* __restrict__ q,
     const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nf,
     int64_t nq, int64_t faces_per_split, int splits, float kappa) {
-  __shared__ FaceRec32 sface[kFaceTile];
-  __shared__ double sred[6][kBlockThreads / kWave];
+  __shared__ SweepF32Smem sm;
   const int tid = threadIdx.x;
   const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
 
   int64_t qidx[QPL];
+  bool valid[QPL];
   double Ox[QPL], Oy[QPL], Oz[QPL];
-  double lo0 = __builtin_inf(), lo1 = __builtin_inf(), lo2 = __builtin_inf();
-  double hi0 = -__builtin_inf(), hi1 = -__builtin_inf(), hi2 = -__builtin_inf();
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
     const int64_t slot = qbase + k * kBlockThreads + tid;
-    const bool ok = slot < nq;
-    qidx[k] = ok ? (perm ? (int64_t)perm[slot] : slot) : -1;
-    if (ok) {
-      Ox[k] = q[3 * qidx[k]];
-      Oy[k] = q[3 * qidx[k] + 1];
-      Oz[k] = q[3 * qidx[k] + 2];
-      lo0 = fmin(lo0, Ox[k]); hi0 = fmax(hi0, Ox[k]);
-      lo1 = fmin(lo1, Oy[k]); hi1 = fmax(hi1, Oy[k]);
-      lo2 = fmin(lo2, Oz[k]); hi2 = fmax(hi2, Oz[k]);
-    }
+    valid[k] = slot < nq;
+    qidx[k] = valid[k] ? (perm ? (int64_t)perm[slot] : slot) : 0;
+    Ox[k] = valid[k] ? q[3 * qidx[k]] : 0.0;
+    Oy[k] = valid[k] ? q[3 * qidx[k] + 1] : 0.0;
+    Oz[k] = valid[k] ? q[3 * qidx[k] + 2] : 0.0;
   }
-  lo0 = wave_min_f64(lo0); lo1 = wave_min_f64(lo1); lo2 = wave_min_f64(lo2);
-  hi0 = wave_max_f64(hi0); hi1 = wave_max_f64(hi1); hi2 = wave_max_f64(hi2);
-  const int wave = tid / kWave;
-  if ((tid & (kWave - 1)) == 0) {
-    sred[0][wave] = lo0; sred[1][wave] = lo1; sred[2][wave] = lo2;
-    sred[3][wave] = hi0; sred[4][wave] = hi1; sred[5][wave] = hi2;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int w = 0; w < kBlockThreads / kWave; ++w) {
-    lo0 = fmin(lo0, sred[0][w]); lo1 = fmin(lo1, sred[1][w]); lo2 = fmin(lo2, sred[2][w]);
-    hi0 = fmax(hi0, sred[3][w]); hi1 = fmax(hi1, sred[4][w]); hi2 = fmax(hi2, sred[5][w]);
-  }
-  const double gx = 0.5 * (lo0 + hi0), gy = 0.5 * (lo1 + hi1), gz = 0.5 * (lo2 + hi2);
+  double gx, gy, gz;
+  block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz);
 
   float ox[QPL], oy[QPL], oz[QPL];
   double acc[QPL];
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
-    const bool ok = qidx[k] >= 0;
-    ox[k] = ok ? (float)(Ox[k] - gx) : 0.0f;
-    oy[k] = ok ? (float)(Oy[k] - gy) : 0.0f;
-    oz[k] = ok ? (float)(Oz[k] - gz) : 0.0f;
+    ox[k] = valid[k] ? (float)(Ox[k] - gx) : 0.0f;
+    oy[k] = valid[k] ? (float)(Oy[k] - gy) : 0.0f;
+    oz[k] = valid[k] ? (float)(Oz[k] - gz) : 0.0f;
     acc[k] = 0.0;
   }
-
   const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
   const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
-  for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
-    const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
-    __syncthreads();
-    if (tid < cnt) {
-      const int64_t f = t0 + tid;
-      sface[tid] = make_face_rec32(tri[f], tri[nf + f], tri[2 * nf + f],
-                                   tri[3 * nf + f], tri[4 * nf + f], tri[5 * nf + f],
-                                   tri[6 * nf + f], tri[7 * nf + f], tri[8 * nf + f],
-                                   gx, gy, gz);
-    }
-    __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
-      const float4 r0 = sface[j].r0, r1 = sface[j].r1, r2 = sface[j].r2,
-                   r3 = sface[j].r3;
-#pragma unroll
-      for (int k = 0; k < QPL; ++k) {
-        bool risky;
-        const float ang = pair_rec32(r0, r1, r2, r3, ox[k], oy[k], oz[k], kappa, risky);
-        double a64 = (double)ang;
-        if (risky) a64 = pair_rec32_f64(r0, r1, r2, ox[k], oy[k], oz[k]);
-        acc[k] += a64;
-      }
-    }
-  }
+  sweep_f32c<QPL>(tri, nf, f0, f1, gx, gy, gz, ox, oy, oz, kappa, acc, sm);
 
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
-    if (qidx[k] >= 0) {
+    if (valid[k]) {
       if (splits == 1)
         out[qidx[k]] = acc[k] * kInv2Pi;
       else

@@ -56,3 +56,26 @@ hipError_t scatter_corners_launch(const double* gtri, const void* F, int fdtype,
                                   int64_t nv, double* gv, hipStream_t stream);
 
 }  // namespace wn
+
+namespace wn {
+
+// Batched persistent forward (wn_batch.hip).
+struct BatchPlan;
+const char* batch_check(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
+                        int precision, int qpl, int64_t chunk_faces);
+hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
+                        int precision, int qpl, int64_t chunk_faces, int sort, float kappa,
+                        BatchPlan** out);
+hipError_t batch_run(BatchPlan* p, const double* tri, const double* q, double* dst,
+                     bool use_graph, hipStream_t stream);
+void batch_destroy(BatchPlan* p);
+// out6 = {items, partial slots, workspace bytes, persistent grid, faces, queries}
+void batch_info(const BatchPlan* p, int64_t* out6);
+
+// Morton ordering inside segments [seg_off[s], seg_off[s+1]) (wn_sort.hip):
+// each segment stays in place, its queries are reordered along a Z-curve.
+size_t morton_segmented_workspace_bytes(int64_t nq);
+hipError_t morton_order_segmented(const double* q, int64_t nq, const int64_t* seg_off, int nseg,
+                                  int32_t* perm, void* ws, size_t ws_bytes, hipStream_t stream);
+
+}  // namespace wn

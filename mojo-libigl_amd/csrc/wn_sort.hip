@@ -142,4 +142,89 @@ hipError_t morton_order(const double* q, int64_t nq, int32_t* perm, void* ws,
                                             (int)nq, 0, 30, stream);
 }
 
+// ---------------------------------------------------------------------------
+// Segmented variant: key = (segment << 30) | code, codes on the bbox of all
+// queries; the segment bits keep every segment in place.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+__global__ __launch_bounds__(kBlockThreads) void morton_seg_kernel(
+    const double* __restrict__ q, int64_t nq, const unsigned long long* __restrict__ box,
+    const int64_t* __restrict__ seg_off, int nseg, unsigned long long* __restrict__ keys,
+    int32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (i >= nq) return;
+  int lo = 0, hi = nseg;  // seg_off[lo] <= i < seg_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (seg_off[mid] <= i) lo = mid; else hi = mid;
+  }
+  uint32_t c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double a = from_order_key(box[d]);
+    const double b = from_order_key(box[3 + d]);
+    const double ext = b - a;
+    double t = ext > 0.0 ? (q[3 * i + d] - a) / ext : 0.0;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    c[d] = (uint32_t)(t * 1023.0);
+  }
+  const uint32_t code = spread10(c[0]) | (spread10(c[1]) << 1) | (spread10(c[2]) << 2);
+  keys[i] = ((unsigned long long)lo << 30) | code;
+  vals[i] = (int32_t)i;
+}
+
+size_t seg_sort_temp_bytes(int64_t nq) {
+  size_t temp = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned long long*)nullptr,
+                                     (unsigned long long*)nullptr, (const int32_t*)nullptr,
+                                     (int32_t*)nullptr, (int)nq, 0, 64);
+  return temp;
+}
+
+int bits_for(int n) {
+  int b = 0;
+  while ((1ll << b) < (long long)n) ++b;
+  return b;
+}
+
+}  // namespace
+
+size_t morton_segmented_workspace_bytes(int64_t nq) {
+  size_t b = align_up(6 * sizeof(unsigned long long));
+  b += 2 * align_up((size_t)nq * sizeof(unsigned long long));
+  b += align_up((size_t)nq * sizeof(int32_t));
+  b += align_up(seg_sort_temp_bytes(nq));
+  return b;
+}
+
+hipError_t morton_order_segmented(const double* q, int64_t nq, const int64_t* seg_off, int nseg,
+                                  int32_t* perm, void* ws, size_t ws_bytes, hipStream_t stream) {
+  if (nq <= 0) return hipSuccess;
+  if (ws_bytes < morton_segmented_workspace_bytes(nq)) return hipErrorInvalidValue;
+  char* cur = (char*)ws;
+  unsigned long long* box = (unsigned long long*)cur;
+  cur += align_up(6 * sizeof(unsigned long long));
+  unsigned long long* keys_in = (unsigned long long*)cur;
+  cur += align_up((size_t)nq * sizeof(unsigned long long));
+  unsigned long long* keys_out = (unsigned long long*)cur;
+  cur += align_up((size_t)nq * sizeof(unsigned long long));
+  int32_t* vals_in = (int32_t*)cur;
+  cur += align_up((size_t)nq * sizeof(int32_t));
+  size_t temp = seg_sort_temp_bytes(nq);
+
+  bbox_init_kernel<<<1, kBlockThreads, 0, stream>>>(box);
+  int64_t blocks = (nq + kBlockThreads - 1) / kBlockThreads;
+  if (blocks > 1024) blocks = 1024;
+  bbox_kernel<<<(unsigned)blocks, kBlockThreads, 0, stream>>>(q, nq, box);
+  morton_seg_kernel<<<(unsigned)((nq + kBlockThreads - 1) / kBlockThreads), kBlockThreads, 0,
+                      stream>>>(q, nq, box, seg_off, nseg, keys_in, vals_in);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int end_bit = 30 + bits_for(nseg);
+  return hipcub::DeviceRadixSort::SortPairs(cur, temp, keys_in, keys_out, vals_in, perm,
+                                            (int)nq, 0, end_bit > 64 ? 64 : end_bit, stream);
+}
+
 }  // namespace wn

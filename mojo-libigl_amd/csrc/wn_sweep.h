@@ -1,0 +1,128 @@
+// Block-level building blocks shared by the single-mesh forward kernels and
+// the batched persistent kernel: the local-origin reduction and the sweep of
+// a face range through LDS tiles for one block's queries.
+//
+// Contract of a sweep: all 256 threads of the block call it with the same
+// face range [f0, f1) of a 9-row SoA whose rows are `stride` apart; every
+// thread accumulates its QPL queries' angle sums (unscaled) into acc[] in
+// face order. The function begins and ends with __syncthreads().
+#pragma once
+
+#include "wn_device.h"
+#include "wn_internal.h"
+
+namespace wn {
+
+struct SweepF64Smem {
+  double2 face[kFaceTile][5];
+};
+
+struct SweepF32Smem {
+  FaceRec32 face[kFaceTile];
+  double red[6][kBlockThreads / kWave];
+};
+
+template <int QPL>
+__device__ __forceinline__ void sweep_f64(const double* __restrict__ tri, int64_t stride,
+                                          int64_t f0, int64_t f1, const double (&ox)[QPL],
+                                          const double (&oy)[QPL], const double (&oz)[QPL],
+                                          double (&acc)[QPL], SweepF64Smem& sm) {
+  const int tid = threadIdx.x;
+  for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
+    const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t f = t0 + tid;
+      sm.face[tid][0] = make_double2(tri[f], tri[stride + f]);
+      sm.face[tid][1] = make_double2(tri[2 * stride + f], tri[3 * stride + f]);
+      sm.face[tid][2] = make_double2(tri[4 * stride + f], tri[5 * stride + f]);
+      sm.face[tid][3] = make_double2(tri[6 * stride + f], tri[7 * stride + f]);
+      sm.face[tid][4] = make_double2(tri[8 * stride + f], 0.0);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const double2 p0 = sm.face[j][0], p1 = sm.face[j][1], p2 = sm.face[j][2],
+                    p3 = sm.face[j][3], p4 = sm.face[j][4];
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        acc[k] += pair_f64(p0.x - ox[k], p0.y - oy[k], p1.x - oz[k], p1.y - ox[k],
+                           p2.x - oy[k], p2.y - oz[k], p3.x - ox[k], p3.y - oy[k],
+                           p4.x - oz[k]);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Centre of the bbox of the block's valid queries (valid[k] false = ignored).
+// At least one thread of the block must hold a valid query.
+template <int QPL>
+__device__ __forceinline__ void block_origin(const double (&Ox)[QPL], const double (&Oy)[QPL],
+                                             const double (&Oz)[QPL], const bool (&valid)[QPL],
+                                             SweepF32Smem& sm, double& gx, double& gy,
+                                             double& gz) {
+  const int tid = threadIdx.x;
+  double lo0 = __builtin_inf(), lo1 = __builtin_inf(), lo2 = __builtin_inf();
+  double hi0 = -__builtin_inf(), hi1 = -__builtin_inf(), hi2 = -__builtin_inf();
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    if (valid[k]) {
+      lo0 = fmin(lo0, Ox[k]); hi0 = fmax(hi0, Ox[k]);
+      lo1 = fmin(lo1, Oy[k]); hi1 = fmax(hi1, Oy[k]);
+      lo2 = fmin(lo2, Oz[k]); hi2 = fmax(hi2, Oz[k]);
+    }
+  }
+  lo0 = wave_min_f64(lo0); lo1 = wave_min_f64(lo1); lo2 = wave_min_f64(lo2);
+  hi0 = wave_max_f64(hi0); hi1 = wave_max_f64(hi1); hi2 = wave_max_f64(hi2);
+  const int wave = tid / kWave;
+  __syncthreads();
+  if ((tid & (kWave - 1)) == 0) {
+    sm.red[0][wave] = lo0; sm.red[1][wave] = lo1; sm.red[2][wave] = lo2;
+    sm.red[3][wave] = hi0; sm.red[4][wave] = hi1; sm.red[5][wave] = hi2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kBlockThreads / kWave; ++w) {
+    lo0 = fmin(lo0, sm.red[0][w]); lo1 = fmin(lo1, sm.red[1][w]); lo2 = fmin(lo2, sm.red[2][w]);
+    hi0 = fmax(hi0, sm.red[3][w]); hi1 = fmax(hi1, sm.red[4][w]); hi2 = fmax(hi2, sm.red[5][w]);
+  }
+  gx = 0.5 * (lo0 + hi0);
+  gy = 0.5 * (lo1 + hi1);
+  gz = 0.5 * (lo2 + hi2);
+}
+
+template <int QPL>
+__device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64_t stride,
+                                           int64_t f0, int64_t f1, double gx, double gy,
+                                           double gz, const float (&ox)[QPL],
+                                           const float (&oy)[QPL], const float (&oz)[QPL],
+                                           float kappa, double (&acc)[QPL], SweepF32Smem& sm) {
+  const int tid = threadIdx.x;
+  for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
+    const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t f = t0 + tid;
+      sm.face[tid] = make_face_rec32(tri[f], tri[stride + f], tri[2 * stride + f],
+                                     tri[3 * stride + f], tri[4 * stride + f],
+                                     tri[5 * stride + f], tri[6 * stride + f],
+                                     tri[7 * stride + f], tri[8 * stride + f], gx, gy, gz);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const float4 r0 = sm.face[j].r0, r1 = sm.face[j].r1, r2 = sm.face[j].r2,
+                   r3 = sm.face[j].r3;
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        bool risky;
+        const float ang = pair_rec32(r0, r1, r2, r3, ox[k], oy[k], oz[k], kappa, risky);
+        double a64 = (double)ang;
+        if (risky) a64 = pair_rec32_f64(r0, r1, r2, ox[k], oy[k], oz[k]);
+        acc[k] += a64;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace wn

@@ -38,6 +38,10 @@ _SIGNATURES = {
     "wn_forward_workspace_bytes": ([I] * 6, I),
     "wn_forward": ([I] * 9 + [D] + [I] * 3, I),
     "wn_pack_soa": ([I] * 9, I),
+    "wn_batch_create": ([I] * 7 + [D], I),
+    "wn_batch_run": ([I] * 6, I),
+    "wn_batch_info": ([I] * 2, I),
+    "wn_batch_destroy": ([I], I),
     "wn_backward": ([I] * 11, I),
     "wn_scatter_corners": ([I] * 7, I),
 }

@@ -1,0 +1,124 @@
+"""Batched winding numbers over many meshes of uneven sizes.
+
+One persistent HIP launch (optionally replayed from a hipGraph) evaluates
+every (mesh, query) pair set of the batch; see wn_batch_create in
+include/wn.h. Device tensors come from PyTorch; the work happens in the C ABI.
+
+    plan = WindingBatch(face_counts, query_counts, precision="fp32c")
+    tri = pack_batch([V0, V1, ...], [F0, F1, ...])      # 9 x F_total, device
+    w = plan.run(tri, Q_all)                             # (Q_total,) float64
+    plan.close()
+
+``winding_number_batch`` wraps the whole sequence for host arrays.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import PRECISIONS
+from ._lib import last_error, lib
+
+
+def _mode(precision) -> int:
+    return PRECISIONS[str(precision).lower()] if isinstance(precision, str) else int(precision)
+
+
+class WindingBatch:
+    """A planned batch: offsets, work list and device workspace."""
+
+    def __init__(self, face_counts, query_counts, precision="fp32c", qpl=1, chunk_faces=2048,
+                 sort=-1, kappa=-1.0):
+        fc = np.asarray(face_counts, dtype=np.int64)
+        qc = np.asarray(query_counts, dtype=np.int64)
+        if fc.ndim != 1 or fc.shape != qc.shape or len(fc) == 0:
+            raise ValueError("face_counts and query_counts must be equal-length 1-D sequences")
+        if (fc < 0).any() or (qc < 0).any():
+            raise ValueError("counts must be non-negative")
+        self.face_off = np.concatenate([[0], np.cumsum(fc)]).astype(np.int64)
+        self.query_off = np.concatenate([[0], np.cumsum(qc)]).astype(np.int64)
+        self.precision = _mode(precision)
+        self.config = dict(qpl=int(qpl), chunk_faces=int(chunk_faces), sort=int(sort), kappa=float(kappa))
+        handle = lib().wn_batch_create(self.face_off.ctypes.data, self.query_off.ctypes.data,
+                                       len(fc), self.precision, int(qpl), int(chunk_faces),
+                                       int(sort), float(kappa))
+        if handle <= 0:
+            raise RuntimeError(f"wn_batch_create failed: {last_error()}")
+        self.handle = handle
+
+    @property
+    def n_faces(self) -> int:
+        return int(self.face_off[-1])
+
+    @property
+    def n_queries(self) -> int:
+        return int(self.query_off[-1])
+
+    def info(self) -> dict:
+        out = np.zeros(6, dtype=np.int64)
+        if lib().wn_batch_info(self.handle, out.ctypes.data) != 0:
+            raise RuntimeError(last_error())
+        keys = ("items", "partials", "workspace_bytes", "grid", "faces", "queries")
+        return dict(zip(keys, (int(v) for v in out)))
+
+    def run(self, tri, Q, out=None, use_graph=True, stream=None):
+        """w for all queries. tri: (9, F_total) float64 device tensor, Q:
+        (Q_total, 3) float64 device tensor."""
+        import torch
+
+        if tri.shape != (9, self.n_faces) or tri.dtype != torch.float64 or not tri.is_contiguous():
+            raise ValueError("tri must be a contiguous (9, F_total) float64 tensor")
+        if Q.shape != (self.n_queries, 3) or Q.dtype != torch.float64 or not Q.is_contiguous():
+            raise ValueError("Q must be a contiguous (Q_total, 3) float64 tensor")
+        if out is None:
+            out = torch.empty(self.n_queries, dtype=torch.float64, device=Q.device)
+        s = (stream or torch.cuda.current_stream()).cuda_stream
+        st = lib().wn_batch_run(self.handle, tri.data_ptr(), Q.data_ptr(), out.data_ptr(),
+                                1 if use_graph else 0, s)
+        if st != 0:
+            raise RuntimeError(f"wn_batch_run failed on the GPU: {last_error()}")
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "handle", 0):
+            lib().wn_batch_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack_batch(vertices, faces, device="cuda"):
+    """Concatenate meshes (V_i, F_i) and gather them on the device into one
+    9 x F_total float64 SoA (indices offset per mesh)."""
+    import torch
+
+    from .autograd import pack_faces
+
+    Vs, Fs, off = [], [], 0
+    for V, F in zip(vertices, faces):
+        Vt = torch.as_tensor(V, device=device)
+        Ft = torch.as_tensor(F, device=device).to(torch.int64)
+        Vs.append(Vt.to(torch.float64))
+        Fs.append(Ft + off)
+        off += Vt.shape[0]
+    return pack_faces(torch.cat(Vs), torch.cat(Fs))
+
+
+def winding_number_batch(vertices, faces, queries, precision="fp32c", use_graph=False, **config):
+    """Host convenience: lists of (V_i, F_i, O_i) -> list of w_i (numpy)."""
+    import torch
+
+    tri = pack_batch(vertices, faces)
+    Q = torch.as_tensor(np.concatenate([np.asarray(o, dtype=np.float64).reshape(-1, 3)
+                                        for o in queries]), device="cuda")
+    plan = WindingBatch([len(f) for f in faces], [len(np.asarray(o).reshape(-1, 3)) for o in queries],
+                        precision=precision, **config)
+    try:
+        w = plan.run(tri, Q.contiguous(), use_graph=use_graph).cpu().numpy()
+    finally:
+        plan.close()
+    return [w[a:b] for a, b in zip(plan.query_off[:-1], plan.query_off[1:])]

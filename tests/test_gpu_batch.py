@@ -1,0 +1,124 @@
+"""Batched persistent forward (wn_batch_*) against the oracle and against
+per-mesh launches. Bars as for the single-mesh forward: FP64 within 1e-10 of
+the oracle; FP32-compensated within 1e-6 on closed meshes and 1e-5 on open
+ones (the open-cylinder meshes of the workload)."""
+
+import numpy as np
+import pytest
+
+import oracle
+import wn_workloads as W
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    B = W.config4(n_meshes=24, seed=7, max_faces=6000, max_queries=1500)
+    # edge cases: a mesh without queries and a mesh without faces
+    V, F, Q = B[3]
+    B[3] = (V, F, Q[:0])
+    V, F, Q = B[5]
+    B[5] = (V, F[:0], Q)
+    return B
+
+
+def _run(B, precision, use_graph=False, **cfg):
+    from igl.batch import winding_number_batch
+
+    return winding_number_batch([b[0] for b in B], [b[1] for b in B], [b[2] for b in B],
+                                precision=precision, use_graph=use_graph, **cfg)
+
+
+def _closed(F, nv):
+    # every edge shared by exactly two faces
+    e = np.sort(np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]]), axis=1)
+    _, counts = np.unique(e, axis=0, return_counts=True)
+    return len(F) > 0 and (counts == 2).all()
+
+
+@pytest.mark.parametrize("qpl,chunk", [(1, 256), (1, 2048), (2, 1024)])
+def test_batch_matches_oracle(require_gpu, small_batch, qpl, chunk):
+    w64 = _run(small_batch, "fp64", qpl=qpl, chunk_faces=chunk)
+    w32 = _run(small_batch, "fp32c", qpl=qpl, chunk_faces=chunk)
+    worst64 = worst32 = 0.0
+    for (V, F, Q), a, b in zip(small_batch, w64, w32):
+        assert a.shape == (len(Q),)
+        if len(Q) == 0:
+            continue
+        ref = oracle.winding(V.astype(np.float64), F, Q)
+        e64 = np.abs(a - ref).max()
+        e32 = np.abs(b - ref).max()
+        worst64, worst32 = max(worst64, e64), max(worst32, e32)
+        assert e64 <= 1e-10
+        assert e32 <= (1e-6 if _closed(F, len(V)) else 1e-5)
+    print(f"batch qpl={qpl} chunk={chunk}: worst fp64 {worst64:.2e} fp32c {worst32:.2e}", flush=True)
+
+
+def test_graph_replays_match_eager(require_gpu, small_batch):
+    from igl.batch import WindingBatch, pack_batch
+
+    B = small_batch
+    tri = pack_batch([b[0] for b in B], [b[1] for b in B])
+    Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
+    plan = WindingBatch([len(b[1]) for b in B], [len(b[2]) for b in B], precision="fp32c",
+                        chunk_faces=512)
+    try:
+        eager = plan.run(tri, Q, use_graph=False).clone()
+        for i in range(4):
+            # unrelated eager work and a sync between replays
+            torch.randn(1000, device="cuda").sum().item()
+            out = plan.run(tri, Q, use_graph=True)
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager), f"replay {i}"
+        info = plan.info()
+        assert info["items"] > 0 and info["partials"] > 0
+    finally:
+        plan.close()
+
+
+def test_batch_equals_single_mesh_launches(require_gpu, small_batch):
+    import igl
+
+    w = _run(small_batch, "fp64")
+    for (V, F, Q), a in zip(small_batch, w):
+        if len(Q) == 0:
+            continue
+        single = igl.winding_number(V, F, Q, device="gpu")
+        assert np.abs(single - a).max() <= 1e-12
+
+
+def test_invalid_plans_are_rejected(require_gpu):
+    from igl.batch import WindingBatch
+
+    with pytest.raises(RuntimeError):
+        WindingBatch([10, 20], [5, 5], chunk_faces=100)
+    with pytest.raises(RuntimeError):
+        WindingBatch([10, 20], [5, 5], qpl=4)
+    with pytest.raises(ValueError):
+        WindingBatch([10, 20], [5])
+
+
+def test_config4_full_size(require_gpu):
+    """512 meshes (6.5M faces, 5.0M queries, 7.3e10 pairs): FP32-compensated
+    against FP64 on the GPU, FP64 against the oracle on a subsample."""
+    B = W.config4()
+    w64 = _run(B, "fp64")
+    w32 = _run(B, "fp32c")
+    closed_err, open_err = 0.0, 0.0
+    rng = np.random.default_rng(0)
+    for i, ((V, F, Q), a, b) in enumerate(zip(B, w64, w32)):
+        d = np.abs(a - b).max() if len(Q) else 0.0
+        if _closed(F, len(V)):
+            closed_err = max(closed_err, d)
+        else:
+            open_err = max(open_err, d)
+        if i % 64 == 0:
+            sub = rng.choice(len(Q), min(32, len(Q)), replace=False)
+            ref = oracle.winding(V.astype(np.float64), F, Q[sub])
+            assert np.abs(a[sub] - ref).max() <= 1e-10
+    print(f"C4 fp32c vs fp64: closed meshes max {closed_err:.2e}, open meshes max {open_err:.2e}",
+          flush=True)
+    assert closed_err <= 1e-6
+    assert open_err <= 1e-5
