@@ -1,19 +1,29 @@
 #!/usr/bin/env python
 """Throughput of the generalized-winding-number hot path on one MI355X.
 
-One step = one forward pass of the FP32-compensated kernel (wn_forward,
-precision 1) over the closed torus R = 1, r = 0.35 on a 120 x 118 grid
-(28,320 faces) x 1,048,576 queries uniform in the 20 %-padded bbox, inputs
-resident in HBM. The same steps are timed for the FP64-plain kernel, the
-comparison point of the metric, and both outputs are compared.
+Default workload (the metric's configuration): one step = one forward pass of
+the FP32-compensated kernel (wn_forward, precision 1) over the closed torus
+R = 1, r = 0.35 on a 120 x 118 grid (28,320 faces) x 1,048,576 queries
+uniform in the 20 %-padded bbox, inputs resident in HBM. The same steps are
+timed for the FP64-plain kernel, the comparison point of the metric, and the
+outputs are compared with each other and, on a subsample, with the oracle.
 
 Prints ONE JSON line. `value` = G query-face pairs/s of the FP32-compensated
 kernel; `roofline` prices 66 algorithmic FLOP per pair (62 basic ops + 3 sqrt
-+ 1 atan2, counted as one each) against MI355X's published FP32 vector peak;
++ 1 atan2, one each) against MI355X's published FP32 vector peak;
 `cpu_baseline` is the FP64 oracle (a C restatement of the reference CPU path)
 on the host cores over a bounded query sample.
 
-    python bench.py [--steps K] [--warmup W] [--queries N] [--gpus 1]
+Other workloads (one JSON line each, same timing method):
+  --workload batch   512 meshes of uneven size in one persistent launch
+                     (graph replay) vs 512 separate launches
+  --workload stress  1.31M-face displaced sphere x 1M queries, error of the
+                     FP32-compensated kernel by distance to the surface
+  --workload grad    50k-face sphere x 262k queries, forward + adjoint
+  --workload readme  the reference README's GPU row: 28,322-face open grid x
+                     1,024 queries, end to end from host arrays
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
 """
 
 from __future__ import annotations
@@ -39,18 +49,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(tri, Q, seconds: float):
-    """FP64 oracle over a bounded sample of the same queries, all usable host
-    cores (at most 16, the box's CPU share), contiguous query spans."""
-    import numpy as np
-
-    import oracle
-
+def host_cores() -> int:
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    return max(1, min(cores, 16))
+
+
+def cpu_baseline(tri, Q, seconds: float):
+    """FP64 oracle over a bounded sample of the same queries, all usable host
+    cores (at most 16, the box's CPU share), contiguous query spans."""
+    import oracle
+
+    cores = host_cores()
     nf = tri.shape[1]
     probe = Q[: 64 * cores]
     t0 = time.perf_counter()
@@ -73,12 +85,324 @@ def cpu_baseline(tri, Q, seconds: float):
     }
 
 
+class Forward:
+    """wn_forward on device tensors with a resolved configuration."""
+
+    def __init__(self, h, trid, Qd, precision, qpl=0, splits=0, sort=-1, kappa=-1.0):
+        import numpy as np
+        import torch
+
+        import igl
+
+        self.h, self.trid, self.Qd, self.precision = h, trid, Qd, precision
+        self.nf, self.nq = trid.shape[1], Qd.shape[0]
+        cfg = np.zeros(4, dtype=np.int64)
+        st = h.wn_resolve_config(self.nf, self.nq, precision, qpl, splits, sort, kappa, cfg.ctypes.data)
+        assert st == 0, igl._lib.last_error()
+        self.qpl, self.splits, self.sort = int(cfg[0]), int(cfg[1]), int(cfg[2])
+        self.kappa = kappa
+        self.nbytes = h.wn_forward_workspace_bytes(self.nf, self.nq, precision, self.qpl, self.splits,
+                                                   self.sort)
+        assert self.nbytes >= 0, igl._lib.last_error()
+        self.ws = torch.empty(max(self.nbytes, 1), dtype=torch.uint8, device=Qd.device)
+        self.out = torch.empty(self.nq, dtype=torch.float64, device=Qd.device)
+        self.stream = torch.cuda.current_stream()
+
+    def __call__(self):
+        import igl
+
+        st = self.h.wn_forward(self.trid.data_ptr(), self.Qd.data_ptr(), self.out.data_ptr(), self.nf,
+                               self.nq, self.precision, self.qpl, self.splits, self.sort, self.kappa,
+                               self.ws.data_ptr(), self.nbytes, self.stream.cuda_stream)
+        if st != 0:
+            raise RuntimeError(igl._lib.last_error())
+
+
+def timed(fn, steps, warmup):
+    """Device ms/step (HIP events on torch's current stream, which is the
+    stream every call here launches on) and wall ms/step."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(s)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    end.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return start.elapsed_time(end) / steps, wall / steps * 1e3
+
+
+def base_line(args, metric, value, unit, ms, dtype, workload, **extra):
+    line = {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": {"workload": workload},
+    }
+    line.update(extra)
+    return line
+
+
+def roofline(pairs, ms, peak=FP32_VECTOR_PEAK_TFLOPS):
+    achieved = pairs * FLOP_PER_PAIR / (ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None}
+
+
+# ---------------------------------------------------------------------------
+
+
+def run_torus(args, h):
+    import numpy as np
+    import torch
+
+    import igl
+    import oracle
+    import wn_workloads as W
+
+    log("generating workload")
+    V, F, Q = W.config2(args.queries)
+    tri = igl.pack_faces(V, F)
+    nf, nq = len(F), len(Q)
+    trid = torch.tensor(tri, device="cuda")
+    Qd = torch.tensor(Q, device="cuda")
+
+    f32 = Forward(h, trid, Qd, 1, args.qpl, args.splits)
+    log(f"fp32c config qpl={f32.qpl} splits={f32.splits} sort={f32.sort}; "
+        f"{args.warmup} warm-up + {args.steps} timed steps")
+    ms32, wall32 = timed(f32, args.steps, args.warmup)
+    log(f"fp32c {ms32:.3f} ms/step (device), {wall32:.3f} ms/step (wall)")
+    w32 = f32.out.cpu().numpy()
+
+    result_fp64, acc = None, {}
+    if not args.no_fp64:
+        f64 = Forward(h, trid, Qd, 0)
+        ms64, _ = timed(f64, args.steps, args.warmup)
+        log(f"fp64 {ms64:.3f} ms/step (device)")
+        w64 = f64.out.cpu().numpy()
+        d = np.abs(w32 - w64)
+        acc = {"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw_fp32c_vs_fp64": float(d.mean()),
+               "fp64_max_dist_from_integer": float(np.abs(w64 - np.round(w64)).max())}
+        result_fp64 = {"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64, "qpl": f64.qpl,
+                       "splits": f64.splits,
+                       "roofline_frac_fp64_peak": roofline(nf * nq, ms64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
+        sub = np.arange(0, nq, max(1, nq // 512))
+        ref = oracle.winding_soa(tri, Q[sub])
+        acc["max_abs_dw_fp64_vs_oracle_sub512"] = float(np.abs(w64[sub] - ref).max())
+        acc["max_abs_dw_fp32c_vs_oracle_sub512"] = float(np.abs(w32[sub] - ref).max())
+
+    line = base_line(
+        args, METRIC, nf * nq / (ms32 * 1e-3) / 1e9, "G query-face pairs/s", ms32, "f32",
+        f"closed torus R=1 r=0.35 120x118 ({nf} faces) x {nq} queries uniform in the 20%-padded "
+        "bbox; FP32-compensated forward (FP32 pairs, FP64 accumulation, FP64 re-evaluation of "
+        "ill-conditioned pairs), inputs resident in HBM",
+        roofline=roofline(nf * nq, ms32), fp64_plain=result_fp64,
+        speedup_fp32c_vs_fp64=(result_fp64["ms_per_step"] / ms32) if result_fp64 else None,
+        accuracy=acc)
+    line["config"].update(faces=nf, queries=nq, queries_per_lane=f32.qpl, splits=f32.splits,
+                          morton_sort=bool(f32.sort), wall_ms_per_step=wall32)
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(tri, Q, args.cpu_seconds)
+    return line
+
+
+def run_batch(args, h):
+    import numpy as np
+    import torch
+
+    from igl import _autotune
+    from igl.autograd import pack_faces
+    from igl.batch import WindingBatch, pack_batch
+
+    import wn_workloads as W
+
+    log("generating 512-mesh batch")
+    B = W.config4()
+    fc = np.array([len(b[1]) for b in B])
+    qc = np.array([len(b[2]) for b in B])
+    pairs = int((fc * qc).sum())
+    tri = pack_batch([b[0] for b in B], [b[1] for b in B])
+    Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
+    log("autotuning (qpl x chunk)")
+    t0 = time.perf_counter()
+    qpl, chunk = _autotune.tune_batch(fc, qc, tri, Q, 1, use_cache=False)
+    tune_s = time.perf_counter() - t0
+    plan = WindingBatch(fc, qc, precision="fp32c", qpl=qpl, chunk_faces=chunk)
+    out = torch.empty(len(Q), dtype=torch.float64, device="cuda")
+    ms_graph, wall_graph = timed(lambda: plan.run(tri, Q, out=out, use_graph=True), args.steps,
+                                 args.warmup)
+    log(f"batched fp32c (graph) {ms_graph:.3f} ms/step; qpl={qpl} chunk={chunk}")
+    w_batch = out.cpu().numpy()
+
+    # 512 separate launches of the single-mesh forward (per-mesh SoA packed
+    # beforehand, outside the timed region)
+    singles = []
+    off = 0
+    for V, F, Qm in B:
+        tri_m = pack_faces(torch.tensor(V, device="cuda"), torch.tensor(F, device="cuda"))
+        singles.append(Forward(h, tri_m, Q[off:off + len(Qm)], 1))
+        off += len(Qm)
+
+    def all_single():
+        for f in singles:
+            f()
+
+    ms_sep, _ = timed(all_single, args.steps, args.warmup)
+    log(f"512 separate launches {ms_sep:.3f} ms/step")
+    w_sep = np.concatenate([f.out.cpu().numpy() for f in singles])
+    plan64 = WindingBatch(fc, qc, precision="fp64", qpl=1, chunk_faces=chunk)
+    w64 = plan64.run(tri, Q, use_graph=False).cpu().numpy()
+    ms64, _ = timed(lambda: plan64.run(tri, Q, out=out, use_graph=True), max(1, args.steps // 2), 1)
+    plan64.close()
+    info = plan.info()
+    plan.close()
+    d = np.abs(w_batch - w64)
+    return base_line(
+        args, "G query-face pairs/s, batched FP32-comp, 512 uneven meshes, 1 MI355X",
+        pairs / (ms_graph * 1e-3) / 1e9, "G query-face pairs/s", ms_graph, "f32",
+        f"512 meshes (torus / geodesic sphere / open cylinder with holes), {int(fc.sum())} faces, "
+        f"{int(qc.sum())} queries, {pairs} pairs; one persistent launch replayed from a hipGraph",
+        roofline=roofline(pairs, ms_graph),
+        separate_launches={"value": pairs / (ms_sep * 1e-3) / 1e9, "ms_per_step": ms_sep},
+        speedup_batched_vs_separate=ms_sep / ms_graph,
+        fp64_batched={"value": pairs / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64},
+        tuning={"qpl": qpl, "chunk_faces": chunk, "seconds": tune_s, "items": info["items"],
+                "grid": info["grid"]},
+        accuracy={"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw": float(d.mean()),
+                  "max_abs_dw_batched_vs_separate": float(np.abs(w_batch - w_sep).max())})
+
+
+def run_stress(args, h):
+    import numpy as np
+    import torch
+
+    import igl
+    import wn_workloads as W
+
+    log("generating 1.31M-face displaced sphere")
+    V, F = W.config3_mesh()
+    Q, dist = W.config3_queries(V, F, args.queries)
+    tri = igl.pack_faces(V, F)
+    nf, nq = len(F), len(Q)
+    trid = torch.tensor(tri, device="cuda")
+    Qd = torch.tensor(Q, device="cuda")
+    f32 = Forward(h, trid, Qd, 1)
+    ms32, _ = timed(f32, args.steps, min(args.warmup, 1))
+    f64 = Forward(h, trid, Qd, 0)
+    ms64, _ = timed(f64, max(1, args.steps // 2), min(args.warmup, 1))
+    w32, w64 = f32.out.cpu().numpy(), f64.out.cpu().numpy()
+    d = np.abs(w32 - w64)
+    buckets = {}
+    uni = np.isnan(dist)
+    buckets["uniform"] = {"n": int(uni.sum()), "max": float(d[uni].max()), "mean": float(d[uni].mean())}
+    mag = np.abs(dist)
+    for lo in range(-7, -2):
+        sel = (~uni) & (mag >= 10.0 ** lo) & (mag < 10.0 ** (lo + 1))
+        if sel.any():
+            buckets[f"1e{lo}..1e{lo + 1}"] = {"n": int(sel.sum()), "max": float(d[sel].max()),
+                                               "mean": float(d[sel].mean())}
+    near = ~uni
+    expect = np.where(dist[near] < 0, 1.0, 0.0)
+    wrong64 = int((np.abs(w64[near] - expect) > 0.5).sum())
+    wrong32 = int((np.abs(w32[near] - expect) > 0.5).sum())
+    return base_line(
+        args, "G query-face pairs/s, FP32-comp precision stress, 1 MI355X",
+        nf * nq / (ms32 * 1e-3) / 1e9, "G query-face pairs/s", ms32, "f32",
+        f"displaced icosphere level 8 ({nf} faces) x {nq} queries (half uniform, half at "
+        "|d| in [1e-7, 1e-2] along the normal)",
+        roofline=roofline(nf * nq, ms32),
+        fp64_plain={"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64},
+        speedup_fp32c_vs_fp64=ms64 / ms32,
+        accuracy={"max_abs_dw_fp32c_vs_fp64": float(d.max()), "by_distance": buckets,
+                  "near_surface_side_errors_fp64": wrong64, "near_surface_side_errors_fp32c": wrong32})
+
+
+def run_grad(args, h):
+    import numpy as np
+    import torch
+
+    import igl.autograd as A
+    import wn_workloads as W
+
+    V, F, Q, t = W.config5()
+    Vd = torch.tensor(V, device="cuda", requires_grad=True)
+    Fd = torch.tensor(F, device="cuda")
+    Od = torch.tensor(Q, device="cuda", requires_grad=True)
+    td = torch.tensor(t, device="cuda")
+
+    def step():
+        Vd.grad = None
+        Od.grad = None
+        w = A.winding_number(Vd, Fd, Od, "fp32c")
+        ((w - td) ** 2).sum().backward()
+
+    ms, _ = timed(step, args.steps, args.warmup)
+    pairs = len(F) * len(Q)
+    tri = A.pack_faces(Vd.detach(), Fd)
+    Qd = Od.detach().double()
+    fwd_ms, _ = timed(lambda: A.forward(tri, Qd, "fp32c"), args.steps, 1)
+    g = torch.ones(len(Q), dtype=torch.float64, device="cuda")
+    bwd_ms, _ = timed(lambda: A.backward(tri, Qd, g), args.steps, 1)
+    return base_line(
+        args, "G query-face pairs/s, forward + adjoint (dL/dO, dL/dV), 1 MI355X",
+        pairs / (ms * 1e-3) / 1e9, "G query-face pairs/s", ms, "f32+f64",
+        f"frequency-50 geodesic sphere ({len(F)} faces, float32) x {len(Q)} queries, "
+        "L = sum (w - t)^2, FP32-compensated forward + FP64 adjoint",
+        forward_ms=fwd_ms, backward_ms=bwd_ms,
+        backward_pairs_per_s=pairs / (bwd_ms * 1e-3) / 1e9,
+        gradient_max={"dL/dV": float(Vd.grad.abs().max()), "dL/dO": float(Od.grad.abs().max())})
+
+
+def run_readme(args, h):
+    import numpy as np
+
+    import igl
+    import oracle
+    import wn_workloads as W
+
+    V, F = W.bench_grid(120)
+    Q = np.random.default_rng(0).uniform([0, 0, -2], [119, 119, 2], size=(1024, 3))
+    best = {}
+    for name, kw in (("gpu_fp64", dict(device="gpu")), ("gpu_fp32c", dict(device="gpu", precision="fp32c")),
+                     ("cpu", dict(device="cpu"))):
+        igl.winding_number(V, F, Q, **kw)
+        times = []
+        for _ in range(max(3, args.steps)):
+            t0 = time.perf_counter()
+            w = igl.winding_number(V, F, Q, **kw)
+            times.append(time.perf_counter() - t0)
+        best[name] = (min(times) * 1e3, w)
+    ref = oracle.winding(V, F, Q)
+    pairs = len(F) * len(Q)
+    ms = best["gpu_fp64"][0]
+    return base_line(
+        args, "G query-face pairs/s end to end, reference README GPU row shape, 1 MI355X",
+        pairs / (ms * 1e-3) / 1e9, "G query-face pairs/s", ms, "f64",
+        f"open height-field grid(120) ({len(F)} faces) x 1024 queries, igl.winding_number from host "
+        "arrays (pack, H2D, kernel, D2H), best of runs",
+        reference_published_ms={"mojo_gpu": 53.47, "mojo_cpu_4threads": 215.09, "libigl_cpu": 138.86},
+        vs_reference_gpu_row=53.47 / ms,
+        end_to_end_ms={k: v[0] for k, v in best.items()},
+        accuracy={k: float(np.abs(v[1] - ref).max()) for k, v in best.items()})
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="torus",
+                    choices=("torus", "batch", "stress", "grad", "readme"))
     ap.add_argument("--queries", type=int, default=1_048_576)
+    ap.add_argument("--qpl", type=int, default=0, help="queries per lane (0 = library default)")
+    ap.add_argument("--splits", type=int, default=0, help="face splits (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-plain comparison")
@@ -88,129 +412,17 @@ def main() -> int:
               "(the winding-number path runs on a single MI355X here).", file=sys.stderr)
         return 2
 
-    import numpy as np
-    import torch
+    import torch  # noqa: F401  (load PyTorch's HIP runtime before the library)
 
     import igl
-    import wn_workloads as W
 
     h = igl.lib()
     if h.wn_device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
-    log("generating workload")
-    V, F, Q = W.config2(args.queries)
-    tri = igl.pack_faces(V, F)
-    nf, nq = len(F), len(Q)
-    dev = torch.device("cuda")
-    trid = torch.tensor(tri, device=dev)
-    Qd = torch.tensor(Q, device=dev)
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
-
-    def setup(precision):
-        cfg = np.zeros(4, dtype=np.int64)
-        st = h.wn_resolve_config(nf, nq, precision, 0, 0, -1, -1.0, cfg.ctypes.data)
-        assert st == 0, igl._lib.last_error()
-        qpl, splits, sort = int(cfg[0]), int(cfg[1]), int(cfg[2])
-        nbytes = h.wn_forward_workspace_bytes(nf, nq, precision, qpl, splits, sort)
-        assert nbytes >= 0, igl._lib.last_error()
-        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
-        out = torch.empty(nq, dtype=torch.float64, device=dev)
-        return dict(qpl=qpl, splits=splits, sort=sort, ws=ws, nbytes=nbytes, out=out)
-
-    def run(precision, s):
-        st = h.wn_forward(trid.data_ptr(), Qd.data_ptr(), s["out"].data_ptr(), nf, nq, precision,
-                          s["qpl"], s["splits"], s["sort"], -1.0, s["ws"].data_ptr(), s["nbytes"],
-                          sptr)
-        if st != 0:
-            raise RuntimeError(igl._lib.last_error())
-
-    def timed(precision, s, steps, warmup):
-        for _ in range(warmup):
-            run(precision, s)
-        torch.cuda.synchronize()
-        start = torch.cuda.Event(enable_timing=True)
-        end = torch.cuda.Event(enable_timing=True)
-        start.record(stream)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            run(precision, s)
-        end.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return start.elapsed_time(end) / steps, wall / steps * 1e3
-
-    s32 = setup(1)
-    log(f"fp32c config qpl={s32['qpl']} splits={s32['splits']} sort={s32['sort']}; "
-        f"{args.warmup} warm-up + {args.steps} timed steps")
-    ms32, wall32 = timed(1, s32, args.steps, args.warmup)
-    log(f"fp32c {ms32:.3f} ms/step (device), {wall32:.3f} ms/step (wall)")
-    w32 = s32["out"].cpu().numpy()
-
-    result_fp64 = None
-    acc = {}
-    if not args.no_fp64:
-        s64 = setup(0)
-        ms64, wall64 = timed(0, s64, args.steps, args.warmup)
-        log(f"fp64 {ms64:.3f} ms/step (device)")
-        w64 = s64["out"].cpu().numpy()
-        d = np.abs(w32 - w64)
-        acc = {"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw_fp32c_vs_fp64": float(d.mean()),
-               "fp64_max_dist_from_integer": float(np.abs(w64 - np.round(w64)).max())}
-        result_fp64 = {"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64,
-                       "qpl": s64["qpl"], "splits": s64["splits"],
-                       "roofline_frac_fp64_peak": nf * nq * FLOP_PER_PAIR / (ms64 * 1e-3) / 1e12
-                       / FP64_VECTOR_PEAK_TFLOPS}
-        import oracle
-
-        sub = np.arange(0, nq, max(1, nq // 512))
-        ref = oracle.winding_soa(tri, Q[sub])
-        acc["max_abs_dw_fp64_vs_oracle_sub512"] = float(np.abs(w64[sub] - ref).max())
-        acc["max_abs_dw_fp32c_vs_oracle_sub512"] = float(np.abs(w32[sub] - ref).max())
-
-    value = nf * nq / (ms32 * 1e-3) / 1e9
-    achieved = nf * nq * FLOP_PER_PAIR / (ms32 * 1e-3) / 1e12
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "G query-face pairs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms32,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"closed torus R=1 r=0.35 120x118 ({nf} faces) x {nq} queries uniform in the "
-                        "20%-padded bbox; FP32-compensated forward (FP32 pairs, FP64 accumulation, "
-                        "FP64 re-evaluation of ill-conditioned pairs), inputs resident in HBM",
-            "faces": nf,
-            "queries": nq,
-            "queries_per_lane": s32["qpl"],
-            "splits": s32["splits"],
-            "morton_sort": bool(s32["sort"]),
-            "wall_ms_per_step": wall32,
-        },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": FP32_VECTOR_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP32_VECTOR_PEAK_TFLOPS,
-            "traffic": None,
-        },
-        "fp64_plain": result_fp64,
-        "speedup_fp32c_vs_fp64": (result_fp64["ms_per_step"] / ms32) if result_fp64 else None,
-        "accuracy": acc,
-    }
-    if not args.no_cpu_baseline:
-        log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(tri, Q, args.cpu_seconds)
-    print(json.dumps(line), flush=True)
+    runner = {"torus": run_torus, "batch": run_batch, "stress": run_stress, "grad": run_grad,
+              "readme": run_readme}[args.workload]
+    print(json.dumps(runner(args, h)), flush=True)
     return 0
 
 
