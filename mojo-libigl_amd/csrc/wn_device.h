@@ -119,6 +119,56 @@ __device__ __noinline__ double pair_rec32_f64(float4 r0, float4 r1, float4 r2,
                   (double)r1.z - qx, (double)r1.w - qy, (double)r2.x - qz);
 }
 
+// atan(r) for |r| <= 0.25: r * Q(r^2), Q of degree 3 in r^2, relative-error
+// minimax on [0, 0.25] (max 1.18e-8), fitted like atan2_f32's polynomial by
+// gen_atan_coeffs.py.
+__device__ __forceinline__ float atan_small_f32(float r) {
+  const float t = r * r;
+  float p = -0.12990966888801511f;
+  p = __builtin_fmaf(p, t, 0.19950626010644533f);
+  p = __builtin_fmaf(p, t, -0.33332723769728184f);
+  p = __builtin_fmaf(p, t, 0.99999998819005977f);
+  return r * p;
+}
+
+// Geometry of one pair in FP32 (see pair_rec32): doubled det and den and the
+// doubled scale 2 la lb lc.
+struct PairGeom32 {
+  float det2, den2, s2;
+};
+
+// A pair is "far" when its angle is small and well conditioned:
+// |det| < kFarTau * den (so den > 0 and |atan argument| < kFarTau) and
+// den > kFarKappa * (2 la lb lc) (away from every edge line; a distant face
+// has den -> 4 la lb lc). A wave whose 64 pairs are all far evaluates
+// atan_small_f32(det / den) instead of the octant-reduced atan2 and skips the
+// conditioning test.
+constexpr float kFarTau = 0.25f;
+constexpr float kFarKappa = 1.0f;
+
+__device__ __forceinline__ PairGeom32 pair_geom32(const float4 r0, const float4 r1,
+                                                  const float4 r2, const float4 r3,
+                                                  float ox, float oy, float oz) {
+  const float ax = r0.x - ox, ay = r0.y - oy, az = r0.z - oz;
+  const float bx = r0.w - ox, by = r1.x - oy, bz = r1.y - oz;
+  const float cx = r1.z - ox, cy = r1.w - oy, cz = r2.x - oz;
+  const float la2 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
+  const float lb2 = __builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx));
+  const float lc2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
+  const float la = __builtin_amdgcn_sqrtf(la2);
+  const float lb = __builtin_amdgcn_sqrtf(lb2);
+  const float lc = __builtin_amdgcn_sqrtf(lc2);
+  PairGeom32 g;
+  g.det2 = __builtin_fmaf(r2.w, az, __builtin_fmaf(r2.z, ay, r2.y * ax));
+  const float sab = (la2 + lb2) - r3.x;
+  const float sbc = (lb2 + lc2) - r3.y;
+  const float sca = (lc2 + la2) - r3.z;
+  const float u2 = 2.0f * (la * lb);
+  g.s2 = u2 * lc;
+  g.den2 = __builtin_fmaf(lc, u2 + sab, __builtin_fmaf(la, sbc, lb * sca));
+  return g;
+}
+
 // FP32 pair from a record. Algebra (exact identities, cheaper and better
 // conditioned than the direct form):
 //   det = N . a              with N = (B-A) x (C-A)  (det(a,b,c) = det(a,b-a,c-a))
@@ -154,6 +204,12 @@ __device__ __forceinline__ float pair_rec32(const float4 r0, const float4 r1,
   const float ang = atan2_f32(det2, den2, mx);
   risky = mx < kappa * s2;
   return ang;
+}
+
+// True on every lane when `pred` holds on every active lane of the wave
+// (the ballot mask compared with exec: scalar work only).
+__device__ __forceinline__ bool wave_all(bool pred) {
+  return __builtin_amdgcn_ballot_w64(pred) == __builtin_amdgcn_read_exec();
 }
 
 // ---------------------------------------------------------------------------

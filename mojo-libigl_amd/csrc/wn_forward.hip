@@ -185,6 +185,23 @@ hipError_t pack_soa_launch(const void* V, const void* F, int64_t nv, int64_t nf,
 // --------------------------------------------------------------------------
 // Host launchers.
 // --------------------------------------------------------------------------
+int fwd_resident_blocks(int precision, int qpl) {
+  const void* fn = nullptr;
+  if (precision == kFP64)
+    fn = qpl == 1 ? (const void*)fwd_f64_kernel<1> : (const void*)fwd_f64_kernel<2>;
+  else
+    fn = qpl == 1 ? (const void*)fwd_f32c_kernel<1>
+                  : (qpl == 2 ? (const void*)fwd_f32c_kernel<2> : (const void*)fwd_f32c_kernel<4>);
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlockThreads, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return cus * per_cu;
+}
+
 static int64_t faces_per_split_of(int64_t nf, int splits) {
   int64_t per = (nf + splits - 1) / splits;
   per = (per + kFaceTile - 1) / kFaceTile * kFaceTile;

@@ -24,6 +24,9 @@ struct FwdConfig {
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
 // Forward pass (wn_forward.hip).
+// Blocks of the forward kernel (precision, qpl) resident at once on the
+// current device (CUs x blocks per CU), 0 if no device answers.
+int fwd_resident_blocks(int precision, int qpl);
 size_t fwd_workspace_bytes(int64_t nf, int64_t nq, const FwdConfig& cfg);
 const char* fwd_check(int64_t nf, int64_t nq, const FwdConfig& cfg);  // nullptr if ok
 hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t nf,

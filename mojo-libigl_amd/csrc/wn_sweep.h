@@ -114,8 +114,22 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
                    r3 = sm.face[j].r3;
 #pragma unroll
       for (int k = 0; k < QPL; ++k) {
-        bool risky;
-        const float ang = pair_rec32(r0, r1, r2, r3, ox[k], oy[k], oz[k], kappa, risky);
+        const PairGeom32 g = pair_geom32(r0, r1, r2, r3, ox[k], oy[k], oz[k]);
+        const bool far = __builtin_fabsf(g.det2) < kFarTau * g.den2 && g.den2 > kFarKappa * g.s2;
+        float ang;
+        bool risky = false;
+        if (wave_all(far)) {
+          // wave-uniform: every lane's angle is small and well conditioned
+          ang = atan_small_f32(g.det2 * __builtin_amdgcn_rcpf(g.den2));
+        } else {
+          // An empty volatile asm gives this path a side effect, so the
+          // compiler keeps a real (wave-uniform) branch instead of computing
+          // both paths and selecting.
+          asm volatile("" ::);
+          float mx;
+          ang = atan2_f32(g.det2, g.den2, mx);
+          risky = mx < kappa * g.s2;
+        }
         double a64 = (double)ang;
         if (risky) a64 = pair_rec32_f64(r0, r1, r2, ox[k], oy[k], oz[k]);
         acc[k] += a64;
