@@ -2,14 +2,16 @@
 """Throughput of the generalized-winding-number hot path on one MI355X.
 
 Default workload (the metric's configuration): one step = one forward pass of
-the FP32-compensated kernel (wn_forward, precision 1) over the closed torus
-R = 1, r = 0.35 on a 120 x 118 grid (28,320 faces) x 1,048,576 queries
-uniform in the 20 %-padded bbox, inputs resident in HBM. The same steps are
-timed for the FP64-plain kernel, the comparison point of the metric, and the
-outputs are compared with each other and, on a subsample, with the oracle.
+the FP32-compensated strip kernel (wn_mesh_forward on a resident igl.Mesh) over
+the closed torus R = 1, r = 0.35 on a 120 x 118 grid (28,320 faces) x
+1,048,576 queries uniform in the 20 %-padded bbox, inputs resident in HBM. The
+same steps are timed for the FP32-compensated face-record kernel (wn_forward,
+precision 1, no mesh preprocessing) and for the FP64-plain kernel, the
+comparison point of the metric; outputs are compared with each other and, on a
+subsample, with the oracle.
 
 Prints ONE JSON line. `value` = G query-face pairs/s of the FP32-compensated
-kernel; `roofline` prices 66 algorithmic FLOP per pair (62 basic ops + 3 sqrt
+strip kernel; `roofline` prices 66 algorithmic FLOP per pair (62 basic ops + 3 sqrt
 + 1 atan2, one each) against MI355X's published FP32 vector peak;
 `cpu_baseline` is the FP64 oracle (a C restatement of the reference CPU path)
 on the host cores over a bounded query sample.
@@ -22,6 +24,7 @@ Other workloads (one JSON line each, same timing method):
   --workload grad    50k-face sphere x 262k queries, forward + adjoint
   --workload readme  the reference README's GPU row: 28,322-face open grid x
                      1,024 queries, end to end from host arrays
+  --workload copy    256 MiB device copy (calibrates the byte counters)
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
 """
@@ -158,6 +161,33 @@ def roofline(pairs, ms, peak=FP32_VECTOR_PEAK_TFLOPS):
 # ---------------------------------------------------------------------------
 
 
+class MeshForward:
+    """wn_mesh_forward (strip kernel) on a resident mesh handle."""
+
+    def __init__(self, h, mesh, Qd, splits=0):
+        import torch
+
+        import igl
+
+        self.h, self.mesh, self.Qd = h, mesh, Qd
+        self.nq = Qd.shape[0]
+        self.splits, self.sort = mesh.resolve(self.nq, splits, -1)
+        self.nbytes = h.wn_mesh_workspace_bytes(mesh.handle, self.nq, self.splits, self.sort)
+        assert self.nbytes >= 0, igl._lib.last_error()
+        self.ws = torch.empty(max(self.nbytes, 1), dtype=torch.uint8, device=Qd.device)
+        self.out = torch.empty(self.nq, dtype=torch.float64, device=Qd.device)
+        self.stream = torch.cuda.current_stream()
+
+    def __call__(self):
+        import igl
+
+        st = self.h.wn_mesh_forward(self.mesh.handle, self.Qd.data_ptr(), self.out.data_ptr(),
+                                    self.nq, self.splits, self.sort, -1.0, self.ws.data_ptr(),
+                                    self.nbytes, self.stream.cuda_stream)
+        if st != 0:
+            raise RuntimeError(igl._lib.last_error())
+
+
 def run_torus(args, h):
     import numpy as np
     import torch
@@ -165,6 +195,7 @@ def run_torus(args, h):
     import igl
     import oracle
     import wn_workloads as W
+    from igl.mesh import Mesh
 
     log("generating workload")
     V, F, Q = W.config2(args.queries)
@@ -173,21 +204,31 @@ def run_torus(args, h):
     trid = torch.tensor(tri, device="cuda")
     Qd = torch.tensor(Q, device="cuda")
 
-    f32 = Forward(h, trid, Qd, 1, args.qpl, args.splits)
-    log(f"fp32c config qpl={f32.qpl} splits={f32.splits} sort={f32.sort}; "
-        f"{args.warmup} warm-up + {args.steps} timed steps")
-    ms32, wall32 = timed(f32, args.steps, args.warmup)
-    log(f"fp32c {ms32:.3f} ms/step (device), {wall32:.3f} ms/step (wall)")
-    w32 = f32.out.cpu().numpy()
+    t0 = time.perf_counter()
+    mesh = Mesh(V, F)
+    create_ms = (time.perf_counter() - t0) * 1e3
+    info = mesh.info()
+    strip = MeshForward(h, mesh, Qd, args.splits)
+    log(f"strip path: {info['records']} records for {nf} faces (built in {create_ms:.1f} ms), "
+        f"splits={strip.splits} sort={strip.sort}; {args.warmup} warm-up + {args.steps} timed steps")
+    ms_strip, wall_strip = timed(strip, args.steps, args.warmup)
+    log(f"fp32c strip {ms_strip:.3f} ms/step (device), {wall_strip:.3f} ms/step (wall)")
+    w_strip = strip.out.cpu().numpy()
+
+    face = Forward(h, trid, Qd, 1, args.qpl, args.splits)
+    ms_face, _ = timed(face, args.steps, args.warmup)
+    log(f"fp32c face-record kernel {ms_face:.3f} ms/step (qpl={face.qpl} splits={face.splits})")
+    w_face = face.out.cpu().numpy()
 
     result_fp64, acc = None, {}
     if not args.no_fp64:
         f64 = Forward(h, trid, Qd, 0)
         ms64, _ = timed(f64, args.steps, args.warmup)
-        log(f"fp64 {ms64:.3f} ms/step (device)")
+        log(f"fp64 plain {ms64:.3f} ms/step (device)")
         w64 = f64.out.cpu().numpy()
-        d = np.abs(w32 - w64)
+        d = np.abs(w_strip - w64)
         acc = {"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw_fp32c_vs_fp64": float(d.mean()),
+               "max_abs_dw_face_kernel_vs_fp64": float(np.abs(w_face - w64).max()),
                "fp64_max_dist_from_integer": float(np.abs(w64 - np.round(w64)).max())}
         result_fp64 = {"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64, "qpl": f64.qpl,
                        "splits": f64.splits,
@@ -195,18 +236,22 @@ def run_torus(args, h):
         sub = np.arange(0, nq, max(1, nq // 512))
         ref = oracle.winding_soa(tri, Q[sub])
         acc["max_abs_dw_fp64_vs_oracle_sub512"] = float(np.abs(w64[sub] - ref).max())
-        acc["max_abs_dw_fp32c_vs_oracle_sub512"] = float(np.abs(w32[sub] - ref).max())
+        acc["max_abs_dw_fp32c_vs_oracle_sub512"] = float(np.abs(w_strip[sub] - ref).max())
+    mesh.close()
 
     line = base_line(
-        args, METRIC, nf * nq / (ms32 * 1e-3) / 1e9, "G query-face pairs/s", ms32, "f32",
+        args, METRIC, nf * nq / (ms_strip * 1e-3) / 1e9, "G query-face pairs/s", ms_strip, "f32",
         f"closed torus R=1 r=0.35 120x118 ({nf} faces) x {nq} queries uniform in the 20%-padded "
         "bbox; FP32-compensated forward (FP32 pairs, FP64 accumulation, FP64 re-evaluation of "
-        "ill-conditioned pairs), inputs resident in HBM",
-        roofline=roofline(nf * nq, ms32), fp64_plain=result_fp64,
-        speedup_fp32c_vs_fp64=(result_fp64["ms_per_step"] / ms32) if result_fp64 else None,
+        "ill-conditioned pairs) over the resident triangle-strip mesh, inputs resident in HBM",
+        roofline=roofline(nf * nq, ms_strip), fp64_plain=result_fp64,
+        speedup_fp32c_vs_fp64=(result_fp64["ms_per_step"] / ms_strip) if result_fp64 else None,
+        fp32c_face_kernel={"value": nf * nq / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face,
+                           "qpl": face.qpl, "splits": face.splits},
         accuracy=acc)
-    line["config"].update(faces=nf, queries=nq, queries_per_lane=f32.qpl, splits=f32.splits,
-                          morton_sort=bool(f32.sort), wall_ms_per_step=wall32)
+    line["config"].update(faces=nf, queries=nq, strip_records=info["records"], splits=strip.splits,
+                          morton_sort=bool(strip.sort), mesh_build_ms=create_ms,
+                          wall_ms_per_step=wall_strip)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(tri, Q, args.cpu_seconds)
@@ -393,13 +438,28 @@ def run_readme(args, h):
         accuracy={k: float(np.abs(v[1] - ref).max()) for k, v in best.items()})
 
 
+def run_copy(args, h):
+    """Device-to-device copy of a known size (256 MiB read + 256 MiB written
+    per launch): the reference point for scaling the FETCH_SIZE / WRITE_SIZE
+    counters of the winding kernels to bytes."""
+    import torch
+
+    n = 256 * 2**20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    ms, _ = timed(lambda: dst.copy_(src), args.steps, args.warmup)
+    return base_line(args, "GB/s device copy (counter calibration)", 2 * n / (ms * 1e-3) / 1e9, "GB/s",
+                     ms, "f32", f"torch copy_ of {n} bytes (read {n} + write {n} bytes per launch)",
+                     bytes_read_per_launch=n, bytes_written_per_launch=n)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="torus",
-                    choices=("torus", "batch", "stress", "grad", "readme"))
+                    choices=("torus", "batch", "stress", "grad", "readme", "copy"))
     ap.add_argument("--queries", type=int, default=1_048_576)
     ap.add_argument("--qpl", type=int, default=0, help="queries per lane (0 = library default)")
     ap.add_argument("--splits", type=int, default=0, help="face splits (0 = library default)")
@@ -421,7 +481,7 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
     runner = {"torus": run_torus, "batch": run_batch, "stress": run_stress, "grad": run_grad,
-              "readme": run_readme}[args.workload]
+              "readme": run_readme, "copy": run_copy}[args.workload]
     print(json.dumps(runner(args, h)), flush=True)
     return 0
 

@@ -99,6 +99,47 @@ int64_t wn_pack_soa(const void *V, const void *F, int64_t nv, int64_t nf,
                     int64_t vdtype, int64_t fdtype, double *tri,
                     int32_t *bad_index, void *stream);
 
+/* ---- Device-resident mesh (triangle strips) ------------------------------ */
+
+/* No reference counterpart (the reference re-packs and re-uploads the mesh on
+ * every call, python/igl/__init__.py:248-250, src/capi.mojo:371-376). Turns
+ * (host) V (nv x 3 float64) and F (nf x 3; fdtype 0 = int64, 1 = int32) into
+ * triangle-strip records, 4 doubles each (x, y, z, flag), in tiles of 255 that
+ * each start a strip; flag 0 = no face ends at the record, +1 / -1 = the
+ * record closes face (w[k-2], w[k-1], w[k]) with that orientation relative to
+ * the face's own corner order. Writes them to (host) out when capacity (in
+ * records) suffices; returns the record count, or a negative status. */
+int64_t wn_mesh_records(const double *V, const void *F, int64_t nv, int64_t nf,
+                        int64_t fdtype, double *out, int64_t capacity);
+
+/* Builds the strip records of (host) V, F and uploads them (the only
+ * allocation of the mesh path). Returns a handle (> 0) or a negative status
+ * (-1 also for a face index outside [0, nv)). */
+int64_t wn_mesh_create(const double *V, const void *F, int64_t nv, int64_t nf,
+                       int64_t fdtype);
+
+/* out3 = {faces, records, tiles}. */
+int64_t wn_mesh_info(int64_t mesh, int64_t *out3);
+
+/* Resolved {splits, sort} for nq queries (splits 0 / sort -1 = default). */
+int64_t wn_mesh_resolve(int64_t mesh, int64_t nq, int64_t splits, int64_t sort,
+                        int64_t *out2);
+
+/* Device workspace bytes of wn_mesh_forward. */
+int64_t wn_mesh_workspace_bytes(int64_t mesh, int64_t nq, int64_t splits,
+                                int64_t sort);
+
+/* FP32-compensated forward over the mesh's strips for (device) q (nq x 3
+ * float64) into (device) dst (nq). Same arithmetic and accuracy contract as
+ * wn_forward precision 1; one vertex offset, length and square root per face
+ * instead of three. Asynchronous on `stream`; allocates nothing. */
+int64_t wn_mesh_forward(int64_t mesh, const double *q, double *dst, int64_t nq,
+                        int64_t splits, int64_t sort, double kappa,
+                        void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Frees a mesh (synchronises the device first). */
+int64_t wn_mesh_destroy(int64_t mesh);
+
 /* ---- Batched forward ------------------------------------------------------ */
 
 /* No reference counterpart (the reference evaluates one mesh per call,

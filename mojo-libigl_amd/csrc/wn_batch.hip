@@ -86,9 +86,10 @@ __global__ __launch_bounds__(kBlockThreads) void batch_kernel(
       const int64_t slot = slot0 + k * kBlockThreads + tid;
       valid[k] = slot < slot_end;
       qidx[k] = valid[k] ? (perm ? (int64_t)perm[slot] : slot) : 0;
-      Ox[k] = valid[k] ? q[3 * qidx[k]] : 0.0;
-      Oy[k] = valid[k] ? q[3 * qidx[k] + 1] : 0.0;
-      Oz[k] = valid[k] ? q[3 * qidx[k] + 2] : 0.0;
+      // q is in launch order (gathered through perm beforehand when sorted)
+      Ox[k] = valid[k] ? q[3 * slot] : 0.0;
+      Oy[k] = valid[k] ? q[3 * slot + 1] : 0.0;
+      Oz[k] = valid[k] ? q[3 * slot + 2] : 0.0;
       acc[k] = 0.0;
     }
     if constexpr (PREC == kFP64) {
@@ -108,32 +109,35 @@ __global__ __launch_bounds__(kBlockThreads) void batch_kernel(
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
       if (valid[k]) {
+        const int64_t slot = slot0 + k * kBlockThreads + tid;
         if (m.chunks == 1)
           dst[qidx[k]] = acc[k] * kInv2Pi;
-        else
-          partial[m.pbase + (qidx[k] - m.query_off) * m.chunks + item.chunk] = acc[k];
+        else  // chunk-major, slot order: a block's writes are contiguous
+          partial[m.pbase + (int64_t)item.chunk * m.nq + (slot - m.query_off)] = acc[k];
       }
     }
   }
 }
 
+// One thread per query slot: sums its mesh's chunk partials in chunk order and
+// writes w through the query permutation.
 __global__ __launch_bounds__(kBlockThreads) void batch_reduce_kernel(
-    const double* __restrict__ partial, double* __restrict__ dst,
-    const BatchMesh* __restrict__ meshes, const int64_t* __restrict__ query_off, int n_meshes,
-    int64_t qtot) {
-  const int64_t qi = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
-  if (qi >= qtot) return;
-  int lo = 0, hi = n_meshes;  // query_off[lo] <= qi < query_off[hi]
+    const double* __restrict__ partial, const int32_t* __restrict__ perm,
+    double* __restrict__ dst, const BatchMesh* __restrict__ meshes,
+    const int64_t* __restrict__ query_off, int n_meshes, int64_t qtot) {
+  const int64_t slot = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (slot >= qtot) return;
+  int lo = 0, hi = n_meshes;  // query_off[lo] <= slot < query_off[hi]
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (query_off[mid] <= qi) lo = mid; else hi = mid;
+    if (query_off[mid] <= slot) lo = mid; else hi = mid;
   }
   const BatchMesh m = meshes[lo];
   if (m.chunks <= 1) return;
-  const double* p = partial + m.pbase + (qi - m.query_off) * m.chunks;
+  const double* p = partial + m.pbase + (slot - m.query_off);
   double s = 0.0;
-  for (int k = 0; k < m.chunks; ++k) s += p[k];
-  dst[qi] = s * kInv2Pi;
+  for (int k = 0; k < m.chunks; ++k) s += p[(int64_t)k * m.nq];
+  dst[perm ? (int64_t)perm[slot] : slot] = s * kInv2Pi;
 }
 
 }  // namespace
@@ -161,6 +165,7 @@ struct BatchPlan {
   int64_t* d_query_off = nullptr;
   double* d_partial = nullptr;
   int32_t* d_perm = nullptr;
+  double* d_qs = nullptr;  // queries in launch order (sorted plans)
   void* d_sort_ws = nullptr;
   size_t sort_bytes = 0;
   // graph cache
@@ -260,8 +265,9 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
   const size_t b_qoff = align_up(sizeof(int64_t) * (size_t)(n_meshes + 1));
   const size_t b_partial = align_up(sizeof(double) * (size_t)std::max<int64_t>(1, pbase));
   const size_t b_perm = sort ? align_up(sizeof(int32_t) * (size_t)std::max<int64_t>(1, p->qtot)) : 0;
+  const size_t b_qs = sort ? align_up(3 * sizeof(double) * (size_t)std::max<int64_t>(1, p->qtot)) : 0;
   p->sort_bytes = sort ? morton_segmented_workspace_bytes(p->qtot) : 0;
-  p->ws_bytes = b_counter + b_meshes + b_items + b_qoff + b_partial + b_perm + p->sort_bytes;
+  p->ws_bytes = b_counter + b_meshes + b_items + b_qoff + b_partial + b_perm + b_qs + p->sort_bytes;
   e = hipMalloc(&p->ws, p->ws_bytes);
   if (e != hipSuccess) {
     delete p;
@@ -275,6 +281,7 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
   p->d_partial = (double*)c; c += b_partial;
   if (sort) {
     p->d_perm = (int32_t*)c; c += b_perm;
+    p->d_qs = (double*)c; c += b_qs;
     p->d_sort_ws = c;
   }
   e = hipMemcpy(p->d_meshes, meshes.data(), sizeof(BatchMesh) * meshes.size(), hipMemcpyHostToDevice);
@@ -297,6 +304,7 @@ static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q
                                 hipStream_t s) {
   counter_reset_kernel<<<1, 64, 0, s>>>(p->d_counter);
   const int32_t* perm = p->sort ? p->d_perm : nullptr;
+  if (p->sort) q = p->d_qs;
   const dim3 grid((unsigned)p->grid);
 #define WN_BATCH_LAUNCH(PREC, QPL)                                                             \
   batch_kernel<PREC, QPL><<<grid, kBlockThreads, 0, s>>>(tri, p->ftot, q, perm, dst,          \
@@ -312,8 +320,8 @@ static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q
 #undef WN_BATCH_LAUNCH
   if (p->n_partials > 0 && p->qtot > 0)
     batch_reduce_kernel<<<dim3((unsigned)((p->qtot + kBlockThreads - 1) / kBlockThreads)),
-                          kBlockThreads, 0, s>>>(p->d_partial, dst, p->d_meshes, p->d_query_off,
-                                                 p->n_meshes, p->qtot);
+                          kBlockThreads, 0, s>>>(p->d_partial, perm, dst, p->d_meshes,
+                                                 p->d_query_off, p->n_meshes, p->qtot);
   return hipGetLastError();
 }
 
@@ -324,6 +332,7 @@ hipError_t batch_run(BatchPlan* p, const double* tri, const double* q, double* d
   if (p->sort) {
     e = morton_order_segmented(q, p->qtot, p->d_query_off, p->n_meshes, p->d_perm,
                                p->d_sort_ws, p->sort_bytes, stream);
+    if (e == hipSuccess) e = gather_queries_launch(q, p->d_perm, p->qtot, p->d_qs, stream);
     if (e != hipSuccess) return e;
   }
   if (!use_graph) return batch_enqueue(p, tri, q, dst, stream);

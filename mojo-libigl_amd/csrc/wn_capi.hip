@@ -232,6 +232,29 @@ wn::BatchPlan* lookup_batch(int64_t handle) {
   return it == batch_registry().end() ? nullptr : *it;
 }
 
+// A device-resident mesh in strip-record form (wn_mesh_*).
+struct MeshHandle {
+  int64_t nf = 0;
+  int64_t nrec = 0;
+  void* d_rec = nullptr;
+};
+
+std::mutex& mesh_registry_mu() {
+  static std::mutex mu;
+  return mu;
+}
+
+std::set<MeshHandle*>& mesh_registry() {
+  static std::set<MeshHandle*> live;
+  return live;
+}
+
+MeshHandle* lookup_mesh(int64_t handle) {
+  std::lock_guard<std::mutex> lock(mesh_registry_mu());
+  auto it = mesh_registry().find((MeshHandle*)(intptr_t)handle);
+  return it == mesh_registry().end() ? nullptr : *it;
+}
+
 }  // namespace
 
 extern "C" {
@@ -385,6 +408,124 @@ int64_t wn_batch_destroy(int64_t handle) {
     batch_registry().erase(it);
   }
   wn::batch_destroy(plan);
+  return kOk;
+}
+
+int64_t wn_mesh_records(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype,
+                        double* out, int64_t capacity) {
+  if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if (nf > 0 && (!V || !F)) return fail(kErrArg, "null host buffer");
+  for (int64_t i = 0; i < 3 * nf; ++i) {
+    const int64_t v = fdtype == 0 ? ((const int64_t*)F)[i] : (int64_t)((const int32_t*)F)[i];
+    if (v < 0 || v >= nv) return fail(kErrArg, "face index is outside V");
+  }
+  std::vector<double> rec;
+  const int64_t n = wn::strip_records(V, F, nf, (int)fdtype, rec);
+  if (out && capacity >= n) std::memcpy(out, rec.data(), rec.size() * sizeof(double));
+  return n;
+}
+
+int64_t wn_mesh_create(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype) {
+  if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if (nf > 0 && (!V || !F)) return fail(kErrArg, "null host buffer");
+  for (int64_t i = 0; i < 3 * nf; ++i) {
+    const int64_t v = fdtype == 0 ? ((const int64_t*)F)[i] : (int64_t)((const int32_t*)F)[i];
+    if (v < 0 || v >= nv) return fail(kErrArg, "face index is outside V");
+  }
+  if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  std::vector<double> rec;
+  MeshHandle* m = new MeshHandle();
+  m->nf = nf;
+  m->nrec = wn::strip_records(V, F, nf, (int)fdtype, rec);
+  if (m->nrec > 0) {
+    hipError_t e = hipMalloc(&m->d_rec, rec.size() * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMemcpy(m->d_rec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (m->d_rec) (void)hipFree(m->d_rec);
+      delete m;
+      return fail(kErrHip, hip_msg("wn_mesh_create", e));
+    }
+  }
+  std::lock_guard<std::mutex> lock(mesh_registry_mu());
+  mesh_registry().insert(m);
+  return (int64_t)(intptr_t)m;
+}
+
+int64_t wn_mesh_info(int64_t mesh, int64_t* out3) {
+  MeshHandle* m = lookup_mesh(mesh);
+  if (!m) return fail(kErrArg, "unknown mesh handle");
+  if (!out3) return fail(kErrArg, "null output");
+  out3[0] = m->nf;
+  out3[1] = m->nrec;
+  out3[2] = m->nrec / 255;
+  return kOk;
+}
+
+int64_t wn_mesh_resolve(int64_t mesh, int64_t nq, int64_t splits, int64_t sort, int64_t* out2) {
+  MeshHandle* m = lookup_mesh(mesh);
+  if (!m) return fail(kErrArg, "unknown mesh handle");
+  if (nq < 0 || nq > (int64_t)0x7fffffff) return fail(kErrArg, "query count out of range");
+  int64_t s = splits;
+  if (s <= 0) {
+    static int resident = -1;
+    if (resident < 0) resident = device_count() > 0 ? wn::strip_resident_blocks() : 0;
+    const int64_t qblocks = std::max<int64_t>(1, (nq + 255) / 256);
+    // best_splits works in units of faces per 256-face tile; records play that role here
+    s = best_splits(std::max<int64_t>(m->nrec, 1), qblocks, resident > 0 ? resident : 2048, nullptr);
+  }
+  if (s < 1 || s > 65535) return fail(kErrArg, "splits must be in [1, 65535]");
+  const int64_t srt = sort < 0 ? (nq >= 4096 ? 1 : 0) : (sort ? 1 : 0);
+  if (out2) {
+    out2[0] = s;
+    out2[1] = srt;
+  }
+  return kOk;
+}
+
+int64_t wn_mesh_workspace_bytes(int64_t mesh, int64_t nq, int64_t splits, int64_t sort) {
+  int64_t cfg[2];
+  const int64_t st = wn_mesh_resolve(mesh, nq, splits, sort, cfg);
+  if (st != kOk) return st;
+  if (cfg[1] && device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  return (int64_t)wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+}
+
+int64_t wn_mesh_forward(int64_t mesh, const double* q, double* dst, int64_t nq, int64_t splits,
+                        int64_t sort, double kappa, void* workspace, int64_t workspace_bytes,
+                        void* stream) {
+  int64_t cfg[2];
+  const int64_t st = wn_mesh_resolve(mesh, nq, splits, sort, cfg);
+  if (st != kOk) return st;
+  if (nq == 0) return kOk;
+  if (!q || !dst) return fail(kErrArg, "null device buffer");
+  MeshHandle* m = lookup_mesh(mesh);
+  const size_t need = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+  if (workspace_bytes < 0 || (size_t)workspace_bytes < need || (need > 0 && !workspace))
+    return fail(kErrWorkspace, "workspace smaller than wn_mesh_workspace_bytes()");
+  hipError_t e = wn::strip_launch((const double4*)m->d_rec, m->nrec, q, dst, nq, (int)cfg[0],
+                                  (int)cfg[1], kappa < 0.0 ? 1.0f : (float)kappa, workspace,
+                                  (size_t)workspace_bytes, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_forward", e));
+  return kOk;
+}
+
+int64_t wn_mesh_destroy(int64_t mesh) {
+  MeshHandle* m = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mesh_registry_mu());
+    auto it = mesh_registry().find((MeshHandle*)(intptr_t)mesh);
+    if (it == mesh_registry().end()) return fail(kErrArg, "unknown mesh handle");
+    m = *it;
+    mesh_registry().erase(it);
+  }
+  if (m->d_rec) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(m->d_rec);
+  }
+  delete m;
   return kOk;
 }
 

@@ -6,10 +6,13 @@
 // and every lane sweeps each tile with all its queries. Accumulation stays in
 // registers in fixed face order, so results are bitwise reproducible.
 //
-// Split-K: grid.y splits the face range. Each (block, split) writes an
-// unscaled partial to partial[q * splits + s]; reduce_splits sums the splits
-// of one query in one wave (DPP/ds_swizzle tree, fixed order) and applies
-// 1/(2 pi). With splits == 1 the kernel writes dst directly.
+// Split-K: grid.y splits the face range. Each (block, split) writes unscaled
+// partials to partial[s * nq + slot] (slot = the query's position in launch
+// order, so a block's writes are contiguous); the reduce pass sums the splits
+// of a slot in fixed order -- one thread per slot for up to 16 splits, one
+// wave per slot with a DPP/ds_swizzle tree beyond -- applies 1/(2 pi) and
+// writes dst through the query permutation. With splits == 1 the kernel
+// writes dst directly.
 //
 // The reference instead stores an nf x nq FP64 angle buffer and reduces it
 // serially per query (src/capi.mojo:297-363, 366-398); nothing of nf*nq size
@@ -54,7 +57,7 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f64_kernel(
       if (splits == 1)
         out[qi] = acc[k] * kInv2Pi;
       else
-        out[qi * splits + blockIdx.y] = acc[k];
+        out[(int64_t)blockIdx.y * nq + qi] = acc[k];
     }
   }
 }
@@ -90,9 +93,10 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f32c_kernel(
     const int64_t slot = qbase + k * kBlockThreads + tid;
     valid[k] = slot < nq;
     qidx[k] = valid[k] ? (perm ? (int64_t)perm[slot] : slot) : 0;
-    Ox[k] = valid[k] ? q[3 * qidx[k]] : 0.0;
-    Oy[k] = valid[k] ? q[3 * qidx[k] + 1] : 0.0;
-    Oz[k] = valid[k] ? q[3 * qidx[k] + 2] : 0.0;
+    // q is in launch order (gathered through perm beforehand when sorted)
+    Ox[k] = valid[k] ? q[3 * slot] : 0.0;
+    Oy[k] = valid[k] ? q[3 * slot + 1] : 0.0;
+    Oz[k] = valid[k] ? q[3 * slot + 2] : 0.0;
   }
   double gx, gy, gz;
   block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz);
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f32c_kernel(
       if (splits == 1)
         out[qidx[k]] = acc[k] * kInv2Pi;
       else
-        out[qidx[k] * splits + blockIdx.y] = acc[k];
+        out[(int64_t)blockIdx.y * nq + qbase + k * kBlockThreads + tid] = acc[k];
     }
   }
 }
@@ -127,17 +131,30 @@ __global__ __launch_bounds__(kBlockThreads) void fill_f64_kernel(double* __restr
   if (i < n) p[i] = v;
 }
 
-// One wave per query: lanes stride over the splits, then a fixed DPP tree.
+// Reduction of split partials partial[s * nq + slot] -> dst[perm[slot]].
+// Few splits: one thread per slot, splits summed in order.
 __global__ __launch_bounds__(kBlockThreads) void reduce_splits_kernel(
-    const double* __restrict__ partial, double* __restrict__ dst, int64_t nq,
-    int splits) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t qi = (int64_t)blockIdx.x * (kBlockThreads / kWave) + threadIdx.x / kWave;
-  if (qi >= nq) return;  // wave-uniform
+    const double* __restrict__ partial, const int32_t* __restrict__ perm,
+    double* __restrict__ dst, int64_t nq, int splits) {
+  const int64_t slot = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (slot >= nq) return;
   double s = 0.0;
-  for (int j = lane; j < splits; j += kWave) s += partial[qi * splits + j];
+  for (int j = 0; j < splits; ++j) s += partial[(int64_t)j * nq + slot];
+  dst[perm ? (int64_t)perm[slot] : slot] = s * kInv2Pi;
+}
+
+// Many splits: one wave per slot, lanes stride over the splits, then a fixed
+// DPP/ds_swizzle tree (wave_sum_f64); lane 0 writes.
+__global__ __launch_bounds__(kBlockThreads) void reduce_splits_wave_kernel(
+    const double* __restrict__ partial, const int32_t* __restrict__ perm,
+    double* __restrict__ dst, int64_t nq, int splits) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t slot = (int64_t)blockIdx.x * (kBlockThreads / kWave) + threadIdx.x / kWave;
+  if (slot >= nq) return;  // wave-uniform
+  double s = 0.0;
+  for (int j = lane; j < splits; j += kWave) s += partial[(int64_t)j * nq + slot];
   s = wave_sum_f64(s);
-  if (lane == 0) dst[qi] = s * kInv2Pi;
+  if (lane == 0) dst[perm ? (int64_t)perm[slot] : slot] = s * kInv2Pi;
 }
 
 // --------------------------------------------------------------------------
@@ -185,6 +202,27 @@ hipError_t pack_soa_launch(const void* V, const void* F, int64_t nv, int64_t nf,
 // --------------------------------------------------------------------------
 // Host launchers.
 // --------------------------------------------------------------------------
+hipError_t fill_launch(double* p, int64_t n, double v, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  fill_f64_kernel<<<dim3((unsigned)((n + kBlockThreads - 1) / kBlockThreads)), kBlockThreads, 0,
+                    stream>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t reduce_splits_launch(const double* partial, const int32_t* perm, double* dst,
+                                int64_t nq, int splits, hipStream_t stream) {
+  if (nq <= 0) return hipSuccess;
+  if (splits <= 16) {
+    reduce_splits_kernel<<<dim3((unsigned)((nq + kBlockThreads - 1) / kBlockThreads)),
+                           kBlockThreads, 0, stream>>>(partial, perm, dst, nq, splits);
+  } else {
+    const int per_block = kBlockThreads / kWave;
+    reduce_splits_wave_kernel<<<dim3((unsigned)((nq + per_block - 1) / per_block)),
+                                kBlockThreads, 0, stream>>>(partial, perm, dst, nq, splits);
+  }
+  return hipGetLastError();
+}
+
 int fwd_resident_blocks(int precision, int qpl) {
   const void* fn = nullptr;
   if (precision == kFP64)
@@ -227,6 +265,7 @@ size_t fwd_workspace_bytes(int64_t nf, int64_t nq, const FwdConfig& cfg) {
   if (cfg.splits > 1) bytes += align_up((size_t)nq * (size_t)cfg.splits * sizeof(double));
   if (cfg.precision == kFP32C && cfg.sort) {
     bytes += align_up((size_t)nq * sizeof(int32_t));
+    bytes += align_up((size_t)nq * 3 * sizeof(double));
     bytes += morton_workspace_bytes(nq);
   }
   return bytes;
@@ -247,8 +286,12 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
   if (cfg.precision == kFP32C && cfg.sort) {
     perm = (int32_t*)cursor;
     cursor += align_up((size_t)nq * sizeof(int32_t));
+    double* qs = (double*)cursor;
+    cursor += align_up((size_t)nq * 3 * sizeof(double));
     hipError_t e = morton_order(q, nq, perm, cursor, morton_workspace_bytes(nq), stream);
+    if (e == hipSuccess) e = gather_queries_launch(q, perm, nq, qs, stream);
     if (e != hipSuccess) return e;
+    q = qs;
   }
   double* out = cfg.splits > 1 ? partial : dst;
   const int64_t fps = faces_per_split_of(nf, cfg.splits);
@@ -279,12 +322,7 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (cfg.splits > 1) {
-    const int per_block = kBlockThreads / kWave;
-    const dim3 rgrid((unsigned)((nq + per_block - 1) / per_block));
-    reduce_splits_kernel<<<rgrid, kBlockThreads, 0, stream>>>(partial, dst, nq, cfg.splits);
-    e = hipGetLastError();
-  }
+  if (cfg.splits > 1) e = reduce_splits_launch(partial, perm, dst, nq, cfg.splits, stream);
   return e;
 }
 

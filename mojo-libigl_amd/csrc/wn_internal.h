@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace wn {
 
 enum Precision : int { kFP64 = 0, kFP32C = 1 };
@@ -23,6 +25,22 @@ struct FwdConfig {
 // Rounds to 256 bytes so sub-buffers stay aligned.
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
+// p[0:n] = v; dst[perm[slot]] = sum over splits of partial[s * nq + slot] / (2 pi)
+// (perm may be null: identity).
+hipError_t fill_launch(double* p, int64_t n, double v, hipStream_t stream);
+hipError_t reduce_splits_launch(const double* partial, const int32_t* perm, double* dst,
+                                int64_t nq, int splits, hipStream_t stream);
+
+// Strip forward (wn_strips.cpp builds records on the host, wn_strip.hip runs
+// them). Records: 4 doubles each (x, y, z, flag), a multiple of 255 per mesh.
+int64_t strip_records(const double* V, const void* F, int64_t nf, int fdtype,
+                      std::vector<double>& out);
+int strip_resident_blocks();
+size_t strip_workspace_bytes(int64_t nq, int splits, int sort);
+hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, double* dst,
+                        int64_t nq, int splits, int sort, float kappa, void* ws, size_t ws_bytes,
+                        hipStream_t stream);
+
 // Forward pass (wn_forward.hip).
 // Blocks of the forward kernel (precision, qpl) resident at once on the
 // current device (CUs x blocks per CU), 0 if no device answers.
@@ -37,6 +55,10 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
 size_t morton_workspace_bytes(int64_t nq);
 hipError_t morton_order(const double* q, int64_t nq, int32_t* perm, void* ws,
                         size_t ws_bytes, hipStream_t stream);
+// qs[slot] = q[perm[slot]] (nq x 3 float64): queries in launch order, so that
+// kernels read them contiguously.
+hipError_t gather_queries_launch(const double* q, const int32_t* perm, int64_t nq, double* qs,
+                                 hipStream_t stream);
 
 // Device-side gather of V[F] into the 9 x nf FP64 face SoA (wn_forward.hip).
 // vdtype: 0 = float64, 1 = float32; fdtype: 0 = int64, 1 = int32.

@@ -108,6 +108,25 @@ size_t sort_temp_bytes(int64_t nq) {
 
 }  // namespace
 
+__global__ __launch_bounds__(kBlockThreads) void gather_queries_kernel(
+    const double* __restrict__ q, const int32_t* __restrict__ perm, int64_t nq,
+    double* __restrict__ qs) {
+  const int64_t slot = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (slot >= nq) return;
+  const int64_t i = perm[slot];
+  qs[3 * slot] = q[3 * i];
+  qs[3 * slot + 1] = q[3 * i + 1];
+  qs[3 * slot + 2] = q[3 * i + 2];
+}
+
+hipError_t gather_queries_launch(const double* q, const int32_t* perm, int64_t nq, double* qs,
+                                 hipStream_t stream) {
+  if (nq <= 0) return hipSuccess;
+  gather_queries_kernel<<<(unsigned)((nq + kBlockThreads - 1) / kBlockThreads), kBlockThreads, 0,
+                          stream>>>(q, perm, nq, qs);
+  return hipGetLastError();
+}
+
 size_t morton_workspace_bytes(int64_t nq) {
   size_t b = align_up(6 * sizeof(unsigned long long));
   b += 3 * align_up((size_t)nq * sizeof(uint32_t));

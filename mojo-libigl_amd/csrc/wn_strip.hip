@@ -1,0 +1,234 @@
+// FP32-compensated forward over triangle-strip records (records built by
+// wn_strips.cpp; format documented there).
+//
+// Same arithmetic as fwd_f32c_kernel (block-local origin, FP32 pairs, FP64
+// accumulation, far-pair fast path, FP64 re-evaluation of ill-conditioned
+// pairs), but faces arrive as a strip: record k brings vertex w_k and closes
+// face (w_{k-2}, w_{k-1}, w_k). Per face the lane computes one offset, one
+// squared length and one square root (the other two vertices are kept in
+// registers from the previous records) and two law-of-cosines dot products
+// (the third is carried), instead of three of each.
+//
+// LDS tile (255 records, a multiple of 3 so the three register slots rotate
+// statically through a 3-way unrolled loop):
+//   R0[j] = (w.x, w.y, w.z, |w_j - w_{j-1}|^2)      (block-local FP32 coords)
+//   R1[j] = (|w_j - w_{j-2}|^2, N2.x, N2.y, N2.z)     (N2 = 2 x face normal)
+//   M[j]  = 0 (no face ends at j), +1 / -1 (face orientation vs strip order)
+// Squared lengths and N2 are computed in FP64 from the rounded FP32 corners.
+
+#include "wn_device.h"
+#include "wn_internal.h"
+#include "wn_sweep.h"
+
+namespace wn {
+
+constexpr int kStripTileRecords = 255;
+
+struct StripSmem {
+  float4 r0[kStripTileRecords];
+  float4 r1[kStripTileRecords];
+  float m[kStripTileRecords];
+  double red[6][kBlockThreads / kWave];
+};
+
+namespace {
+
+struct VSlot {
+  float ax, ay, az, l2, l;
+};
+
+__device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec, int64_t base,
+                                                 int cnt, double gx, double gy, double gz,
+                                                 StripSmem& sm) {
+  const int j = threadIdx.x;
+  if (j >= cnt) return;
+  const double4 c = rec[base + j];
+  const double4 p = rec[base + (j >= 1 ? j - 1 : j)];
+  const double4 pp = rec[base + (j >= 2 ? j - 2 : j)];
+  const float cx = (float)(c.x - gx), cy = (float)(c.y - gy), cz = (float)(c.z - gz);
+  const float px = (float)(p.x - gx), py = (float)(p.y - gy), pz = (float)(p.z - gz);
+  const float qx = (float)(pp.x - gx), qy = (float)(pp.y - gy), qz = (float)(pp.z - gz);
+  // FP64 on the rounded corners: w_k - w_{k-1}, w_k - w_{k-2}, w_{k-1} - w_{k-2}
+  const double d1x = (double)cx - px, d1y = (double)cy - py, d1z = (double)cz - pz;
+  const double d2x = (double)cx - qx, d2y = (double)cy - qy, d2z = (double)cz - qz;
+  const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
+  // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag
+  const double f2 = 2.0 * c.w;
+  sm.r0[j] = make_float4(cx, cy, cz, (float)(d1x * d1x + d1y * d1y + d1z * d1z));
+  sm.r1[j] = make_float4((float)(d2x * d2x + d2y * d2y + d2z * d2z),
+                         (float)(f2 * (ey * d2z - ez * d2y)), (float)(f2 * (ez * d2x - ex * d2z)),
+                         (float)(f2 * (ex * d2y - ey * d2x)));
+  sm.m[j] = (float)c.w;
+}
+
+// FP64 re-evaluation of face (j-2, j-1, j) from the LDS corners, oriented.
+__device__ __noinline__ double strip_pair_f64(const StripSmem& sm, int j, float ox, float oy,
+                                              float oz) {
+  const float4 a = sm.r0[j - 2], b = sm.r0[j - 1], c = sm.r0[j];
+  const double qx = ox, qy = oy, qz = oz;
+  const double v = pair_f64((double)a.x - qx, (double)a.y - qy, (double)a.z - qz,
+                            (double)b.x - qx, (double)b.y - qy, (double)b.z - qz,
+                            (double)c.x - qx, (double)c.y - qy, (double)c.z - qz);
+  return sm.m[j] > 0.0f ? v : -v;
+}
+
+__device__ __forceinline__ void strip_step(const StripSmem& sm, int j, VSlot& P, VSlot& Q,
+                                           VSlot& R, float& s_carry, double& acc, float ox,
+                                           float oy, float oz, float kappa) {
+  const float4 r0 = sm.r0[j];
+  R.ax = r0.x - ox;
+  R.ay = r0.y - oy;
+  R.az = r0.z - oz;
+  R.l2 = __builtin_fmaf(R.az, R.az, __builtin_fmaf(R.ay, R.ay, R.ax * R.ax));
+  R.l = __builtin_amdgcn_sqrtf(R.l2);
+  const float4 r1 = sm.r1[j];
+  const float s_qr = (Q.l2 + R.l2) - r0.w;  // 2 b.c
+  const float s_pr = (P.l2 + R.l2) - r1.x;  // 2 c.a
+  if (sm.m[j] != 0.0f) {
+    // face (a, b, c) = (P, Q, R): s_ab = s_carry, s_bc = s_qr, s_ca = s_pr
+    const float det2 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
+    const float u2 = 2.0f * (P.l * Q.l);
+    const float s2 = u2 * R.l;
+    const float den2 =
+        __builtin_fmaf(R.l, u2 + s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
+    const bool far = __builtin_fabsf(det2) < kFarTau * den2 && den2 > kFarKappa * s2;
+    float ang;
+    bool risky = false;
+    if (wave_all(far)) {
+      ang = atan_small_f32(det2 * __builtin_amdgcn_rcpf(den2));
+    } else {
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      float mx;
+      ang = atan2_f32(det2, den2, mx);
+      risky = mx < kappa * s2;
+    }
+    double a64 = (double)ang;
+    if (risky) a64 = strip_pair_f64(sm, j, ox, oy, oz);
+    acc += a64;
+  }
+  s_carry = s_qr;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void fwd_strip_kernel(
+    const double4* __restrict__ rec, int64_t ntiles, const double* __restrict__ q,
+    const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nq,
+    int64_t tiles_per_split, int splits, float kappa) {
+  __shared__ StripSmem sm;
+  const int tid = threadIdx.x;
+  const int64_t slot = (int64_t)blockIdx.x * kBlockThreads + tid;
+  bool valid[1] = {slot < nq};
+  const int64_t qidx = valid[0] ? (perm ? (int64_t)perm[slot] : slot) : 0;
+  // q is in launch order (gathered through perm beforehand when sorted)
+  double Ox[1] = {valid[0] ? q[3 * slot] : 0.0};
+  double Oy[1] = {valid[0] ? q[3 * slot + 1] : 0.0};
+  double Oz[1] = {valid[0] ? q[3 * slot + 2] : 0.0};
+
+  // block-local origin (same reduction as the face-record kernel)
+  double gx, gy, gz;
+  {
+    double lo0 = valid[0] ? Ox[0] : __builtin_inf(), hi0 = valid[0] ? Ox[0] : -__builtin_inf();
+    double lo1 = valid[0] ? Oy[0] : __builtin_inf(), hi1 = valid[0] ? Oy[0] : -__builtin_inf();
+    double lo2 = valid[0] ? Oz[0] : __builtin_inf(), hi2 = valid[0] ? Oz[0] : -__builtin_inf();
+    lo0 = wave_min_f64(lo0); lo1 = wave_min_f64(lo1); lo2 = wave_min_f64(lo2);
+    hi0 = wave_max_f64(hi0); hi1 = wave_max_f64(hi1); hi2 = wave_max_f64(hi2);
+    const int wave = tid / kWave;
+    if ((tid & (kWave - 1)) == 0) {
+      sm.red[0][wave] = lo0; sm.red[1][wave] = lo1; sm.red[2][wave] = lo2;
+      sm.red[3][wave] = hi0; sm.red[4][wave] = hi1; sm.red[5][wave] = hi2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kBlockThreads / kWave; ++w) {
+      lo0 = fmin(lo0, sm.red[0][w]); lo1 = fmin(lo1, sm.red[1][w]); lo2 = fmin(lo2, sm.red[2][w]);
+      hi0 = fmax(hi0, sm.red[3][w]); hi1 = fmax(hi1, sm.red[4][w]); hi2 = fmax(hi2, sm.red[5][w]);
+    }
+    gx = 0.5 * (lo0 + hi0);
+    gy = 0.5 * (lo1 + hi1);
+    gz = 0.5 * (lo2 + hi2);
+  }
+  const float ox = valid[0] ? (float)(Ox[0] - gx) : 0.0f;
+  const float oy = valid[0] ? (float)(Oy[0] - gy) : 0.0f;
+  const float oz = valid[0] ? (float)(Oz[0] - gz) : 0.0f;
+
+  double acc = 0.0;
+  VSlot A{0.f, 0.f, 0.f, 0.f, 0.f}, B = A, C = A;
+  float s_carry = 0.0f;
+  const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
+  const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
+  for (int64_t t = t_begin; t < t_end; ++t) {
+    __syncthreads();
+    stage_strip_tile(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
+    __syncthreads();
+    for (int j = 0; j < kStripTileRecords; j += 3) {
+      strip_step(sm, j, A, B, C, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j + 1, B, C, A, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j + 2, C, A, B, s_carry, acc, ox, oy, oz, kappa);
+    }
+  }
+  if (valid[0]) {
+    if (splits == 1)
+      out[qidx] = acc * kInv2Pi;
+    else
+      out[(int64_t)blockIdx.y * nq + slot] = acc;
+  }
+}
+
+}  // namespace
+
+int strip_resident_blocks() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fwd_strip_kernel,
+                                                   kBlockThreads, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return cus * per_cu;
+}
+
+size_t strip_workspace_bytes(int64_t nq, int splits, int sort) {
+  size_t bytes = 0;
+  if (splits > 1) bytes += align_up((size_t)nq * (size_t)splits * sizeof(double));
+  if (sort)
+    bytes += align_up((size_t)nq * sizeof(int32_t)) + align_up((size_t)nq * 3 * sizeof(double)) +
+             morton_workspace_bytes(nq);
+  return bytes;
+}
+
+hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, double* dst,
+                        int64_t nq, int splits, int sort, float kappa, void* ws, size_t ws_bytes,
+                        hipStream_t stream) {
+  if (nq == 0) return hipSuccess;
+  if (ws_bytes < strip_workspace_bytes(nq, splits, sort)) return hipErrorInvalidValue;
+  const int64_t ntiles = nrec / kStripTileRecords;
+  char* cursor = (char*)ws;
+  double* partial = nullptr;
+  if (splits > 1) {
+    partial = (double*)cursor;
+    cursor += align_up((size_t)nq * (size_t)splits * sizeof(double));
+  }
+  int32_t* perm = nullptr;
+  if (sort) {
+    perm = (int32_t*)cursor;
+    cursor += align_up((size_t)nq * sizeof(int32_t));
+    double* qs = (double*)cursor;
+    cursor += align_up((size_t)nq * 3 * sizeof(double));
+    hipError_t e = morton_order(q, nq, perm, cursor, morton_workspace_bytes(nq), stream);
+    if (e == hipSuccess) e = gather_queries_launch(q, perm, nq, qs, stream);
+    if (e != hipSuccess) return e;
+    q = qs;
+  }
+  if (ntiles == 0) return fill_launch(dst, nq, 0.0, stream);
+  const int64_t tps = (ntiles + splits - 1) / splits;
+  const int used_splits = (int)((ntiles + tps - 1) / tps);
+  const dim3 grid((unsigned)((nq + kBlockThreads - 1) / kBlockThreads), (unsigned)used_splits);
+  fwd_strip_kernel<<<grid, kBlockThreads, 0, stream>>>(rec, ntiles, q, perm,
+                                                       used_splits > 1 ? partial : dst, nq, tps,
+                                                       used_splits, kappa);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || used_splits == 1) return e;
+  return reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
+}
+
+}  // namespace wn

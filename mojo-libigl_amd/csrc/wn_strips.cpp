@@ -1,0 +1,204 @@
+// Triangle-strip records for the strip forward kernel (host side).
+//
+// No counterpart in the reference (src/capi.mojo:205-294 evaluates every face
+// from its three corners). A strip w0, w1, w2, w3, ... lists faces
+// (w0,w1,w2), (w1,w2,w3), ...: each face after the first adds one vertex, so
+// a kernel walking the strip evaluates per-vertex quantities (offset from the
+// query, its length) once per vertex instead of three times per face.
+//
+// Records: one per strip vertex, (x, y, z, flag), flag = 0 for the first two
+// vertices of a strip segment (no face ends there), +1 / -1 when the record
+// closes face (w_{k-2}, w_{k-1}, w_k) whose own corner order is a cyclic
+// rotation of that order (+1) or of its reverse (-1).
+// Tiles: records are grouped in tiles of kStripTile = 255 (a multiple of 3,
+// for the kernel's 3-way register rotation); every tile starts a new strip
+// segment, so tiles are independent. A strip crossing a tile boundary is cut
+// and continued in the next tile from its last two vertices; the tail of a
+// tile that cannot hold another face is padded with flag-0 records.
+//
+// Stripification: greedy. Faces are adjacent across an edge shared by exactly
+// two faces (edges of more than two faces are treated as borders). From each
+// unvisited face (in index order) the walk tries the three rotations of its
+// corners and keeps the longest strip of unvisited faces, always crossing the
+// edge (w_{k-1}, w_k).
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+namespace wn {
+
+constexpr int64_t kStripTile = 255;
+
+namespace {
+
+struct EdgeRef {
+  int64_t lo, hi;  // vertex ids, lo < hi
+  int64_t face;
+  int32_t local;  // edge (c[local], c[(local + 1) % 3]) of the face
+};
+
+template <typename FT>
+int64_t corner(const FT* F, int64_t f, int c) {
+  return (int64_t)F[3 * f + c];
+}
+
+template <typename FT>
+struct Stripifier {
+  const FT* F;
+  int64_t nf;
+  std::vector<int64_t> nbr;  // 3 per face: face across local edge e, -1 if none
+  std::vector<uint8_t> used;
+
+  Stripifier(const FT* F_, int64_t nf_) : F(F_), nf(nf_), nbr(3 * nf_, -1), used(nf_, 0) {
+    std::vector<EdgeRef> e;
+    e.reserve((size_t)(3 * nf));
+    for (int64_t f = 0; f < nf; ++f)
+      for (int k = 0; k < 3; ++k) {
+        int64_t a = corner(F, f, k), b = corner(F, f, (k + 1) % 3);
+        if (a == b) continue;
+        e.push_back({std::min(a, b), std::max(a, b), f, k});
+      }
+    std::sort(e.begin(), e.end(), [](const EdgeRef& x, const EdgeRef& y) {
+      return x.lo != y.lo ? x.lo < y.lo : (x.hi != y.hi ? x.hi < y.hi : x.face < y.face);
+    });
+    for (size_t i = 0; i < e.size();) {
+      size_t j = i;
+      while (j < e.size() && e[j].lo == e[i].lo && e[j].hi == e[i].hi) ++j;
+      if (j - i == 2) {
+        nbr[3 * e[i].face + e[i].local] = e[i + 1].face;
+        nbr[3 * e[i + 1].face + e[i + 1].local] = e[i].face;
+      }
+      i = j;
+    }
+  }
+
+  // Local edge index of edge {a, b} in face f, -1 if absent.
+  int local_edge(int64_t f, int64_t a, int64_t b) const {
+    for (int k = 0; k < 3; ++k) {
+      const int64_t u = corner(F, f, k), v = corner(F, f, (k + 1) % 3);
+      if ((u == a && v == b) || (u == b && v == a)) return k;
+    }
+    return -1;
+  }
+
+  int64_t third(int64_t f, int64_t a, int64_t b) const {
+    for (int k = 0; k < 3; ++k) {
+      const int64_t c = corner(F, f, k);
+      if (c != a && c != b) return c;
+    }
+    return corner(F, f, 0);
+  }
+
+  // +1 if (p, q, r) is a cyclic rotation of face f's corner order, else -1.
+  int orientation(int64_t f, int64_t p, int64_t q) const {
+    for (int k = 0; k < 3; ++k)
+      if (corner(F, f, k) == p) return corner(F, f, (k + 1) % 3) == q ? 1 : -1;
+    return 1;
+  }
+
+  // Walk from face f with strip order (p, q, r); returns faces and vertices.
+  void walk(int64_t f, int64_t p, int64_t q, int64_t r, bool mark, std::vector<int64_t>& faces,
+            std::vector<int64_t>& verts) {
+    faces.assign(1, f);
+    verts.assign({p, q, r});
+    std::vector<int64_t> touched;
+    if (!mark) touched.push_back(f);
+    used[f] = 1;
+    int64_t cur = f;
+    for (;;) {
+      const int64_t a = verts[verts.size() - 2], b = verts.back();
+      const int le = local_edge(cur, a, b);
+      if (le < 0) break;
+      const int64_t g = nbr[3 * cur + le];
+      if (g < 0 || used[g]) break;
+      const int64_t s = third(g, a, b);
+      if (s == a || s == b) break;
+      used[g] = 1;
+      if (!mark) touched.push_back(g);
+      faces.push_back(g);
+      verts.push_back(s);
+      cur = g;
+    }
+    if (!mark)
+      for (int64_t t : touched) used[t] = 0;
+  }
+};
+
+struct Emitter {
+  std::vector<double>* out;  // 4 doubles per record
+  const double* V;
+  int vstride;
+  int64_t in_tile = 0;
+
+  void record(int64_t v, double flag) {
+    out->push_back(V[3 * v]);
+    out->push_back(V[3 * v + 1]);
+    out->push_back(V[3 * v + 2]);
+    out->push_back(flag);
+    if (++in_tile == kStripTile) in_tile = 0;
+  }
+  void pad_tile(int64_t v) {
+    while (in_tile != 0) record(v, 0.0);
+  }
+};
+
+template <typename FT>
+int64_t build_records(const double* V, const FT* F, int64_t nf, std::vector<double>& out) {
+  Stripifier<FT> s(F, nf);
+  Emitter em{&out, V, 3};
+  std::vector<int64_t> faces, verts, best_f, best_v;
+  int64_t last_v = nf > 0 ? corner(F, 0, 0) : 0;
+  for (int64_t f = 0; f < nf; ++f) {
+    if (s.used[f]) continue;
+    // try the three rotations, keep the longest (no marking while trying)
+    size_t best = 0;
+    int best_rot = 0;
+    for (int rot = 0; rot < 3; ++rot) {
+      s.walk(f, corner(F, f, rot), corner(F, f, (rot + 1) % 3), corner(F, f, (rot + 2) % 3),
+             false, faces, verts);
+      if (faces.size() > best) {
+        best = faces.size();
+        best_rot = rot;
+      }
+    }
+    s.walk(f, corner(F, f, best_rot), corner(F, f, (best_rot + 1) % 3),
+           corner(F, f, (best_rot + 2) % 3), true, faces, verts);
+    // emit in segments that fit the current tile
+    size_t i = 0;  // index of the first face of the segment
+    while (i < faces.size()) {
+      int64_t space = kStripTile - em.in_tile;
+      if (space < 3) {
+        em.pad_tile(last_v);
+        space = kStripTile;
+      }
+      const size_t m = std::min<size_t>(faces.size() - i, (size_t)(space - 2));
+      em.record(verts[i], 0.0);
+      em.record(verts[i + 1], 0.0);
+      for (size_t k = 0; k < m; ++k) {
+        const int64_t face = faces[i + k];
+        em.record(verts[i + k + 2], (double)s.orientation(face, verts[i + k], verts[i + k + 1]));
+      }
+      last_v = verts[i + m + 1];
+      i += m;
+    }
+  }
+  em.pad_tile(last_v);
+  return (int64_t)(out.size() / 4);
+}
+
+}  // namespace
+
+// Builds the records of mesh (V: nv x 3 float64, F: nf x 3 of fdtype
+// 0 = int64, 1 = int32). Indices must be in [0, nv). Returns the record count
+// (a multiple of kStripTile; 0 for an empty mesh).
+int64_t strip_records(const double* V, const void* F, int64_t nf, int fdtype,
+                      std::vector<double>& out) {
+  out.clear();
+  if (nf <= 0) return 0;
+  out.reserve((size_t)(4 * (nf + nf / 4 + kStripTile)));
+  return fdtype == 0 ? build_records(V, (const int64_t*)F, nf, out)
+                     : build_records(V, (const int32_t*)F, nf, out);
+}
+
+}  // namespace wn

@@ -195,3 +195,12 @@ def winding_number(V, F, O, device="cpu", *, precision="fp64"):
 def winding_number_for_point(V, F, p):
     """Winding number at one point, CPU (python/igl/__init__.py:282-283)."""
     return float(winding_number(V, F, p))
+
+
+def __getattr__(name):
+    # Lazy: these need PyTorch only when used.
+    if name == "Mesh":
+        from .mesh import Mesh
+
+        return Mesh
+    raise AttributeError(name)

@@ -38,6 +38,13 @@ _SIGNATURES = {
     "wn_forward_workspace_bytes": ([I] * 6, I),
     "wn_forward": ([I] * 9 + [D] + [I] * 3, I),
     "wn_pack_soa": ([I] * 9, I),
+    "wn_mesh_records": ([I] * 7, I),
+    "wn_mesh_create": ([I] * 5, I),
+    "wn_mesh_info": ([I] * 2, I),
+    "wn_mesh_resolve": ([I] * 5, I),
+    "wn_mesh_workspace_bytes": ([I] * 4, I),
+    "wn_mesh_forward": ([I] * 6 + [D] + [I] * 3, I),
+    "wn_mesh_destroy": ([I], I),
     "wn_batch_create": ([I] * 7 + [D], I),
     "wn_batch_run": ([I] * 6, I),
     "wn_batch_info": ([I] * 2, I),

@@ -1,0 +1,97 @@
+"""Device-resident mesh for repeated winding-number queries.
+
+``Mesh(V, F)`` converts the mesh once into triangle-strip records (host C++,
+wn_mesh_create) and keeps them in HBM; ``Mesh.winding_number(O)`` then runs
+the FP32-compensated strip kernel (wn_mesh_forward), which evaluates one
+vertex offset, length and square root per face instead of three. Same
+accuracy contract as ``igl.winding_number(..., precision="fp32c")``.
+
+    m = igl.Mesh(V, F)
+    w = m.winding_number(O)        # numpy in -> numpy out, or a CUDA tensor
+    m.close()
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import addr, last_error, lib
+
+
+def strip_records(V, F) -> np.ndarray:
+    """(records, 4) float64 array (x, y, z, flag) of mesh (V, F), computed on
+    the host by the library's stripifier."""
+    from . import _mesh
+
+    vertices, faces = _mesh(V, F, three_dimensional=True)
+    h = lib()
+    n = h.wn_mesh_records(addr(vertices), addr(faces), len(vertices), len(faces), 0, 0, 0)
+    if n < 0:
+        raise ValueError(last_error())
+    out = np.empty((max(n, 0), 4), dtype=np.float64)
+    if n:
+        got = h.wn_mesh_records(addr(vertices), addr(faces), len(vertices), len(faces), 0,
+                                addr(out), n)
+        if got != n:
+            raise RuntimeError(last_error())
+    return out
+
+
+class Mesh:
+    """A mesh resident on the GPU in strip-record form."""
+
+    def __init__(self, V, F):
+        from . import _mesh
+
+        vertices, faces = _mesh(V, F, three_dimensional=True)
+        h = lib()
+        handle = h.wn_mesh_create(addr(vertices), addr(faces), len(vertices), len(faces), 0)
+        if handle <= 0:
+            raise RuntimeError(f"wn_mesh_create failed: {last_error()}")
+        self.handle = handle
+        self.n_faces = len(faces)
+
+    def info(self) -> dict:
+        out = np.zeros(3, dtype=np.int64)
+        if lib().wn_mesh_info(self.handle, out.ctypes.data) != 0:
+            raise RuntimeError(last_error())
+        return {"faces": int(out[0]), "records": int(out[1]), "tiles": int(out[2])}
+
+    def resolve(self, nq: int, splits=0, sort=-1):
+        out = np.zeros(2, dtype=np.int64)
+        if lib().wn_mesh_resolve(self.handle, nq, splits, sort, out.ctypes.data) != 0:
+            raise RuntimeError(last_error())
+        return int(out[0]), int(out[1])
+
+    def winding_number(self, O, splits=0, sort=-1, kappa=-1.0, out=None):
+        """w at the rows of O. O: (q, 3) array-like (result numpy float64) or
+        a CUDA tensor (result a float64 CUDA tensor on the current stream)."""
+        import torch
+
+        host = not (isinstance(O, torch.Tensor) and O.is_cuda)
+        Q = torch.as_tensor(np.asarray(O, dtype=np.float64).reshape(-1, 3) if host else O)
+        Q = Q.to(device="cuda", dtype=torch.float64).contiguous()
+        nq = Q.shape[0]
+        splits, sort = self.resolve(nq, splits, sort)
+        nbytes = lib().wn_mesh_workspace_bytes(self.handle, nq, splits, sort)
+        if nbytes < 0:
+            raise RuntimeError(last_error())
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=Q.device)
+        w = out if out is not None else torch.empty(nq, dtype=torch.float64, device=Q.device)
+        st = lib().wn_mesh_forward(self.handle, Q.data_ptr(), w.data_ptr(), nq, splits, sort,
+                                   float(kappa), ws.data_ptr(), nbytes,
+                                   torch.cuda.current_stream().cuda_stream)
+        if st != 0:
+            raise RuntimeError(f"wn_mesh_forward failed on the GPU: {last_error()}")
+        return w.cpu().numpy() if host else w
+
+    def close(self) -> None:
+        if getattr(self, "handle", 0):
+            lib().wn_mesh_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
