@@ -152,10 +152,31 @@ def base_line(args, metric, value, unit, ms, dtype, workload, **extra):
     return line
 
 
-def roofline(pairs, ms, peak=FP32_VECTOR_PEAK_TFLOPS):
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "fwd_torus28k_1M_counters.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on the torus workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each scaled
+    by the reading of a device copy of known size); None if absent."""
+    try:
+        text = open(COUNTER_PROFILE).read()
+    except OSError:
+        return None
+    for row in text.splitlines():
+        cells = [c.strip() for c in row.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == f"`{kernel}`":
+            try:
+                return (float(cells[3].replace(",", "")) + float(cells[4].replace(",", ""))) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
+def roofline(pairs, ms, peak=FP32_VECTOR_PEAK_TFLOPS, kernel=None):
     achieved = pairs * FLOP_PER_PAIR / (ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None}
+            "frac": achieved / peak, "traffic": measured_traffic(kernel) if kernel else None}
 
 
 # ---------------------------------------------------------------------------
@@ -244,7 +265,9 @@ def run_torus(args, h):
         f"closed torus R=1 r=0.35 120x118 ({nf} faces) x {nq} queries uniform in the 20%-padded "
         "bbox; FP32-compensated forward (FP32 pairs, FP64 accumulation, FP64 re-evaluation of "
         "ill-conditioned pairs) over the resident triangle-strip mesh, inputs resident in HBM",
-        roofline=roofline(nf * nq, ms_strip), fp64_plain=result_fp64,
+        roofline=roofline(nf * nq, ms_strip, kernel="wn::fwd_strip_kernel"), fp64_plain=result_fp64,
+        traffic_source="bytes per launch of wn::fwd_strip_kernel from profiles/fwd_torus28k_1M_counters.md "
+                       "(rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 256 MiB copy)",
         speedup_fp32c_vs_fp64=(result_fp64["ms_per_step"] / ms_strip) if result_fp64 else None,
         fp32c_face_kernel={"value": nf * nq / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face,
                            "qpl": face.qpl, "splits": face.splits},

@@ -90,34 +90,20 @@ int resident_blocks(int precision, int qpl) {
   return cache[precision][qpl];
 }
 
-// Score of a launch shape: the fraction of the resident-block rounds that are
-// filled, times a tail term (the last round ends unevenly; more, shorter
-// rounds even it out), times a per-block overhead factor (block setup costs
-// about a 128-face stretch of work). Calibrated on the 28,320-face x 1M-query
-// torus, where it picks the measured optimum (6 splits at one query per
-// lane for FP32-compensated).
-double shape_score(int64_t nf, int64_t qblocks, int64_t splits, int64_t resident) {
-  const int64_t blocks = qblocks * splits;
-  const int64_t rounds = (blocks + resident - 1) / resident;
-  const double fill = (double)blocks / (double)(rounds * resident);
-  const double per_block = (double)nf / (double)splits;
-  return fill * (1.0 - 0.25 / (double)rounds) * per_block / (per_block + 128.0);
-}
-
-int64_t best_splits(int64_t nf, int64_t qblocks, int64_t resident, double* score_out) {
-  const int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(4096, nf / wn::kFaceTile));
-  double best = -1.0;
-  int64_t best_s = 1;
-  for (int64_t s = 1; s <= max_s; ++s) {
-    const double sc = shape_score(nf, qblocks, s, resident);
-    if (sc > best + 1e-9) {
-      best = sc;
-      best_s = s;
-    }
-    if (qblocks * s >= 32 * resident) break;  // tail term already above 0.99
-  }
-  if (score_out) *score_out = best;
-  return best_s;
+// Face splits for a launch: enough (query block x split) blocks for about 24
+// rounds of the device's resident blocks, so that the uneven end of the last
+// round is short, capped at one 256-face tile (or strip tile) per split.
+// Measured on the 28,320-face torus (strip and face kernels): 1M queries ->
+// 12 splits (+4 % over 5), 65k queries -> >= 64 splits, 4k queries -> one
+// tile per split; more rounds than ~24 start to cost block set-up, and so do
+// one-tile splits while two-tile splits still give >= 4 rounds (65k queries:
+// 56 splits beat 112).
+int64_t best_splits(int64_t tiles, int64_t qblocks, int64_t resident) {
+  resident = std::max<int64_t>(resident, 1);
+  int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(4096, tiles));
+  if (tiles >= 2 && qblocks * (tiles / 2) >= 4 * resident) max_s = std::min<int64_t>(max_s, tiles / 2);
+  const int64_t want = (24 * std::max<int64_t>(resident, 1) + qblocks - 1) / std::max<int64_t>(qblocks, 1);
+  return std::max<int64_t>(1, std::min<int64_t>(max_s, want));
 }
 
 wn::FwdConfig make_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
@@ -138,7 +124,8 @@ wn::FwdConfig make_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl
   // (full 8-wave occupancy beats the LDS-read amortisation of 2 or 4).
   if (c.qpl <= 0) c.qpl = 1;
   if (c.splits <= 0 && c.qpl >= 1 && c.qpl <= 4)
-    c.splits = (int)best_splits(nf, qblocks_of(c.qpl), resident_of(c.qpl), nullptr);
+    c.splits = (int)best_splits((nf + wn::kFaceTile - 1) / wn::kFaceTile, qblocks_of(c.qpl),
+                                resident_of(c.qpl));
   if (c.sort < 0) c.sort = (c.precision == wn::kFP32C && nq >= 4096) ? 1 : 0;
   if (kappa < 0.0) c.kappa = 1.0f;
   return c;
@@ -473,8 +460,7 @@ int64_t wn_mesh_resolve(int64_t mesh, int64_t nq, int64_t splits, int64_t sort, 
     static int resident = -1;
     if (resident < 0) resident = device_count() > 0 ? wn::strip_resident_blocks() : 0;
     const int64_t qblocks = std::max<int64_t>(1, (nq + 255) / 256);
-    // best_splits works in units of faces per 256-face tile; records play that role here
-    s = best_splits(std::max<int64_t>(m->nrec, 1), qblocks, resident > 0 ? resident : 2048, nullptr);
+    s = best_splits(std::max<int64_t>(m->nrec / 255, 1), qblocks, resident > 0 ? resident : 2048);
   }
   if (s < 1 || s > 65535) return fail(kErrArg, "splits must be in [1, 65535]");
   const int64_t srt = sort < 0 ? (nq >= 4096 ? 1 : 0) : (sort ? 1 : 0);

@@ -212,6 +212,15 @@ __device__ __forceinline__ bool wave_all(bool pred) {
   return __builtin_amdgcn_ballot_w64(pred) == __builtin_amdgcn_read_exec();
 }
 
+// Far-pair test of every active lane: |det2| < kFarTau * den2 and
+// den2 > kFarKappa * s2 (see pair_geom32). Each compare is balloted on its
+// own so both masks stay in SGPRs (no VALU round trip through a bool).
+__device__ __forceinline__ bool wave_all_far(float det2, float den2, float s2) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det2) < kFarTau * den2);
+  const uint64_t m2 = __builtin_amdgcn_ballot_w64(den2 > kFarKappa * s2);
+  return (m1 & m2) == __builtin_amdgcn_read_exec();
+}
+
 // ---------------------------------------------------------------------------
 // Wavefront reductions.
 // ---------------------------------------------------------------------------

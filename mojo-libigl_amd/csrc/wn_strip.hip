@@ -72,9 +72,13 @@ __device__ __noinline__ double strip_pair_f64(const StripSmem& sm, int j, float 
   return sm.m[j] > 0.0f ? v : -v;
 }
 
-__device__ __forceinline__ void strip_step(const StripSmem& sm, int j, VSlot& P, VSlot& Q,
-                                           VSlot& R, float& s_carry, double& acc, float ox,
-                                           float oy, float oz, float kappa) {
+// flag_bits: the record's flag as raw bits (0 = no face ends here), read
+// from global memory at a wave-uniform address, so it arrives in SGPRs and
+// the emit test below is a scalar branch.
+__device__ __forceinline__ void strip_step(const StripSmem& sm, int j, long long flag_bits,
+                                           VSlot& P, VSlot& Q, VSlot& R, float& s_carry,
+                                           double& acc, float ox, float oy, float oz,
+                                           float kappa) {
   const float4 r0 = sm.r0[j];
   R.ax = r0.x - ox;
   R.ay = r0.y - oy;
@@ -84,17 +88,16 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, VSlot& P,
   const float4 r1 = sm.r1[j];
   const float s_qr = (Q.l2 + R.l2) - r0.w;  // 2 b.c
   const float s_pr = (P.l2 + R.l2) - r1.x;  // 2 c.a
-  if (sm.m[j] != 0.0f) {
+  if (flag_bits != 0) {
     // face (a, b, c) = (P, Q, R): s_ab = s_carry, s_bc = s_qr, s_ca = s_pr
     const float det2 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
     const float u2 = 2.0f * (P.l * Q.l);
     const float s2 = u2 * R.l;
     const float den2 =
         __builtin_fmaf(R.l, u2 + s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
-    const bool far = __builtin_fabsf(det2) < kFarTau * den2 && den2 > kFarKappa * s2;
     float ang;
     bool risky = false;
-    if (wave_all(far)) {
+    if (wave_all_far(det2, den2, s2)) {
       ang = atan_small_f32(det2 * __builtin_amdgcn_rcpf(den2));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
@@ -109,7 +112,7 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, VSlot& P,
   s_carry = s_qr;
 }
 
-__global__ __launch_bounds__(kBlockThreads) void fwd_strip_kernel(
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8))) void fwd_strip_kernel(
     const double4* __restrict__ rec, int64_t ntiles, const double* __restrict__ q,
     const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nq,
     int64_t tiles_per_split, int splits, float kappa) {
@@ -159,10 +162,11 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_strip_kernel(
     __syncthreads();
     stage_strip_tile(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
     __syncthreads();
+    const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     for (int j = 0; j < kStripTileRecords; j += 3) {
-      strip_step(sm, j, A, B, C, s_carry, acc, ox, oy, oz, kappa);
-      strip_step(sm, j + 1, B, C, A, s_carry, acc, ox, oy, oz, kappa);
-      strip_step(sm, j + 2, C, A, B, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j, flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j + 1, flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j + 2, flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz, kappa);
     }
   }
   if (valid[0]) {

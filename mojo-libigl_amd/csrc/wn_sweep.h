@@ -115,10 +115,9 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
 #pragma unroll
       for (int k = 0; k < QPL; ++k) {
         const PairGeom32 g = pair_geom32(r0, r1, r2, r3, ox[k], oy[k], oz[k]);
-        const bool far = __builtin_fabsf(g.det2) < kFarTau * g.den2 && g.den2 > kFarKappa * g.s2;
         float ang;
         bool risky = false;
-        if (wave_all(far)) {
+        if (wave_all_far(g.det2, g.den2, g.s2)) {
           // wave-uniform: every lane's angle is small and well conditioned
           ang = atan_small_f32(g.det2 * __builtin_amdgcn_rcpf(g.den2));
         } else {
