@@ -178,7 +178,7 @@ int64_t wn_batch_destroy(int64_t handle);
  * layout). FP64 (precision must be 0). (device) all buffers; asynchronous on
  * `stream`; allocates nothing. grad_tri is accumulated with FP64 atomics, so
  * its rounding depends on scheduling; grad_q is exact-order when splits == 1.
- * qpl 0 = default (1), splits 0 = default. */
+ * qpl (queries per lane) 1, 2 or 4, 0 = default (4); splits 0 = default. */
 int64_t wn_backward(const double *tri, const double *q, const double *grad_w,
                     double *grad_q, double *grad_tri, int64_t nf, int64_t nq,
                     int64_t precision, int64_t qpl, int64_t splits, void *stream);

@@ -14,10 +14,14 @@
 // Pairs with det^2 + den^2 == 0 or a zero corner distance (query on an edge
 // line or at a corner, where theta is not differentiable) contribute 0.
 //
-// FP64 throughout. Face tiles are restaged from the 9 x nf SoA per block (no
-// per-pair quantity is stored). dL/dO stays in registers (atomics only when
-// the face range is split); the face gradient is summed over the wave (DPP),
-// over the block (LDS ds_add_f64), then added to grad_tri with one FP64
+// FP64 throughout; corner distances and the 1/(det^2 + den^2) weight come
+// from the hardware reciprocal (square root) estimates refined by Newton
+// steps. Face tiles are restaged from the 9 x nf SoA per block (no per-pair
+// quantity is stored). Each lane holds 1, 2 or 4 queries, which amortise the
+// per-face reduction of the face gradient over the wave. dL/dO stays in
+// registers (atomics only when the face range is split); the face gradient
+// is reduce-scattered over 16-lane rows (DPP), summed over the rows and
+// waves of the block (LDS ds_add_f64), then added to grad_tri with one FP64
 // atomic per face-corner component per block -- so grad_tri and grad_V are
 // sums in an unspecified order (not bitwise reproducible), grad_q with
 // splits == 1 is.
@@ -33,14 +37,39 @@ struct PairGrad {
   double ax, ay, az, bx, by, bz, cx, cy, cz;  // d theta / d(a, b, c)
 };
 
+// 1/sqrt(x) and 1/x in FP64 from the hardware estimates plus two Newton
+// steps each (quadratic convergence from the estimate to full double
+// precision); x > 0 finite.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = __builtin_fma(-x * r, r, 1.0);  // 1 - x r^2
+    r = __builtin_fma(0.5 * r, e, r);
+  }
+  return r;
+}
+
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = __builtin_fma(-x, r, 1.0);  // 1 - x r
+    r = __builtin_fma(r, e, r);
+  }
+  return r;
+}
+
+// d theta / d(a, b, c): the formulas above, with reciprocal square roots for
+// the corner distances and one reciprocal for det^2 + den^2 (refined to full
+// FP64 precision) in place of square roots and divisions.
 __device__ __forceinline__ PairGrad pair_grad_f64(double ax, double ay, double az, double bx,
                                                   double by, double bz, double cx, double cy,
                                                   double cz) {
   PairGrad g;
-  const double la = sqrt(ax * ax + ay * ay + az * az);
-  const double lb = sqrt(bx * bx + by * by + bz * bz);
-  const double lc = sqrt(cx * cx + cy * cy + cz * cz);
-  // cross products
+  const double la2 = ax * ax + ay * ay + az * az;
+  const double lb2 = bx * bx + by * by + bz * bz;
+  const double lc2 = cx * cx + cy * cy + cz * cz;
   const double bcx = by * cz - bz * cy, bcy = bz * cx - bx * cz, bcz = bx * cy - by * cx;
   const double cax = cy * az - cz * ay, cay = cz * ax - cx * az, caz = cx * ay - cy * ax;
   const double abx = ay * bz - az * by, aby = az * bx - ax * bz, abz = ax * by - ay * bx;
@@ -48,28 +77,65 @@ __device__ __forceinline__ PairGrad pair_grad_f64(double ax, double ay, double a
   const double ab = ax * bx + ay * by + az * bz;
   const double bc = bx * cx + by * cy + bz * cz;
   const double ca = cx * ax + cy * ay + cz * az;
+  const double ra = rsqrt_nr(la2), rb = rsqrt_nr(lb2), rc = rsqrt_nr(lc2);
+  const double la = la2 * ra, lb = lb2 * rb, lc = lc2 * rc;
   const double den = la * lb * lc + ab * lc + bc * la + ca * lb;
   const double r2 = det * det + den * den;
-  if (!(r2 > 0.0) || !(la > 0.0) || !(lb > 0.0) || !(lc > 0.0)) {
+  // A zero corner distance makes la (or lb, lc) NaN and so r2; r2 == 0 puts
+  // the query on an edge. Both contribute 0, as does an overflowed r2.
+  if (!(r2 > 0.0 && r2 < __builtin_inf())) {
     g.ax = g.ay = g.az = g.bx = g.by = g.bz = g.cx = g.cy = g.cz = 0.0;
     return g;
   }
-  const double inv = 1.0 / r2;
+  const double inv = rcp_nr(r2);
   const double p = den * inv;   // weight of d det
   const double s = -det * inv;  // weight of d den
-  const double ka = s * (lb * lc + bc) / la;
-  const double kb = s * (lc * la + ca) / lb;
-  const double kc = s * (la * lb + ab) / lc;
-  g.ax = p * bcx + ka * ax + s * (bx * lc + cx * lb);
-  g.ay = p * bcy + ka * ay + s * (by * lc + cy * lb);
-  g.az = p * bcz + ka * az + s * (bz * lc + cz * lb);
-  g.bx = p * cax + kb * bx + s * (cx * la + ax * lc);
-  g.by = p * cay + kb * by + s * (cy * la + ay * lc);
-  g.bz = p * caz + kb * bz + s * (cz * la + az * lc);
-  g.cx = p * abx + kc * cx + s * (ax * lb + bx * la);
-  g.cy = p * aby + kc * cy + s * (ay * lb + by * la);
-  g.cz = p * abz + kc * cz + s * (az * lb + bz * la);
+  const double ka = s * (lb * lc + bc) * ra;
+  const double kb = s * (lc * la + ca) * rb;
+  const double kc = s * (la * lb + ab) * rc;
+  const double sla = s * la, slb = s * lb, slc = s * lc;
+  g.ax = p * bcx + ka * ax + slc * bx + slb * cx;
+  g.ay = p * bcy + ka * ay + slc * by + slb * cy;
+  g.az = p * bcz + ka * az + slc * bz + slb * cz;
+  g.bx = p * cax + kb * bx + sla * cx + slc * ax;
+  g.by = p * cay + kb * by + sla * cy + slc * ay;
+  g.bz = p * caz + kb * bz + sla * cz + slc * az;
+  g.cx = p * abx + kc * cx + slb * ax + sla * bx;
+  g.cy = p * aby + kc * cy + slb * ay + sla * by;
+  g.cz = p * abz + kc * cz + slb * az + sla * bz;
   return g;
+}
+
+// Reduce-scatter of nine per-lane values over each 16-lane row. Four DPP
+// exchanges with partners lane^15 (row_mirror), lane^7 (row_half_mirror),
+// lane^2 and lane^1 (quad_perm) -- independent masks, so every result sums
+// all 16 lanes once; at each exchange a lane keeps about half of its
+// components (by the bit its partner differs in) and adds the partner's
+// copy. Lane l of a row returns the row sum of component row_slot(l) (9 =
+// none).
+__device__ __forceinline__ int row_slot(int lane) {
+  // Components kept per lane bit 3, 2, 1, 0: {0..4 | 5..8}, then the first
+  // three | last two, the first two | the third, the first | the second.
+  // Nibble l: {0, 1, 2, -, 3, 4, -, -, 5, 6, 7, -, 8, -, -, -}.
+  constexpr uint64_t kSlots = 0x9998976599439210ull;
+  return (int)((kSlots >> (4 * (lane & 15))) & 0xF);
+}
+
+__device__ __forceinline__ double row_reduce_scatter9(const double (&v)[9], int lane) {
+  const bool b3 = lane & 8, b2 = lane & 4, b1 = lane & 2, b0 = lane & 1;
+  double u[5];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    u[p] = (b3 ? v[5 + p] : v[p]) + dpp_move_f64<0x140>(b3 ? v[p] : v[5 + p]);
+  u[4] = v[4] + dpp_move_f64<0x140>(v[4]);
+  double w[3];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+    w[p] = (b2 ? u[3 + p] : u[p]) + dpp_move_f64<0x141>(b2 ? u[p] : u[3 + p]);
+  w[2] = u[2] + dpp_move_f64<0x141>(u[2]);
+  const double x0 = (b1 ? w[2] : w[0]) + dpp_move_f64<0x4E>(b1 ? w[0] : w[2]);
+  const double x1 = w[1] + dpp_move_f64<0x4E>(w[1]);
+  return (b0 ? x1 : x0) + dpp_move_f64<0xB1>(b0 ? x0 : x1);
 }
 
 template <int QPL>
@@ -81,6 +147,7 @@ __global__ __launch_bounds__(kBlockThreads) void bwd_f64_kernel(
   __shared__ double sacc[kFaceTile][9];
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
+  const int slot = row_slot(lane);
   const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
 
   double ox[QPL], oy[QPL], oz[QPL], g[QPL], dx[QPL], dy[QPL], dz[QPL];
@@ -129,19 +196,10 @@ __global__ __launch_bounds__(kBlockThreads) void bwd_f64_kernel(
         fa[3] += gk * pg.bx; fa[4] += gk * pg.by; fa[5] += gk * pg.bz;
         fa[6] += gk * pg.cx; fa[7] += gk * pg.cy; fa[8] += gk * pg.cz;
       }
-      double mine = 0.0;
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        const double s = wave_sum_f64(fa[i]);
-        const double s0 = __longlong_as_double(
-            ((long long)(uint32_t)__builtin_amdgcn_readfirstlane(
-                 (int)(uint32_t)((uint64_t)__double_as_longlong(s) >> 32))
-             << 32) |
-            (long long)(uint32_t)__builtin_amdgcn_readfirstlane(
-                (int)(uint32_t)__double_as_longlong(s)));
-        if (lane == i) mine = s0;
-      }
-      if (lane < 9) atomicAdd(&sacc[j][lane], mine);
+      // Nine lanes per row add their component's row sum (four rows: 4-way
+      // same-address LDS atomics).
+      const double rs = row_reduce_scatter9(fa, lane);
+      if (slot < 9) atomicAdd(&sacc[j][slot], rs);
     }
     __syncthreads();
     if (tid < cnt) {
@@ -194,7 +252,7 @@ inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + kBlockThreads - 1) / 
 
 const char* bwd_check(int64_t nf, int64_t nq, int qpl, int splits) {
   if (nf < 0 || nq < 0) return "negative size";
-  if (qpl != 1 && qpl != 2) return "backward queries_per_lane must be 1 or 2";
+  if (qpl != 1 && qpl != 2 && qpl != 4) return "backward queries_per_lane must be 1, 2 or 4";
   if (splits < 1 || splits > 65535) return "splits must be in [1, 65535]";
   if (nq > (int64_t)0x7fffffff) return "more than 2^31-1 queries per call";
   return nullptr;
@@ -212,8 +270,10 @@ hipError_t bwd_launch(const double* tri, const double* q, const double* gw, doub
   const dim3 grid((unsigned)((nq + qblock - 1) / qblock), (unsigned)splits);
   if (qpl == 1)
     bwd_f64_kernel<1><<<grid, kBlockThreads, 0, stream>>>(tri, q, gw, gq, gtri, nf, nq, per, splits);
-  else
+  else if (qpl == 2)
     bwd_f64_kernel<2><<<grid, kBlockThreads, 0, stream>>>(tri, q, gw, gq, gtri, nf, nq, per, splits);
+  else
+    bwd_f64_kernel<4><<<grid, kBlockThreads, 0, stream>>>(tri, q, gw, gq, gtri, nf, nq, per, splits);
   return hipGetLastError();
 }
 

@@ -327,11 +327,13 @@ int64_t wn_backward(const double* tri, const double* q, const double* grad_w, do
                     double* grad_tri, int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
                     int64_t splits, void* stream) {
   if (precision != wn::kFP64) return fail(kErrArg, "backward supports precision 0 (FP64) only");
-  const int lanes = (int)(qpl <= 0 ? 1 : qpl);
+  const int lanes = (int)(qpl <= 0 ? 4 : qpl);
   int s = (int)splits;
   if (s <= 0) {
+    // about 4096 blocks (several rounds of the resident blocks), at least
+    // two face tiles per split
     const int64_t qblocks = (nq + 256 * lanes - 1) / (256 * lanes);
-    int64_t t = (2048 + qblocks - 1) / (qblocks > 0 ? qblocks : 1);
+    int64_t t = (4096 + qblocks - 1) / (qblocks > 0 ? qblocks : 1);
     const int64_t max_s = nf / (2 * wn::kFaceTile);
     if (t > max_s) t = max_s;
     if (t < 1) t = 1;
