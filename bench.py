@@ -287,7 +287,7 @@ def run_batch(args, h):
 
     from igl import _autotune
     from igl.autograd import pack_faces
-    from igl.batch import WindingBatch, pack_batch
+    from igl.batch import StripBatch, WindingBatch, pack_batch, pack_batch_strips
 
     import wn_workloads as W
 
@@ -298,16 +298,32 @@ def run_batch(args, h):
     pairs = int((fc * qc).sum())
     tri = pack_batch([b[0] for b in B], [b[1] for b in B])
     Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
-    log("autotuning (qpl x chunk)")
+    t0 = time.perf_counter()
+    rec, rc = pack_batch_strips([b[0] for b in B], [b[1] for b in B])
+    torch.cuda.synchronize()
+    strip_build_s = time.perf_counter() - t0
+    log(f"strip records {int(rc.sum())} for {int(fc.sum())} faces in {strip_build_s:.2f} s")
+    log("autotuning strip chunk")
+    t0 = time.perf_counter()
+    chunk_tiles = _autotune.tune_batch_strips(rc, qc, rec, Q, use_cache=False)
+    tune_strip_s = time.perf_counter() - t0
+    splan = StripBatch(rc, qc, chunk_tiles=chunk_tiles)
+    out = torch.empty(len(Q), dtype=torch.float64, device="cuda")
+    ms_graph, wall_graph = timed(lambda: splan.run(rec, Q, out=out, use_graph=True), args.steps,
+                                 args.warmup)
+    log(f"batched strips fp32c (graph) {ms_graph:.3f} ms/step; chunk_tiles={chunk_tiles}")
+    w_batch = out.cpu().numpy()
+    sinfo = splan.info()
+    splan.close()
+
+    log("autotuning face batch (qpl x chunk)")
     t0 = time.perf_counter()
     qpl, chunk = _autotune.tune_batch(fc, qc, tri, Q, 1, use_cache=False)
     tune_s = time.perf_counter() - t0
     plan = WindingBatch(fc, qc, precision="fp32c", qpl=qpl, chunk_faces=chunk)
-    out = torch.empty(len(Q), dtype=torch.float64, device="cuda")
-    ms_graph, wall_graph = timed(lambda: plan.run(tri, Q, out=out, use_graph=True), args.steps,
-                                 args.warmup)
-    log(f"batched fp32c (graph) {ms_graph:.3f} ms/step; qpl={qpl} chunk={chunk}")
-    w_batch = out.cpu().numpy()
+    ms_face, _ = timed(lambda: plan.run(tri, Q, out=out, use_graph=True), args.steps, args.warmup)
+    log(f"batched face records fp32c (graph) {ms_face:.3f} ms/step; qpl={qpl} chunk={chunk}")
+    w_face = out.cpu().numpy()
 
     # 512 separate launches of the single-mesh forward (per-mesh SoA packed
     # beforehand, outside the timed region)
@@ -336,15 +352,21 @@ def run_batch(args, h):
         args, "G query-face pairs/s, batched FP32-comp, 512 uneven meshes, 1 MI355X",
         pairs / (ms_graph * 1e-3) / 1e9, "G query-face pairs/s", ms_graph, "f32",
         f"512 meshes (torus / geodesic sphere / open cylinder with holes), {int(fc.sum())} faces, "
-        f"{int(qc.sum())} queries, {pairs} pairs; one persistent launch replayed from a hipGraph",
+        f"{int(qc.sum())} queries, {pairs} pairs; triangle-strip records, one persistent launch "
+        "replayed from a hipGraph",
         roofline=roofline(pairs, ms_graph),
+        face_records_batched={"value": pairs / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face},
         separate_launches={"value": pairs / (ms_sep * 1e-3) / 1e9, "ms_per_step": ms_sep},
         speedup_batched_vs_separate=ms_sep / ms_graph,
         fp64_batched={"value": pairs / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64},
-        tuning={"qpl": qpl, "chunk_faces": chunk, "seconds": tune_s, "items": info["items"],
-                "grid": info["grid"]},
+        strip_records={"records": int(rc.sum()), "host_build_seconds": strip_build_s},
+        tuning={"chunk_tiles": chunk_tiles, "seconds": tune_strip_s, "items": sinfo["items"],
+                "grid": sinfo["grid"],
+                "face_batch": {"qpl": qpl, "chunk_faces": chunk, "seconds": tune_s,
+                               "items": info["items"], "grid": info["grid"]}},
         accuracy={"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw": float(d.mean()),
-                  "max_abs_dw_batched_vs_separate": float(np.abs(w_batch - w_sep).max())})
+                  "max_abs_dw_face_batch_vs_fp64": float(np.abs(w_face - w64).max()),
+                  "max_abs_dw_batched_vs_separate": float(np.abs(w_face - w_sep).max())})
 
 
 def run_stress(args, h):

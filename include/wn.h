@@ -156,7 +156,19 @@ int64_t wn_batch_create(const int64_t *face_off, const int64_t *query_off,
                         int64_t n_meshes, int64_t precision, int64_t qpl,
                         int64_t chunk_faces, int64_t sort, double kappa);
 
-/* Runs a plan: (device) tri (9 x F_total), q (Q_total x 3), dst (Q_total).
+/* No reference counterpart. A batch plan over meshes in triangle-strip
+ * record form (wn_mesh_records per mesh, concatenated): mesh m owns records
+ * [rec_off[m], rec_off[m+1]) (a multiple of 255 each) and queries
+ * [query_off[m], query_off[m+1]). FP32-compensated, one query per lane;
+ * chunk_tiles = 255-record tiles per work item (0 = 8); sort -1 = Morton
+ * order. Run with wn_batch_run, passing the records (4 doubles each) as
+ * `tri`. Returns a handle (> 0) or a negative status. */
+int64_t wn_batch_create_strips(const int64_t *rec_off, const int64_t *query_off,
+                               int64_t n_meshes, int64_t chunk_tiles, int64_t sort,
+                               double kappa);
+
+/* Runs a plan: (device) tri (9 x F_total; the records of a strip plan),
+ * q (Q_total x 3), dst (Q_total).
  * use_graph != 0 replays a hipGraph of the persistent launch, captured on
  * first use for these buffer addresses. Asynchronous on `stream`. Calls on
  * one handle must not overlap. */
@@ -164,7 +176,7 @@ int64_t wn_batch_run(int64_t handle, const double *tri, const double *q,
                      double *dst, int64_t use_graph, void *stream);
 
 /* out6 = {work items, partial slots, workspace bytes, persistent grid,
- * F_total, Q_total}. */
+ * F_total (tiles for a strip plan), Q_total}. */
 int64_t wn_batch_info(int64_t handle, int64_t *out6);
 
 /* Frees a plan (synchronises the device first). */

@@ -16,6 +16,10 @@
 // partials to fixed slots, summed in chunk order by a reduce kernel, so the
 // result does not depend on which block took which item.
 //
+// Strip plans hold every mesh as triangle-strip records (wn_strips.cpp) in
+// one buffer, each mesh a whole number of 255-record tiles; items then cover
+// chunks of tiles and run the strip kernel's sweep (wn_strip.hip).
+//
 // The fill-counter / persistent / reduce sequence can be captured once into
 // a hipGraph and replayed (fill kernel, no memset node). The optional Morton
 // ordering of queries (segmented by mesh) runs eagerly before the graph.
@@ -30,23 +34,6 @@
 #include "wn_sweep.h"
 
 namespace wn {
-
-struct BatchMesh {
-  int64_t face_off;
-  int64_t query_off;
-  int64_t pbase;  // first partial slot (chunks > 1)
-  int32_t nf;
-  int32_t nq;
-  int32_t chunks;
-  int32_t pad;
-};
-
-struct BatchItem {
-  int32_t mesh;
-  int32_t qtile;
-  int32_t chunk;
-  int32_t pad;
-};
 
 namespace {
 
@@ -147,6 +134,7 @@ __global__ __launch_bounds__(kBlockThreads) void batch_reduce_kernel(
 // ---------------------------------------------------------------------------
 
 struct BatchPlan {
+  bool strips = false;  // strip plan: face units are 255-record tiles
   int precision = kFP32C;
   int qpl = 1;
   int sort = 0;
@@ -199,6 +187,93 @@ const char* batch_check(const int64_t* face_off, const int64_t* query_off, int64
   return nullptr;
 }
 
+namespace {
+
+// Items sorted by decreasing cost (longest processing time first), the
+// persistent grid (resident blocks of the plan's kernel, at most one per
+// item), then one device allocation holding counter, meshes, items,
+// offsets, partial slots and the sort buffers.
+hipError_t plan_finish(BatchPlan* p, const std::vector<BatchMesh>& meshes,
+                       std::vector<std::pair<int64_t, BatchItem>>& items, int64_t n_partials,
+                       const int64_t* query_off, int resident) {
+  std::stable_sort(items.begin(), items.end(),
+                   [](const auto& a, const auto& b) { return a.first > b.first; });
+  p->n_items = (int64_t)items.size();
+  p->n_partials = n_partials;
+  std::vector<BatchItem> flat(items.size());
+  for (size_t i = 0; i < items.size(); ++i) flat[i] = items[i].second;
+  p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(1, resident),
+                                                        std::max<int64_t>(1, p->n_items)));
+  const int64_t n_meshes = p->n_meshes;
+  const size_t b_counter = align_up(sizeof(int32_t));
+  const size_t b_meshes = align_up(sizeof(BatchMesh) * (size_t)n_meshes);
+  const size_t b_items = align_up(sizeof(BatchItem) * std::max<size_t>(1, flat.size()));
+  const size_t b_qoff = align_up(sizeof(int64_t) * (size_t)(n_meshes + 1));
+  const size_t b_partial = align_up(sizeof(double) * (size_t)std::max<int64_t>(1, n_partials));
+  const size_t b_perm = p->sort ? align_up(sizeof(int32_t) * (size_t)std::max<int64_t>(1, p->qtot)) : 0;
+  const size_t b_qs = p->sort ? align_up(3 * sizeof(double) * (size_t)std::max<int64_t>(1, p->qtot)) : 0;
+  p->sort_bytes = p->sort ? morton_segmented_workspace_bytes(p->qtot) : 0;
+  p->ws_bytes = b_counter + b_meshes + b_items + b_qoff + b_partial + b_perm + b_qs + p->sort_bytes;
+  hipError_t e = hipMalloc(&p->ws, p->ws_bytes);
+  if (e != hipSuccess) {
+    p->ws = nullptr;
+    return e;
+  }
+  char* c = (char*)p->ws;
+  p->d_counter = (int32_t*)c; c += b_counter;
+  p->d_meshes = (BatchMesh*)c; c += b_meshes;
+  p->d_items = (BatchItem*)c; c += b_items;
+  p->d_query_off = (int64_t*)c; c += b_qoff;
+  p->d_partial = (double*)c; c += b_partial;
+  if (p->sort) {
+    p->d_perm = (int32_t*)c; c += b_perm;
+    p->d_qs = (double*)c; c += b_qs;
+    p->d_sort_ws = c;
+  }
+  e = hipMemcpy(p->d_meshes, meshes.data(), sizeof(BatchMesh) * meshes.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !flat.empty())
+    e = hipMemcpy(p->d_items, flat.data(), sizeof(BatchItem) * flat.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_query_off, query_off, sizeof(int64_t) * (size_t)(n_meshes + 1),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+  return e;
+}
+
+// Mesh table and work items for units of `unit` faces (1 for face plans,
+// 255 records for strip plans): chunks of `chunk` units, query tiles of QB.
+int64_t plan_items(const int64_t* unit_off, const int64_t* query_off, int64_t n_meshes,
+                   int64_t chunk, int64_t QB, int64_t faces_per_unit, int64_t overhead,
+                   std::vector<BatchMesh>& meshes,
+                   std::vector<std::pair<int64_t, BatchItem>>& items) {
+  meshes.assign((size_t)n_meshes, BatchMesh{});
+  int64_t pbase = 0;
+  for (int64_t m = 0; m < n_meshes; ++m) {
+    BatchMesh bm{};
+    bm.face_off = unit_off[m];
+    bm.query_off = query_off[m];
+    bm.nf = (int32_t)(unit_off[m + 1] - unit_off[m]);
+    bm.nq = (int32_t)(query_off[m + 1] - query_off[m]);
+    bm.chunks = (int32_t)std::max<int64_t>(1, (bm.nf + chunk - 1) / chunk);
+    bm.pbase = pbase;
+    if (bm.chunks > 1) pbase += (int64_t)bm.nq * bm.chunks;
+    meshes[(size_t)m] = bm;
+    const int64_t tiles = (bm.nq + QB - 1) / QB;
+    for (int64_t j = 0; j < tiles; ++j) {
+      const int64_t qn = std::min<int64_t>(QB, bm.nq - j * QB);
+      for (int32_t k = 0; k < bm.chunks; ++k) {
+        const int64_t fn = std::min<int64_t>(chunk, bm.nf - (int64_t)k * chunk) * faces_per_unit;
+        // cost in pairs, plus a fixed per-item overhead
+        const int64_t cost = qn * std::max<int64_t>(fn, 0) + overhead;
+        items.push_back({cost, BatchItem{(int32_t)m, (int32_t)j, k, 0}});
+      }
+    }
+  }
+  return pbase;
+}
+
+}  // namespace
+
 hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
                         int precision, int qpl, int64_t chunk_faces, int sort, float kappa,
                         BatchPlan** out) {
@@ -214,36 +289,10 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
   p->query_off.assign(query_off, query_off + n_meshes + 1);
 
   const int64_t QB = (int64_t)kBlockThreads * qpl;
-  std::vector<BatchMesh> meshes((size_t)n_meshes);
+  std::vector<BatchMesh> meshes;
   std::vector<std::pair<int64_t, BatchItem>> items;
-  int64_t pbase = 0;
-  for (int64_t m = 0; m < n_meshes; ++m) {
-    BatchMesh bm{};
-    bm.face_off = face_off[m];
-    bm.query_off = query_off[m];
-    bm.nf = (int32_t)(face_off[m + 1] - face_off[m]);
-    bm.nq = (int32_t)(query_off[m + 1] - query_off[m]);
-    bm.chunks = (int32_t)std::max<int64_t>(1, (bm.nf + chunk_faces - 1) / chunk_faces);
-    bm.pbase = pbase;
-    if (bm.chunks > 1) pbase += (int64_t)bm.nq * bm.chunks;
-    meshes[(size_t)m] = bm;
-    const int64_t tiles = (bm.nq + QB - 1) / QB;
-    for (int64_t j = 0; j < tiles; ++j) {
-      const int64_t qn = std::min<int64_t>(QB, bm.nq - j * QB);
-      for (int32_t k = 0; k < bm.chunks; ++k) {
-        const int64_t fn = std::min<int64_t>(chunk_faces, bm.nf - (int64_t)k * chunk_faces);
-        // cost in pairs, plus a fixed per-item overhead of one face tile
-        const int64_t cost = qn * std::max<int64_t>(fn, 0) + QB * kFaceTile;
-        items.push_back({cost, BatchItem{(int32_t)m, (int32_t)j, k, 0}});
-      }
-    }
-  }
-  std::stable_sort(items.begin(), items.end(),
-                   [](const auto& a, const auto& b) { return a.first > b.first; });
-  p->n_items = (int64_t)items.size();
-  p->n_partials = pbase;
-  std::vector<BatchItem> flat(items.size());
-  for (size_t i = 0; i < items.size(); ++i) flat[i] = items[i].second;
+  const int64_t pbase = plan_items(face_off, query_off, n_meshes, chunk_faces, QB, 1,
+                                   QB * kFaceTile, meshes, items);
 
   // persistent grid: CUs x resident blocks of this kernel instance
   int dev = 0, cus = 0, per_cu = 0;
@@ -252,47 +301,10 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
   if (e == hipSuccess)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_ptr(precision, qpl),
                                                      kBlockThreads, 0);
-  if (e != hipSuccess) {
-    delete p;
-    return e;
-  }
-  int64_t grid = (int64_t)std::max(1, cus) * std::max(1, per_cu);
-  p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(grid, std::max<int64_t>(1, p->n_items)));
-
-  const size_t b_counter = align_up(sizeof(int32_t));
-  const size_t b_meshes = align_up(sizeof(BatchMesh) * (size_t)n_meshes);
-  const size_t b_items = align_up(sizeof(BatchItem) * std::max<size_t>(1, flat.size()));
-  const size_t b_qoff = align_up(sizeof(int64_t) * (size_t)(n_meshes + 1));
-  const size_t b_partial = align_up(sizeof(double) * (size_t)std::max<int64_t>(1, pbase));
-  const size_t b_perm = sort ? align_up(sizeof(int32_t) * (size_t)std::max<int64_t>(1, p->qtot)) : 0;
-  const size_t b_qs = sort ? align_up(3 * sizeof(double) * (size_t)std::max<int64_t>(1, p->qtot)) : 0;
-  p->sort_bytes = sort ? morton_segmented_workspace_bytes(p->qtot) : 0;
-  p->ws_bytes = b_counter + b_meshes + b_items + b_qoff + b_partial + b_perm + b_qs + p->sort_bytes;
-  e = hipMalloc(&p->ws, p->ws_bytes);
-  if (e != hipSuccess) {
-    delete p;
-    return e;
-  }
-  char* c = (char*)p->ws;
-  p->d_counter = (int32_t*)c; c += b_counter;
-  p->d_meshes = (BatchMesh*)c; c += b_meshes;
-  p->d_items = (BatchItem*)c; c += b_items;
-  p->d_query_off = (int64_t*)c; c += b_qoff;
-  p->d_partial = (double*)c; c += b_partial;
-  if (sort) {
-    p->d_perm = (int32_t*)c; c += b_perm;
-    p->d_qs = (double*)c; c += b_qs;
-    p->d_sort_ws = c;
-  }
-  e = hipMemcpy(p->d_meshes, meshes.data(), sizeof(BatchMesh) * meshes.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !flat.empty())
-    e = hipMemcpy(p->d_items, flat.data(), sizeof(BatchItem) * flat.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(p->d_query_off, query_off, sizeof(int64_t) * (size_t)(n_meshes + 1),
-                  hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+    e = plan_finish(p, meshes, items, pbase, query_off, std::max(1, cus) * std::max(1, per_cu));
   if (e != hipSuccess) {
-    (void)hipFree(p->ws);
+    if (p->ws) (void)hipFree(p->ws);
     delete p;
     return e;
   }
@@ -300,11 +312,56 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
   return hipSuccess;
 }
 
-static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q, double* dst,
-                                hipStream_t s) {
-  counter_reset_kernel<<<1, 64, 0, s>>>(p->d_counter);
-  const int32_t* perm = p->sort ? p->d_perm : nullptr;
-  if (p->sort) q = p->d_qs;
+const char* batch_strip_check(const int64_t* rec_off, const int64_t* query_off, int64_t n_meshes,
+                              int64_t chunk_tiles) {
+  if (n_meshes < 1 || n_meshes > (1 << 24)) return "n_meshes must be in [1, 2^24]";
+  if (!rec_off || !query_off) return "null offsets";
+  if (chunk_tiles < 1) return "chunk_tiles must be positive";
+  if (rec_off[0] != 0 || query_off[0] != 0) return "offsets must start at 0";
+  for (int64_t m = 0; m < n_meshes; ++m) {
+    if (rec_off[m + 1] < rec_off[m] || query_off[m + 1] < query_off[m]) return "offsets must be non-decreasing";
+    if ((rec_off[m + 1] - rec_off[m]) % 255) return "each mesh's record count must be a multiple of 255";
+    if ((rec_off[m + 1] - rec_off[m]) / 255 > 0x7fffffff || query_off[m + 1] - query_off[m] > 0x7fffffff)
+      return "a mesh has more than 2^31-1 tiles or queries";
+  }
+  if (query_off[n_meshes] > 0x7fffffff) return "more than 2^31-1 queries in the batch";
+  return nullptr;
+}
+
+hipError_t batch_create_strips(const int64_t* rec_off, const int64_t* query_off, int64_t n_meshes,
+                               int64_t chunk_tiles, int sort, float kappa, BatchPlan** out) {
+  BatchPlan* p = new BatchPlan();
+  p->strips = true;
+  p->precision = kFP32C;
+  p->qpl = 1;
+  p->sort = sort;
+  p->kappa = kappa;
+  p->chunk_faces = chunk_tiles;  // in tiles
+  p->n_meshes = (int)n_meshes;
+  p->ftot = rec_off[n_meshes] / 255;  // tiles
+  p->qtot = query_off[n_meshes];
+  p->query_off.assign(query_off, query_off + n_meshes + 1);
+
+  std::vector<int64_t> tile_off((size_t)n_meshes + 1);
+  for (int64_t m = 0; m <= n_meshes; ++m) tile_off[(size_t)m] = rec_off[m] / 255;
+  std::vector<BatchMesh> meshes;
+  std::vector<std::pair<int64_t, BatchItem>> items;
+  const int64_t pbase = plan_items(tile_off.data(), query_off, n_meshes, chunk_tiles,
+                                   kBlockThreads, 255, kBlockThreads * 255, meshes, items);
+  const int resident = batch_strip_resident_blocks();
+  hipError_t e = resident > 0 ? hipSuccess : hipErrorNoDevice;
+  if (e == hipSuccess) e = plan_finish(p, meshes, items, pbase, query_off, resident);
+  if (e != hipSuccess) {
+    if (p->ws) (void)hipFree(p->ws);
+    delete p;
+    return e;
+  }
+  *out = p;
+  return hipSuccess;
+}
+
+static void face_batch_launch(const BatchPlan* p, const double* tri, const double* q,
+                              const int32_t* perm, double* dst, hipStream_t s) {
   const dim3 grid((unsigned)p->grid);
 #define WN_BATCH_LAUNCH(PREC, QPL)                                                             \
   batch_kernel<PREC, QPL><<<grid, kBlockThreads, 0, s>>>(tri, p->ftot, q, perm, dst,          \
@@ -318,6 +375,22 @@ static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q
     if (p->qpl == 1) WN_BATCH_LAUNCH(kFP32C, 1); else WN_BATCH_LAUNCH(kFP32C, 2);
   }
 #undef WN_BATCH_LAUNCH
+}
+
+// tri: the 9 x F_total face SoA, or the strip records of a strip plan.
+static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q, double* dst,
+                                hipStream_t s) {
+  counter_reset_kernel<<<1, 64, 0, s>>>(p->d_counter);
+  const int32_t* perm = p->sort ? p->d_perm : nullptr;
+  if (p->sort) q = p->d_qs;
+  if (p->strips) {
+    const hipError_t e = batch_strip_enqueue((const double4*)tri, q, perm, dst, p->d_partial,
+                                             p->d_meshes, p->d_items, p->n_items, p->d_counter,
+                                             p->chunk_faces, p->kappa, p->grid, s);
+    if (e != hipSuccess) return e;
+  } else {
+    face_batch_launch(p, tri, q, perm, dst, s);
+  }
   if (p->n_partials > 0 && p->qtot > 0)
     batch_reduce_kernel<<<dim3((unsigned)((p->qtot + kBlockThreads - 1) / kBlockThreads)),
                           kBlockThreads, 0, s>>>(p->d_partial, perm, dst, p->d_meshes,

@@ -85,6 +85,36 @@ hipError_t scatter_corners_launch(const double* gtri, const void* F, int fdtype,
 namespace wn {
 
 // Batched persistent forward (wn_batch.hip).
+//
+// Per mesh: its first face (first strip tile for strip plans), first query,
+// first partial slot (chunks > 1), and counts. Face units are faces for face
+// plans and 255-record tiles for strip plans.
+struct BatchMesh {
+  int64_t face_off;
+  int64_t query_off;
+  int64_t pbase;
+  int32_t nf;
+  int32_t nq;
+  int32_t chunks;
+  int32_t pad;
+};
+
+// One work item: a query tile of a mesh against one face (tile) chunk.
+struct BatchItem {
+  int32_t mesh;
+  int32_t qtile;
+  int32_t chunk;
+  int32_t pad;
+};
+
+// Persistent strip kernel over a work list (wn_strip.hip): one query per
+// lane, blocks of kBlockThreads queries, chunks of chunk_tiles tiles.
+int batch_strip_resident_blocks();
+hipError_t batch_strip_enqueue(const double4* rec, const double* q, const int32_t* perm,
+                               double* dst, double* partial, const BatchMesh* meshes,
+                               const BatchItem* items, int64_t n_items, int32_t* counter,
+                               int64_t chunk_tiles, float kappa, int grid, hipStream_t stream);
+
 struct BatchPlan;
 const char* batch_check(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
                         int precision, int qpl, int64_t chunk_faces);
@@ -93,6 +123,12 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
                         BatchPlan** out);
 hipError_t batch_run(BatchPlan* p, const double* tri, const double* q, double* dst,
                      bool use_graph, hipStream_t stream);
+// Strip plans: rec_off in records (each mesh a multiple of 255), chunks of
+// chunk_tiles tiles; FP32-compensated, one query per lane.
+const char* batch_strip_check(const int64_t* rec_off, const int64_t* query_off, int64_t n_meshes,
+                              int64_t chunk_tiles);
+hipError_t batch_create_strips(const int64_t* rec_off, const int64_t* query_off, int64_t n_meshes,
+                               int64_t chunk_tiles, int sort, float kappa, BatchPlan** out);
 void batch_destroy(BatchPlan* p);
 // out6 = {items, partial slots, workspace bytes, persistent grid, faces, queries}
 void batch_info(const BatchPlan* p, int64_t* out6);

@@ -112,6 +112,59 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, long long
   s_carry = s_qr;
 }
 
+// Block-local origin: centre of the bbox of the block's valid queries (the
+// same reduction as the face-record kernels). Begins and ends with a barrier.
+__device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double Oy, double Oz,
+                                                   StripSmem& sm, double& gx, double& gy,
+                                                   double& gz) {
+  const int tid = threadIdx.x;
+  double lo0 = valid ? Ox : __builtin_inf(), hi0 = valid ? Ox : -__builtin_inf();
+  double lo1 = valid ? Oy : __builtin_inf(), hi1 = valid ? Oy : -__builtin_inf();
+  double lo2 = valid ? Oz : __builtin_inf(), hi2 = valid ? Oz : -__builtin_inf();
+  lo0 = wave_min_f64(lo0); lo1 = wave_min_f64(lo1); lo2 = wave_min_f64(lo2);
+  hi0 = wave_max_f64(hi0); hi1 = wave_max_f64(hi1); hi2 = wave_max_f64(hi2);
+  const int wave = tid / kWave;
+  __syncthreads();
+  if ((tid & (kWave - 1)) == 0) {
+    sm.red[0][wave] = lo0; sm.red[1][wave] = lo1; sm.red[2][wave] = lo2;
+    sm.red[3][wave] = hi0; sm.red[4][wave] = hi1; sm.red[5][wave] = hi2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kBlockThreads / kWave; ++w) {
+    lo0 = fmin(lo0, sm.red[0][w]); lo1 = fmin(lo1, sm.red[1][w]); lo2 = fmin(lo2, sm.red[2][w]);
+    hi0 = fmax(hi0, sm.red[3][w]); hi1 = fmax(hi1, sm.red[4][w]); hi2 = fmax(hi2, sm.red[5][w]);
+  }
+  gx = 0.5 * (lo0 + hi0);
+  gy = 0.5 * (lo1 + hi1);
+  gz = 0.5 * (lo2 + hi2);
+  __syncthreads();
+}
+
+// Unscaled angle sum of one query (block-local FP32 offset o) over strip
+// tiles [t_begin, t_end). All threads of the block call it together.
+__device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, int64_t t_begin,
+                                              int64_t t_end, double gx, double gy, double gz,
+                                              float ox, float oy, float oz, float kappa,
+                                              StripSmem& sm) {
+  double acc = 0.0;
+  VSlot A{0.f, 0.f, 0.f, 0.f, 0.f}, B = A, C = A;
+  float s_carry = 0.0f;
+  for (int64_t t = t_begin; t < t_end; ++t) {
+    __syncthreads();
+    stage_strip_tile(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
+    __syncthreads();
+    const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
+    for (int j = 0; j < kStripTileRecords; j += 3) {
+      strip_step(sm, j, flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j + 1, flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j + 2, flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz, kappa);
+    }
+  }
+  __syncthreads();
+  return acc;
+}
+
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8))) void fwd_strip_kernel(
     const double4* __restrict__ rec, int64_t ntiles, const double* __restrict__ q,
     const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nq,
@@ -126,54 +179,65 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
   double Oy[1] = {valid[0] ? q[3 * slot + 1] : 0.0};
   double Oz[1] = {valid[0] ? q[3 * slot + 2] : 0.0};
 
-  // block-local origin (same reduction as the face-record kernel)
   double gx, gy, gz;
-  {
-    double lo0 = valid[0] ? Ox[0] : __builtin_inf(), hi0 = valid[0] ? Ox[0] : -__builtin_inf();
-    double lo1 = valid[0] ? Oy[0] : __builtin_inf(), hi1 = valid[0] ? Oy[0] : -__builtin_inf();
-    double lo2 = valid[0] ? Oz[0] : __builtin_inf(), hi2 = valid[0] ? Oz[0] : -__builtin_inf();
-    lo0 = wave_min_f64(lo0); lo1 = wave_min_f64(lo1); lo2 = wave_min_f64(lo2);
-    hi0 = wave_max_f64(hi0); hi1 = wave_max_f64(hi1); hi2 = wave_max_f64(hi2);
-    const int wave = tid / kWave;
-    if ((tid & (kWave - 1)) == 0) {
-      sm.red[0][wave] = lo0; sm.red[1][wave] = lo1; sm.red[2][wave] = lo2;
-      sm.red[3][wave] = hi0; sm.red[4][wave] = hi1; sm.red[5][wave] = hi2;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int w = 0; w < kBlockThreads / kWave; ++w) {
-      lo0 = fmin(lo0, sm.red[0][w]); lo1 = fmin(lo1, sm.red[1][w]); lo2 = fmin(lo2, sm.red[2][w]);
-      hi0 = fmax(hi0, sm.red[3][w]); hi1 = fmax(hi1, sm.red[4][w]); hi2 = fmax(hi2, sm.red[5][w]);
-    }
-    gx = 0.5 * (lo0 + hi0);
-    gy = 0.5 * (lo1 + hi1);
-    gz = 0.5 * (lo2 + hi2);
-  }
+  strip_block_origin(valid[0], Ox[0], Oy[0], Oz[0], sm, gx, gy, gz);
   const float ox = valid[0] ? (float)(Ox[0] - gx) : 0.0f;
   const float oy = valid[0] ? (float)(Oy[0] - gy) : 0.0f;
   const float oz = valid[0] ? (float)(Oz[0] - gz) : 0.0f;
 
-  double acc = 0.0;
-  VSlot A{0.f, 0.f, 0.f, 0.f, 0.f}, B = A, C = A;
-  float s_carry = 0.0f;
   const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
   const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
-  for (int64_t t = t_begin; t < t_end; ++t) {
-    __syncthreads();
-    stage_strip_tile(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
-    __syncthreads();
-    const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
-    for (int j = 0; j < kStripTileRecords; j += 3) {
-      strip_step(sm, j, flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz, kappa);
-      strip_step(sm, j + 1, flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz, kappa);
-      strip_step(sm, j + 2, flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz, kappa);
-    }
-  }
+  const double acc = strip_sweep(rec, t_begin, t_end, gx, gy, gz, ox, oy, oz, kappa, sm);
   if (valid[0]) {
     if (splits == 1)
       out[qidx] = acc * kInv2Pi;
     else
       out[(int64_t)blockIdx.y * nq + slot] = acc;
+  }
+}
+
+// Persistent strip kernel over a work list (mesh, query tile, tile chunk):
+// the strip counterpart of the face-record batch kernel, with the same
+// item protocol (atomic counter, block-uniform exit) and the same output
+// rule (single-chunk meshes write w, others write unscaled chunk-major
+// partials summed in chunk order afterwards).
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8))) void batch_strip_kernel(
+    const double4* __restrict__ rec, const double* __restrict__ q,
+    const int32_t* __restrict__ perm, double* __restrict__ dst, double* __restrict__ partial,
+    const BatchMesh* __restrict__ meshes, const BatchItem* __restrict__ items, int64_t n_items,
+    int32_t* __restrict__ counter, int64_t chunk_tiles, float kappa) {
+  __shared__ StripSmem sm;
+  __shared__ int32_t s_item;
+  const int tid = threadIdx.x;
+  for (;;) {
+    __syncthreads();  // every thread has read the previous s_item
+    if (tid == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int32_t it = s_item;
+    if (it >= n_items) break;  // block-uniform exit
+    const BatchItem item = items[it];
+    const BatchMesh m = meshes[item.mesh];
+    const int64_t slot = m.query_off + (int64_t)item.qtile * kBlockThreads + tid;
+    const bool valid = slot < m.query_off + m.nq;
+    // q is in launch order (gathered through perm beforehand when sorted)
+    const double Ox = valid ? q[3 * slot] : 0.0;
+    const double Oy = valid ? q[3 * slot + 1] : 0.0;
+    const double Oz = valid ? q[3 * slot + 2] : 0.0;
+    double gx, gy, gz;
+    strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz);
+    const float ox = valid ? (float)(Ox - gx) : 0.0f;
+    const float oy = valid ? (float)(Oy - gy) : 0.0f;
+    const float oz = valid ? (float)(Oz - gz) : 0.0f;
+    const int64_t t0 = m.face_off + (int64_t)item.chunk * chunk_tiles;
+    int64_t t1 = t0 + chunk_tiles;
+    if (t1 > m.face_off + m.nf) t1 = m.face_off + m.nf;
+    const double acc = strip_sweep(rec, t0, t1, gx, gy, gz, ox, oy, oz, kappa, sm);
+    if (valid) {
+      if (m.chunks == 1)
+        dst[perm ? (int64_t)perm[slot] : slot] = acc * kInv2Pi;
+      else  // chunk-major, slot order: a block's writes are contiguous
+        partial[m.pbase + (int64_t)item.chunk * m.nq + (slot - m.query_off)] = acc;
+    }
   }
 }
 
@@ -233,6 +297,27 @@ hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, doubl
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || used_splits == 1) return e;
   return reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
+}
+
+int batch_strip_resident_blocks() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)batch_strip_kernel,
+                                                   kBlockThreads, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return cus * per_cu;
+}
+
+hipError_t batch_strip_enqueue(const double4* rec, const double* q, const int32_t* perm,
+                               double* dst, double* partial, const BatchMesh* meshes,
+                               const BatchItem* items, int64_t n_items, int32_t* counter,
+                               int64_t chunk_tiles, float kappa, int grid, hipStream_t stream) {
+  batch_strip_kernel<<<dim3((unsigned)grid), kBlockThreads, 0, stream>>>(
+      rec, q, perm, dst, partial, meshes, items, n_items, counter, chunk_tiles, kappa);
+  return hipGetLastError();
 }
 
 }  // namespace wn

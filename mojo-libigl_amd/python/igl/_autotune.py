@@ -133,6 +133,35 @@ def tune_batch(face_counts, query_counts, tri, Q, precision: int, repeats: int =
     return best
 
 
+def tune_batch_strips(record_counts, query_counts, rec, Q, repeats: int = 2,
+                      use_cache: bool = True, chunks=(2, 4, 8, 16, 32)) -> int:
+    """Best chunk_tiles (255-record tiles per work item) for a strip batch."""
+    from .batch import StripBatch
+
+    rc = np.asarray(record_counts, dtype=np.int64)
+    qc = np.asarray(query_counts, dtype=np.int64)
+    sig = f"{len(rc)}:{_bucket(int(rc.sum()))}:{_bucket(int(qc.sum()))}:{_bucket(int((rc * qc).sum()))}"
+    key = f"batch_strips:{sig}"
+    data = _load() if use_cache else {}
+    if key in data:
+        return int(data[key]["config"])
+    results = []
+    for chunk in chunks:
+        plan = StripBatch(rc, qc, chunk_tiles=chunk)
+        try:
+            ms = _time(lambda: plan.run(rec, Q, use_graph=False), repeats)
+        finally:
+            plan.close()
+        results.append((ms, chunk))
+    results.sort()
+    best = int(results[0][1])
+    if use_cache:
+        data = _load()
+        data[key] = {"config": best, "ms": results[0][0], "tried": [[r[1], r[0]] for r in results]}
+        _store(data)
+    return best
+
+
 def device_name() -> str:
     try:
         import torch

@@ -46,6 +46,7 @@ _SIGNATURES = {
     "wn_mesh_forward": ([I] * 6 + [D] + [I] * 3, I),
     "wn_mesh_destroy": ([I], I),
     "wn_batch_create": ([I] * 7 + [D], I),
+    "wn_batch_create_strips": ([I] * 5 + [D], I),
     "wn_batch_run": ([I] * 6, I),
     "wn_batch_info": ([I] * 2, I),
     "wn_batch_destroy": ([I], I),

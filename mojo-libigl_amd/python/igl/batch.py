@@ -10,6 +10,14 @@ include/wn.h. Device tensors come from PyTorch; the work happens in the C ABI.
     plan.close()
 
 ``winding_number_batch`` wraps the whole sequence for host arrays.
+
+Strip plans (``StripBatch``) take the meshes as triangle-strip records built
+once on the host (``pack_batch_strips``) and run the strip kernel, which
+evaluates one vertex per face instead of three:
+
+    rec, counts = pack_batch_strips([V0, V1, ...], [F0, F1, ...])
+    plan = StripBatch(counts, query_counts)
+    w = plan.run(rec, Q_all)
 """
 
 from __future__ import annotations
@@ -61,13 +69,18 @@ class WindingBatch:
         keys = ("items", "partials", "workspace_bytes", "grid", "faces", "queries")
         return dict(zip(keys, (int(v) for v in out)))
 
-    def run(self, tri, Q, out=None, use_graph=True, stream=None):
-        """w for all queries. tri: (9, F_total) float64 device tensor, Q:
-        (Q_total, 3) float64 device tensor."""
+    def _check_faces(self, tri) -> None:
         import torch
 
         if tri.shape != (9, self.n_faces) or tri.dtype != torch.float64 or not tri.is_contiguous():
             raise ValueError("tri must be a contiguous (9, F_total) float64 tensor")
+
+    def run(self, tri, Q, out=None, use_graph=True, stream=None):
+        """w for all queries. tri: (9, F_total) float64 device tensor (the
+        records of a strip plan), Q: (Q_total, 3) float64 device tensor."""
+        import torch
+
+        self._check_faces(tri)
         if Q.shape != (self.n_queries, 3) or Q.dtype != torch.float64 or not Q.is_contiguous():
             raise ValueError("Q must be a contiguous (Q_total, 3) float64 tensor")
         if out is None:
@@ -91,6 +104,50 @@ class WindingBatch:
             pass
 
 
+class StripBatch(WindingBatch):
+    """A planned batch over meshes in strip-record form (FP32-compensated)."""
+
+    def __init__(self, record_counts, query_counts, chunk_tiles=0, sort=-1, kappa=-1.0):
+        rc = np.asarray(record_counts, dtype=np.int64)
+        qc = np.asarray(query_counts, dtype=np.int64)
+        if rc.ndim != 1 or rc.shape != qc.shape or len(rc) == 0:
+            raise ValueError("record_counts and query_counts must be equal-length 1-D sequences")
+        if (rc < 0).any() or (qc < 0).any() or (rc % 255).any():
+            raise ValueError("record counts must be non-negative multiples of 255")
+        self.face_off = np.concatenate([[0], np.cumsum(rc)]).astype(np.int64)
+        self.query_off = np.concatenate([[0], np.cumsum(qc)]).astype(np.int64)
+        self.precision = PRECISIONS["fp32c"]
+        self.config = dict(chunk_tiles=int(chunk_tiles), sort=int(sort), kappa=float(kappa))
+        handle = lib().wn_batch_create_strips(self.face_off.ctypes.data, self.query_off.ctypes.data,
+                                              len(rc), int(chunk_tiles), int(sort), float(kappa))
+        if handle <= 0:
+            raise RuntimeError(f"wn_batch_create_strips failed: {last_error()}")
+        self.handle = handle
+
+    @property
+    def n_records(self) -> int:
+        return int(self.face_off[-1])
+
+    def _check_faces(self, rec) -> None:
+        import torch
+
+        if rec.shape != (self.n_records, 4) or rec.dtype != torch.float64 or not rec.is_contiguous():
+            raise ValueError("records must be a contiguous (R_total, 4) float64 tensor")
+
+
+def pack_batch_strips(vertices, faces, device="cuda"):
+    """Strip records of every mesh (host stripifier), concatenated on the
+    device: ((R_total, 4) float64 tensor, per-mesh record counts)."""
+    import torch
+
+    from .mesh import strip_records
+
+    recs = [strip_records(V, F) for V, F in zip(vertices, faces)]
+    counts = np.array([len(r) for r in recs], dtype=np.int64)
+    rec = np.concatenate(recs) if recs else np.zeros((0, 4))
+    return torch.as_tensor(np.ascontiguousarray(rec), device=device), counts
+
+
 def pack_batch(vertices, faces, device="cuda"):
     """Concatenate meshes (V_i, F_i) and gather them on the device into one
     9 x F_total float64 SoA (indices offset per mesh)."""
@@ -108,15 +165,21 @@ def pack_batch(vertices, faces, device="cuda"):
     return pack_faces(torch.cat(Vs), torch.cat(Fs))
 
 
-def winding_number_batch(vertices, faces, queries, precision="fp32c", use_graph=False, **config):
-    """Host convenience: lists of (V_i, F_i, O_i) -> list of w_i (numpy)."""
+def winding_number_batch(vertices, faces, queries, precision="fp32c", use_graph=False,
+                         strips=False, **config):
+    """Host convenience: lists of (V_i, F_i, O_i) -> list of w_i (numpy).
+    strips=True runs an FP32-compensated strip plan (precision ignored)."""
     import torch
 
-    tri = pack_batch(vertices, faces)
     Q = torch.as_tensor(np.concatenate([np.asarray(o, dtype=np.float64).reshape(-1, 3)
                                         for o in queries]), device="cuda")
-    plan = WindingBatch([len(f) for f in faces], [len(np.asarray(o).reshape(-1, 3)) for o in queries],
-                        precision=precision, **config)
+    qc = [len(np.asarray(o).reshape(-1, 3)) for o in queries]
+    if strips:
+        tri, counts = pack_batch_strips(vertices, faces)
+        plan = StripBatch(counts, qc, **config)
+    else:
+        tri = pack_batch(vertices, faces)
+        plan = WindingBatch([len(f) for f in faces], qc, precision=precision, **config)
     try:
         w = plan.run(tri, Q.contiguous(), use_graph=use_graph).cpu().numpy()
     finally:

@@ -122,3 +122,59 @@ def test_config4_full_size(require_gpu):
           flush=True)
     assert closed_err <= 1e-6
     assert open_err <= 1e-5
+
+
+@pytest.mark.parametrize("chunk_tiles", [1, 8, 0])
+def test_strip_batch_matches_oracle(require_gpu, small_batch, chunk_tiles):
+    """Strip plans (records per mesh, FP32-compensated) against the oracle,
+    with many chunks per mesh (chunk_tiles=1), a few, and the default."""
+    w = _run(small_batch, "fp32c", strips=True, chunk_tiles=chunk_tiles)
+    worst = 0.0
+    for (V, F, Q), a in zip(small_batch, w):
+        assert a.shape == (len(Q),)
+        if len(Q) == 0:
+            continue
+        ref = oracle.winding(V.astype(np.float64), F, Q)
+        err = np.abs(a - ref).max()
+        worst = max(worst, err)
+        assert err <= (1e-6 if _closed(F, len(V)) else 1e-5)
+    print(f"strip batch chunk_tiles={chunk_tiles}: worst {worst:.2e}", flush=True)
+
+
+def test_strip_batch_graph_replays_match_eager(require_gpu, small_batch):
+    from igl.batch import StripBatch, pack_batch_strips
+
+    B = small_batch
+    rec, counts = pack_batch_strips([b[0] for b in B], [b[1] for b in B])
+    assert counts[5] == 0  # the mesh without faces has no records
+    Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
+    plan = StripBatch(counts, [len(b[2]) for b in B], chunk_tiles=2)
+    try:
+        eager = plan.run(rec, Q, use_graph=False).clone()
+        for i in range(4):
+            torch.randn(1000, device="cuda").sum().item()
+            out = plan.run(rec, Q, use_graph=True)
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager), f"replay {i}"
+        info = plan.info()
+        assert info["items"] > 0 and info["partials"] > 0
+        assert info["faces"] == int(counts.sum()) // 255
+    finally:
+        plan.close()
+
+
+def test_strip_batch_equals_single_mesh_strip_launches(require_gpu, small_batch):
+    """Same records, same query blocks (both unsorted), same face order: a
+    one-chunk strip plan reproduces igl.Mesh without splits."""
+    from igl.mesh import Mesh
+
+    w = _run(small_batch, "fp32c", strips=True, chunk_tiles=1 << 20, sort=0)
+    for (V, F, Q), a in zip(small_batch, w):
+        if len(Q) == 0 or len(F) == 0:
+            continue
+        m = Mesh(V, F)
+        try:
+            single = m.winding_number(Q, splits=1, sort=0)
+        finally:
+            m.close()
+        assert np.abs(single - a).max() <= 1e-12
