@@ -46,6 +46,19 @@ void mli_winding(const double *tri, const double *q, double *dst, int64_t nf,
 int64_t mli_winding_gpu(const double *tri, const double *q, double *dst,
                         int64_t nf, int64_t nq);
 
+/* Replaces mli_aabb_squared_distance, src/capi.mojo:950-959 (bound at
+ * python/igl/_lib.py:24 as 14 x c_int64). CPU, FP64: for queries
+ * p[q0:q1] (np x 3), the descent of src/capi.mojo:523-587 over the tree
+ * arrays boxes (nnodes x 6), nodes (nnodes x 4), order (wn_aabb_build or the
+ * reference's python/igl/__init__.py:286-319) writes sqrd, indices (int64)
+ * and closest (np x 3). (host) all buffers; stack holds >= stack_size
+ * entries and is owned by one caller at a time. */
+void mli_aabb_squared_distance(const double *p, const double *v, const int64_t *f,
+                               const double *boxes, const int64_t *nodes,
+                               const int64_t *order, double *sqrd, int64_t *indices,
+                               double *closest, int64_t *stack, int64_t np,
+                               int64_t nnodes, int64_t q0, int64_t q1);
+
 /* ---- Status ------------------------------------------------------------- */
 
 /* No reference counterpart (the reference probes nvidia-smi from Python,
@@ -181,6 +194,42 @@ int64_t wn_batch_info(int64_t handle, int64_t *out6);
 
 /* Frees a plan (synchronises the device first). */
 int64_t wn_batch_destroy(int64_t handle);
+
+/* ---- Point-mesh squared distance -------------------------------------- */
+
+/* Replaces the reference's NumPy tree build, python/igl/__init__.py:286-319
+ * (AABB.init): the median-split tree over V (nv x 3 f64, host) and F (nf x 3,
+ * fdtype 0 = int64, 1 = int32), leaves of <= leaf_size faces (0 = 8). Returns
+ * the node count; when boxes (count x 6), nodes (count x 4), order (nf) are
+ * all given and capacity >= count, fills them and *stack_size. Negative
+ * status on bad arguments (IndexError-class: a face index outside V). */
+int64_t wn_aabb_build(const double *V, const void *F, int64_t nv, int64_t nf,
+                      int64_t fdtype, int64_t leaf_size, double *boxes, int64_t *nodes,
+                      int64_t *order, int64_t capacity, int64_t *stack_size);
+
+/* No reference counterpart (the reference has no GPU distance query). Builds
+ * the same tree and keeps it on the current device (nodes, face corners in
+ * leaf order, face ids). Returns a handle (> 0) or a negative status. */
+int64_t wn_aabb_create(const double *V, const void *F, int64_t nv, int64_t nf,
+                       int64_t fdtype, int64_t leaf_size);
+
+/* out4 = {faces, nodes, stack size, device bytes}. */
+int64_t wn_aabb_info(int64_t handle, int64_t *out4);
+
+/* Workspace bytes for wn_aabb_query (sort -1 = Morton order when np >= 4096). */
+int64_t wn_aabb_workspace_bytes(int64_t handle, int64_t np, int64_t sort);
+
+/* GPU counterpart of mli_aabb_squared_distance over a wn_aabb_create tree:
+ * (device) p (np x 3 f64), sqrd (np), indices (np int64), closest (np x 3);
+ * every output equals the CPU path's bit for bit (same descent, same
+ * arithmetic, FMA contraction off). A tree without faces gives +inf, -1, 0.
+ * Asynchronous on `stream`; allocates nothing. */
+int64_t wn_aabb_query(int64_t handle, const double *p, double *sqrd, int64_t *indices,
+                      double *closest, int64_t np, int64_t sort, void *workspace,
+                      int64_t workspace_bytes, void *stream);
+
+/* Frees the device tree (synchronises the device first). */
+int64_t wn_aabb_destroy(int64_t handle);
 
 /* ---- Adjoint ------------------------------------------------------------ */
 

@@ -242,6 +242,46 @@ MeshHandle* lookup_mesh(int64_t handle) {
   return it == mesh_registry().end() ? nullptr : *it;
 }
 
+// A device-resident distance tree (wn_aabb_*).
+struct AabbHandle {
+  int64_t nf = 0;
+  int64_t nnodes = 0;
+  int64_t stack_size = 0;
+  size_t bytes = 0;
+  void* dev = nullptr;
+};
+
+std::mutex& aabb_registry_mu() {
+  static std::mutex mu;
+  return mu;
+}
+
+std::set<AabbHandle*>& aabb_registry() {
+  static std::set<AabbHandle*> live;
+  return live;
+}
+
+AabbHandle* lookup_aabb(int64_t handle) {
+  std::lock_guard<std::mutex> lock(aabb_registry_mu());
+  auto it = aabb_registry().find((AabbHandle*)(intptr_t)handle);
+  return it == aabb_registry().end() ? nullptr : *it;
+}
+
+// Shared argument checks of the tree entry points; nullptr if fine.
+const char* aabb_args(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype,
+                      int64_t leaf) {
+  if (nv < 0 || nf < 0) return "negative size";
+  if (nf > (int64_t)0x7fffffff) return "more than 2^31-1 faces";
+  if (fdtype != 0 && fdtype != 1) return "fdtype must be 0 (int64) or 1 (int32)";
+  if (leaf < 1) return "leaf_size must be positive";
+  if (nf > 0 && (!V || !F)) return "null host buffer";
+  for (int64_t i = 0; i < 3 * nf; ++i) {
+    const int64_t v = fdtype == 0 ? ((const int64_t*)F)[i] : (int64_t)((const int32_t*)F)[i];
+    if (v < 0 || v >= nv) return "face index is outside V";
+  }
+  return nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -542,6 +582,107 @@ int64_t wn_scatter_corners(const double* grad_tri, const void* F, int64_t fdtype
   hipError_t e = wn::scatter_corners_launch(grad_tri, F, (int)fdtype, nf, nv, grad_V,
                                             (hipStream_t)stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_scatter_corners", e));
+  return kOk;
+}
+
+int64_t wn_aabb_build(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype,
+                      int64_t leaf_size, double* boxes, int64_t* nodes, int64_t* order,
+                      int64_t capacity, int64_t* stack_size) {
+  const int64_t leaf = leaf_size <= 0 ? 8 : leaf_size;
+  if (const char* why = aabb_args(V, F, nv, nf, fdtype, leaf)) return fail(kErrArg, why);
+  const int64_t count = wn::aabb_node_count(nf, leaf);
+  if (!boxes || !nodes || !order || capacity < count) return count;
+  wn::AabbTree t;
+  wn::aabb_build(V, F, (int)fdtype, nf, leaf, t);
+  std::memcpy(boxes, t.boxes.data(), t.boxes.size() * sizeof(double));
+  std::memcpy(nodes, t.nodes.data(), t.nodes.size() * sizeof(int64_t));
+  std::memcpy(order, t.order.data(), t.order.size() * sizeof(int64_t));
+  if (stack_size) *stack_size = t.stack_size;
+  return count;
+}
+
+int64_t wn_aabb_create(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype,
+                       int64_t leaf_size) {
+  const int64_t leaf = leaf_size <= 0 ? 8 : leaf_size;
+  if (const char* why = aabb_args(V, F, nv, nf, fdtype, leaf)) return fail(kErrArg, why);
+  if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  wn::AabbTree t;
+  wn::aabb_build(V, F, (int)fdtype, nf, leaf, t);
+  if (t.stack_size > wn::kAabbStack) return fail(kErrArg, "tree deeper than the descent stack");
+  AabbHandle* h = new AabbHandle();
+  h->nf = nf;
+  h->nnodes = (int64_t)t.nodes.size() / 4;
+  h->stack_size = t.stack_size;
+  h->bytes = wn::aabb_device_bytes(h->nnodes, nf);
+  if (h->nnodes > 0) {
+    hipStream_t s = nullptr;
+    hipError_t e = hipMalloc(&h->dev, h->bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = wn::aabb_upload(t, V, F, (int)fdtype, h->dev, s);
+    if (s) (void)hipStreamDestroy(s);
+    if (e != hipSuccess) {
+      if (h->dev) (void)hipFree(h->dev);
+      delete h;
+      return fail(kErrHip, hip_msg("wn_aabb_create", e));
+    }
+  }
+  std::lock_guard<std::mutex> lock(aabb_registry_mu());
+  aabb_registry().insert(h);
+  return (int64_t)(intptr_t)h;
+}
+
+int64_t wn_aabb_info(int64_t handle, int64_t* out4) {
+  AabbHandle* h = lookup_aabb(handle);
+  if (!h) return fail(kErrArg, "unknown tree handle");
+  if (!out4) return fail(kErrArg, "null output");
+  out4[0] = h->nf;
+  out4[1] = h->nnodes;
+  out4[2] = h->stack_size;
+  out4[3] = (int64_t)h->bytes;
+  return kOk;
+}
+
+int64_t wn_aabb_workspace_bytes(int64_t handle, int64_t np, int64_t sort) {
+  if (!lookup_aabb(handle)) return fail(kErrArg, "unknown tree handle");
+  if (np < 0 || np > (int64_t)0x7fffffff) return fail(kErrArg, "query count out of range");
+  const int srt = sort < 0 ? (np >= 4096 ? 1 : 0) : (sort ? 1 : 0);
+  if (srt && device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  return (int64_t)wn::aabb_workspace_bytes(np, srt);
+}
+
+int64_t wn_aabb_query(int64_t handle, const double* p, double* sqrd, int64_t* indices,
+                      double* closest, int64_t np, int64_t sort, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
+  AabbHandle* h = lookup_aabb(handle);
+  if (!h) return fail(kErrArg, "unknown tree handle");
+  if (np < 0 || np > (int64_t)0x7fffffff) return fail(kErrArg, "query count out of range");
+  if (np == 0) return kOk;
+  if (!p || !sqrd || !indices || !closest) return fail(kErrArg, "null device buffer");
+  const int srt = sort < 0 ? (np >= 4096 ? 1 : 0) : (sort ? 1 : 0);
+  const size_t need = wn::aabb_workspace_bytes(np, srt);
+  if (workspace_bytes < 0 || (size_t)workspace_bytes < need || (need > 0 && !workspace))
+    return fail(kErrWorkspace, "workspace smaller than wn_aabb_workspace_bytes()");
+  hipError_t e = wn::aabb_query_launch(h->dev, h->nnodes, h->nf, p, sqrd, indices, closest, np,
+                                       srt, workspace, (size_t)workspace_bytes,
+                                       (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_aabb_query", e));
+  return kOk;
+}
+
+int64_t wn_aabb_destroy(int64_t handle) {
+  AabbHandle* h = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(aabb_registry_mu());
+    auto it = aabb_registry().find((AabbHandle*)(intptr_t)handle);
+    if (it == aabb_registry().end()) return fail(kErrArg, "unknown tree handle");
+    h = *it;
+    aabb_registry().erase(it);
+  }
+  if (h->dev) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(h->dev);
+  }
+  delete h;
   return kOk;
 }
 

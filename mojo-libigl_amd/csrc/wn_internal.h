@@ -7,6 +7,8 @@
 
 #include <vector>
 
+#include "wn_aabb.h"
+
 namespace wn {
 
 enum Precision : int { kFP64 = 0, kFP32C = 1 };
@@ -138,5 +140,22 @@ void batch_info(const BatchPlan* p, int64_t* out6);
 size_t morton_segmented_workspace_bytes(int64_t nq);
 hipError_t morton_order_segmented(const double* q, int64_t nq, const int64_t* seg_off, int nseg,
                                   int32_t* perm, void* ws, size_t ws_bytes, hipStream_t stream);
+
+}  // namespace wn
+
+namespace wn {
+
+// Point-mesh squared distance on the GPU (wn_aabb.hip). The device copy of a
+// tree is one allocation of aabb_device_bytes() filled by aabb_upload(); a
+// descent needs at most stack_size <= kAabbStack entries per query.
+constexpr int kAabbStack = 40;
+size_t aabb_device_bytes(int64_t nnodes, int64_t nf);
+hipError_t aabb_upload(const AabbTree& t, const double* V, const void* F, int fdtype, void* dev,
+                       hipStream_t stream);
+size_t aabb_workspace_bytes(int64_t np, int sort);
+// q, sqrd, index (int64), closest: device buffers of np queries.
+hipError_t aabb_query_launch(const void* dev, int64_t nnodes, int64_t nf, const double* q,
+                             double* sqrd, int64_t* index, double* closest, int64_t np, int sort,
+                             void* ws, size_t ws_bytes, hipStream_t stream);
 
 }  // namespace wn

@@ -197,6 +197,9 @@ def winding_number_for_point(V, F, p):
     return float(winding_number(V, F, p))
 
 
+from .aabb import AABB, point_mesh_squared_distance  # noqa: E402
+
+
 def __getattr__(name):
     # Lazy: these need PyTorch only when used.
     if name == "Mesh":

@@ -47,6 +47,13 @@ _SIGNATURES = {
     "wn_mesh_destroy": ([I], I),
     "wn_batch_create": ([I] * 7 + [D], I),
     "wn_batch_create_strips": ([I] * 5 + [D], I),
+    "mli_aabb_squared_distance": ([I] * 14, None),
+    "wn_aabb_build": ([I] * 11, I),
+    "wn_aabb_create": ([I] * 6, I),
+    "wn_aabb_info": ([I] * 2, I),
+    "wn_aabb_workspace_bytes": ([I] * 3, I),
+    "wn_aabb_query": ([I] * 10, I),
+    "wn_aabb_destroy": ([I], I),
     "wn_batch_run": ([I] * 6, I),
     "wn_batch_info": ([I] * 2, I),
     "wn_batch_destroy": ([I], I),
@@ -57,7 +64,7 @@ _SIGNATURES = {
 # Same policy as the reference (python/igl/_lib.py:37-50): at most four
 # workers, and no split below the measured crossover of the query-face sweep.
 MAX_WORKERS = 4
-PARALLEL_MIN_WORK = {"winding": 120_000}
+PARALLEL_MIN_WORK = {"winding": 120_000, "aabb": 4_000_000}
 
 
 def worker_count(items: int, work: int, kernel: str) -> int:

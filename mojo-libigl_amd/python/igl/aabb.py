@@ -1,0 +1,179 @@
+"""Point-mesh squared distance: ``AABB`` and ``point_mesh_squared_distance``.
+
+Mirrors python/igl/__init__.py:322-404 of the reference (same arguments,
+results and errors): ``AABB.init(V, Ele)`` builds the median-split tree
+(here in the library, ``wn_aabb_build``, instead of NumPy);
+``AABB.squared_distance(V, Ele, P)`` returns ``(sqrd (p,), face index (p,)
+int64, closest point (p, 3))``. ``device="gpu"`` runs the same descent on the
+GPU (``wn_aabb_query``) over a device copy of the tree made on first use;
+its results equal the CPU path's bit for bit. The GPU path raises when no
+HIP device is usable; it never falls back to the CPU.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import addr, fan_out, last_error, lib, worker_count
+
+LEAF_SIZE = 8
+
+
+def _build_tree(vertices: np.ndarray, faces: np.ndarray, leaf_size: int = LEAF_SIZE):
+    """(boxes (n, 6), nodes (n, 4), order (m,), stack size) of the reference's
+    tree (python/igl/__init__.py:286-319), built by wn_aabb_build."""
+    h = lib()
+    args = (addr(vertices), addr(faces), len(vertices), len(faces), 0, leaf_size)
+    count = h.wn_aabb_build(*args, 0, 0, 0, 0, 0)
+    if count < 0:
+        raise RuntimeError(f"wn_aabb_build failed: {last_error()}")
+    boxes = np.empty((count, 6), dtype=np.float64)
+    nodes = np.empty((count, 4), dtype=np.int64)
+    order = np.empty(len(faces), dtype=np.int64)
+    stack = np.zeros(1, dtype=np.int64)
+    got = h.wn_aabb_build(*args, addr(boxes), addr(nodes), addr(order), count, addr(stack))
+    if got != count:
+        raise RuntimeError(f"wn_aabb_build failed: {last_error()}")
+    return boxes, nodes, order, int(stack[0])
+
+
+class AABB:
+    """libigl-compatible AABB precomputation for squared-distance queries
+    (python/igl/__init__.py:322-395)."""
+
+    def __init__(self):
+        self._tree = None
+        self._gpu = 0
+
+    def init(self, V, Ele):
+        from . import _mesh
+
+        vertices, faces = _mesh(V, Ele, three_dimensional=True)
+        self._release_gpu()
+        self._vertices = vertices
+        self._faces = faces
+        self._tree = _build_tree(vertices, faces) if len(faces) else ()
+
+    def _checked(self, V, Ele, P):
+        from . import _float_array, _mesh
+
+        if self._tree is None:
+            raise RuntimeError("AABB.init must be called before squared_distance")
+        vertices, faces = _mesh(V, Ele, three_dimensional=True)
+        if not np.array_equal(vertices, self._vertices) or not np.array_equal(faces, self._faces):
+            raise ValueError("V and Ele must match the mesh passed to AABB.init")
+        points = _float_array(P, "P")
+        if points.ndim != 2 or points.shape[1] != 3:
+            raise ValueError("P must have shape (p, 3)")
+        return points
+
+    def squared_distance(self, V, Ele, P, device="cpu"):
+        points = self._checked(V, Ele, P)
+        selected = str(device).lower()
+        if selected not in ("cpu", "gpu"):
+            raise ValueError("device must be 'cpu' or 'gpu'")
+        if selected == "gpu":
+            return self._squared_distance_gpu(points)
+        vertices, faces = self._vertices, self._faces
+        count = len(points)
+        if not len(faces):
+            return (np.full(count, np.inf), np.full(count, -1, dtype=np.int64),
+                    np.zeros((count, 3)))
+        boxes, nodes, order, stack_size = self._tree
+        sqrd = np.empty(count, dtype=np.float64)
+        indices = np.empty(count, dtype=np.int64)
+        closest = np.empty((count, 3), dtype=np.float64)
+        if count:
+            # one traversal stack per query span (python/igl/__init__.py:370-390)
+            depth = 1 + max(stack_size.bit_length(), 1)
+            workers = worker_count(count, count * len(faces) * depth, "aabb")
+            stacks = np.empty((workers, stack_size), dtype=np.int64)
+
+            def query(lo, hi, span=0):
+                lib().mli_aabb_squared_distance(
+                    addr(points), addr(vertices), addr(faces), addr(boxes), addr(nodes),
+                    addr(order), addr(sqrd), addr(indices), addr(closest),
+                    stacks[span].ctypes.data, count, len(nodes), lo, hi)
+
+            fan_out(query, (), count, workers)
+        return sqrd, indices, closest
+
+    def _squared_distance_gpu(self, points: np.ndarray):
+        import torch
+
+        from . import _gpu_memory_available
+
+        if not _gpu_memory_available():
+            raise RuntimeError("device='gpu' requested but no HIP GPU with >= 4000 MiB free is "
+                               "available (this build does not fall back to the CPU)")
+        P = torch.as_tensor(points, device="cuda")
+        return tuple(t.cpu().numpy() for t in self.query_device(P))
+
+    def _ensure_gpu(self) -> int:
+        if self._tree is None:
+            raise RuntimeError("AABB.init must be called before squared_distance")
+        if not self._gpu:
+            handle = lib().wn_aabb_create(addr(self._vertices), addr(self._faces),
+                                          len(self._vertices), len(self._faces), 0, LEAF_SIZE)
+            if handle <= 0:
+                raise RuntimeError(f"wn_aabb_create failed: {last_error()}")
+            self._gpu = handle
+        return self._gpu
+
+    def query_device(self, P, sort=-1):
+        """(sqrd, face index, closest point) as CUDA tensors for a (p, 3) CUDA
+        tensor P, on the current stream, without host copies. The device
+        tree is made on first use and kept until init() or deletion."""
+        import torch
+
+        h = lib()
+        self._ensure_gpu()
+        P = P.to(dtype=torch.float64).contiguous()
+        n = P.shape[0]
+        sqrd = torch.empty(n, dtype=torch.float64, device=P.device)
+        idx = torch.empty(n, dtype=torch.int64, device=P.device)
+        cl = torch.empty((n, 3), dtype=torch.float64, device=P.device)
+        nbytes = h.wn_aabb_workspace_bytes(self._gpu, n, sort)
+        if nbytes < 0:
+            raise RuntimeError(last_error())
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=P.device)
+        st = h.wn_aabb_query(self._gpu, P.data_ptr(), sqrd.data_ptr(), idx.data_ptr(),
+                             cl.data_ptr(), n, sort, ws.data_ptr(), nbytes,
+                             torch.cuda.current_stream().cuda_stream)
+        if st != 0:
+            raise RuntimeError(f"wn_aabb_query failed on the GPU: {last_error()}")
+        return sqrd, idx, cl
+
+    def gpu_info(self) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        if not self._gpu or lib().wn_aabb_info(self._gpu, out.ctypes.data) != 0:
+            raise RuntimeError("no device tree")
+        return dict(zip(("faces", "nodes", "stack_size", "device_bytes"), (int(v) for v in out)))
+
+    def find(self, V, Ele, q, first=False):
+        raise NotImplementedError("AABB.find is outside the covered subset")
+
+    def intersect_ray(self, V, Ele, orig, dir):
+        raise NotImplementedError("AABB ray intersection is outside the covered subset")
+
+    def intersect_ray_first(self, V, Ele, orig, dir, min_t=np.inf):
+        raise NotImplementedError("AABB ray intersection is outside the covered subset")
+
+    def _release_gpu(self) -> None:
+        if getattr(self, "_gpu", 0):
+            lib().wn_aabb_destroy(self._gpu)
+            self._gpu = 0
+
+    def __del__(self):
+        try:
+            self._release_gpu()
+        except Exception:
+            pass
+
+
+def point_mesh_squared_distance(P, V, Ele, device="cpu"):
+    """(sqrd, face index, closest point) of each row of P to the mesh
+    (python/igl/__init__.py:398-401)."""
+    tree = AABB()
+    tree.init(V, Ele)
+    return tree.squared_distance(V, Ele, P, device=device)

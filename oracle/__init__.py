@@ -15,6 +15,10 @@ Contents, each following the reference file:line it restates:
   the FP64 reference gradient for the adjoint kernel.
 * :func:`ref_gpu_atan2` -- the reference GPU kernel's own arctangent,
   ``src/capi.mojo:326-352``.
+* :func:`build_aabb` -- the median-split tree of ``python/igl/__init__.py:286-319``
+  in NumPy; :func:`aabb_distance` -- the descent of ``src/capi.mojo:401-587``
+  over it (``aabb_oracle.c``); :func:`brute_distance` -- the minimum over all
+  faces, with no tree; :func:`closest_on_triangle` -- one point, one triangle.
 """
 
 from __future__ import annotations
@@ -57,6 +61,12 @@ def lib() -> ctypes.CDLL:
         h.wn_oracle_pair_angle.restype = _D
         h.wn_oracle_ref_gpu_atan2.argtypes = [_D, _D]
         h.wn_oracle_ref_gpu_atan2.restype = _D
+        h.wn_oracle_aabb_distance.argtypes = [_P] * 10 + [_I, _I, _I]
+        h.wn_oracle_aabb_distance.restype = None
+        h.wn_oracle_brute_distance.argtypes = [_P, _P, _P, _I, _P, _P, _P, _I]
+        h.wn_oracle_brute_distance.restype = None
+        h.wn_oracle_closest_on_triangle.argtypes = [_P, _P, _P]
+        h.wn_oracle_closest_on_triangle.restype = None
         _handle = h
     return _handle
 
@@ -163,3 +173,92 @@ def winding_torch(V, F, Q):
     det = (a * torch.linalg.cross(b, c, dim=-1)).sum(-1)
     den = la * lb * lc + (a * b).sum(-1) * lc + (b * c).sum(-1) * la + (c * a).sum(-1) * lb
     return torch.atan2(det, den).sum(-1) / (2.0 * torch.pi)
+
+
+# ---------------------------------------------------------------------------
+# Point-mesh squared distance
+# ---------------------------------------------------------------------------
+
+def build_aabb(V, F, leaf_size: int = 8):
+    """(boxes (n, 6) f64, nodes (n, 4) i64, order i64, stack size): the
+    median-split tree of python/igl/__init__.py:286-319. Per node, in
+    preorder: the box of its faces' corner bounds; leaves of <= leaf_size
+    faces; otherwise faces sorted stably by box centre along the axis of
+    largest centre spread (first axis on ties), split at len // 2."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int64)
+    tri = V[F]
+    lo, hi = tri.min(axis=1), tri.max(axis=1)
+    mid = 0.5 * (lo + hi)
+    nodes, boxes, order = [], [], []
+    deepest = 0
+
+    def visit(ids, depth):
+        nonlocal deepest
+        deepest = max(deepest, depth)
+        me = len(nodes)
+        nodes.append(None)
+        boxes.append(np.concatenate((lo[ids].min(axis=0), hi[ids].max(axis=0))))
+        if len(ids) <= leaf_size:
+            nodes[me] = [-1, -1, len(order), len(ids)]
+            order.extend(int(i) for i in ids)
+            return me
+        spread = mid[ids].max(axis=0) - mid[ids].min(axis=0)
+        axis = int(np.argmax(spread))
+        ids = ids[np.argsort(mid[ids, axis], kind="stable")]
+        half = len(ids) // 2
+        a = visit(ids[:half], depth + 1)
+        b = visit(ids[half:], depth + 1)
+        nodes[me] = [a, b, 0, 0]
+        return me
+
+    visit(np.arange(len(F), dtype=np.int64), 1)
+    return (np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6),
+            np.ascontiguousarray(nodes, dtype=np.int64).reshape(-1, 4),
+            np.asarray(order, dtype=np.int64), deepest + 1)
+
+
+def aabb_distance(P, V, F, tree=None):
+    """(sqrd, face index, closest point) by the reference descent
+    (src/capi.mojo:523-587) over build_aabb's tree."""
+    P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int64)
+    n = len(P)
+    sqrd = np.full(n, np.inf)
+    idx = np.full(n, -1, dtype=np.int64)
+    cl = np.zeros((n, 3))
+    if len(F) == 0 or n == 0:
+        return sqrd, idx, cl
+    boxes, nodes, order, stack_size = tree if tree is not None else build_aabb(V, F)
+    stack = np.empty(stack_size, dtype=np.int64)
+    lib().wn_oracle_aabb_distance(P.ctypes.data, V.ctypes.data, F.ctypes.data, boxes.ctypes.data,
+                                  nodes.ctypes.data, order.ctypes.data, sqrd.ctypes.data,
+                                  idx.ctypes.data, cl.ctypes.data, stack.ctypes.data, len(nodes),
+                                  0, n)
+    return sqrd, idx, cl
+
+
+def brute_distance(P, V, F):
+    """(sqrd, face index, closest point): minimum of closest_on_triangle over
+    every face in index order, first face on exact ties."""
+    P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int64)
+    n = len(P)
+    sqrd = np.empty(n)
+    idx = np.empty(n, dtype=np.int64)
+    cl = np.empty((n, 3))
+    if n:
+        lib().wn_oracle_brute_distance(P.ctypes.data, V.ctypes.data, F.ctypes.data, len(F),
+                                       sqrd.ctypes.data, idx.ctypes.data, cl.ctypes.data, n)
+    return sqrd, idx, cl
+
+
+def closest_on_triangle(p, tri):
+    """(d2, closest point) of point p to the triangle tri (3 x 3 corners)."""
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(3)
+    t = np.ascontiguousarray(tri, dtype=np.float64).reshape(9)
+    out = np.empty(4)
+    lib().wn_oracle_closest_on_triangle(p.ctypes.data, t.ctypes.data, out.ctypes.data)
+    return float(out[0]), out[1:].copy()

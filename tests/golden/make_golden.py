@@ -1,4 +1,4 @@
-"""Regenerate tests/golden/winding_golden.npz.
+"""Regenerate tests/golden/winding_golden.npz and aabb_golden.npz.
 
 Inputs are rebuilt from the reference's own test constructors and seeds
 (reference tests/test_parity.py:12-34 grid/tetrahedron, :91-92 and :100
@@ -52,6 +52,16 @@ def main() -> None:
     out.update(c1_pick=pick1, c1_w=mp_float(oracle.winding_mpmath(V1, F1, Q1[pick1], 50)))
 
     np.savez_compressed(os.path.join(HERE, "winding_golden.npz"), **out)
+
+    # point-mesh squared distance: the reference test's grid(8) and 100
+    # uniform points (tests/test_parity.py:12-26, 131-134), results of the oracle's
+    # tree descent (checked against brute force before writing)
+    Vg, Fg = W.grid(8)
+    P = np.random.default_rng(4).uniform([-1.0, -1.0, -1.0], [8.0, 8.0, 2.0], size=(100, 3))
+    sqrd, index, closest = oracle.aabb_distance(P, Vg, Fg)
+    assert np.array_equal(sqrd, oracle.brute_distance(P, Vg, Fg)[0])
+    np.savez_compressed(os.path.join(HERE, "aabb_golden.npz"), V=Vg, F=Fg, P=P, sqrd=sqrd,
+                        index=index, closest=closest)
     print("wrote", os.path.join(HERE, "winding_golden.npz"))
 
 
