@@ -216,7 +216,8 @@ int64_t wn_aabb_create(const double *V, const void *F, int64_t nv, int64_t nf,
 /* out4 = {faces, nodes, stack size, device bytes}. */
 int64_t wn_aabb_info(int64_t handle, int64_t *out4);
 
-/* Workspace bytes for wn_aabb_query (sort -1 = Morton order when np >= 4096). */
+/* Workspace bytes for wn_aabb_query. sort 1 = walk the points in Morton
+ * order (a Z-curve sort first); 0 or -1 (default) = in the given order. */
 int64_t wn_aabb_workspace_bytes(int64_t handle, int64_t np, int64_t sort);
 
 /* GPU counterpart of mli_aabb_squared_distance over a wn_aabb_create tree:

@@ -645,7 +645,7 @@ int64_t wn_aabb_info(int64_t handle, int64_t* out4) {
 int64_t wn_aabb_workspace_bytes(int64_t handle, int64_t np, int64_t sort) {
   if (!lookup_aabb(handle)) return fail(kErrArg, "unknown tree handle");
   if (np < 0 || np > (int64_t)0x7fffffff) return fail(kErrArg, "query count out of range");
-  const int srt = sort < 0 ? (np >= 4096 ? 1 : 0) : (sort ? 1 : 0);
+  const int srt = sort > 0 ? 1 : 0;  // default: launch order
   if (srt && device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
   return (int64_t)wn::aabb_workspace_bytes(np, srt);
 }
@@ -658,7 +658,7 @@ int64_t wn_aabb_query(int64_t handle, const double* p, double* sqrd, int64_t* in
   if (np < 0 || np > (int64_t)0x7fffffff) return fail(kErrArg, "query count out of range");
   if (np == 0) return kOk;
   if (!p || !sqrd || !indices || !closest) return fail(kErrArg, "null device buffer");
-  const int srt = sort < 0 ? (np >= 4096 ? 1 : 0) : (sort ? 1 : 0);
+  const int srt = sort > 0 ? 1 : 0;  // default: launch order
   const size_t need = wn::aabb_workspace_bytes(np, srt);
   if (workspace_bytes < 0 || (size_t)workspace_bytes < need || (need > 0 && !workspace))
     return fail(kErrWorkspace, "workspace smaller than wn_aabb_workspace_bytes()");

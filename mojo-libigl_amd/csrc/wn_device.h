@@ -246,6 +246,15 @@ __device__ __forceinline__ double swizzle_xor16_f64(double v) {
   return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
+// A value every lane holds identically, moved to scalar registers (lane 0's
+// copy of each 32-bit half).
+__device__ __forceinline__ double uniform_f64(double v) {
+  const uint64_t bits = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bits);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(bits >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 __device__ __forceinline__ double wave_sum_f64(double v) {
   v += dpp_move_f64<0xB1>(v);   // quad_perm [1,0,3,2]
   v += dpp_move_f64<0x4E>(v);   // quad_perm [2,3,0,1]
@@ -254,6 +263,36 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
   v += swizzle_xor16_f64(v);
   v += __shfl_xor(v, 32, kWave);
   return v;
+}
+
+// Lane `lane`'s copy of v, in scalar registers.
+__device__ __forceinline__ double lane_f64(double v, int lane) {
+  const uint64_t bits = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), lane);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Wave minimum / maximum with every lane active, returned in scalar
+// registers: DPP within each 16-lane row (quad_perm, row_ror), ds_swizzle
+// across the two rows of each half-wave, then lanes 0 and 32 combined. Needs
+// no per-lane address registers (unlike a shuffle-based reduction).
+__device__ __forceinline__ double wave_min_uniform_f64(double v) {
+  v = fmin(v, dpp_move_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = fmin(v, dpp_move_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = fmin(v, dpp_move_f64<0x124>(v));  // row_ror:4
+  v = fmin(v, dpp_move_f64<0x128>(v));  // row_ror:8
+  v = fmin(v, swizzle_xor16_f64(v));
+  return fmin(lane_f64(v, 0), lane_f64(v, 32));
+}
+
+__device__ __forceinline__ double wave_max_uniform_f64(double v) {
+  v = fmax(v, dpp_move_f64<0xB1>(v));
+  v = fmax(v, dpp_move_f64<0x4E>(v));
+  v = fmax(v, dpp_move_f64<0x124>(v));
+  v = fmax(v, dpp_move_f64<0x128>(v));
+  v = fmax(v, swizzle_xor16_f64(v));
+  return fmax(lane_f64(v, 0), lane_f64(v, 32));
 }
 
 __device__ __forceinline__ double wave_min_f64(double v) {

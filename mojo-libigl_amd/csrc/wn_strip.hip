@@ -112,6 +112,21 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, long long
   s_carry = s_qr;
 }
 
+// First thread of the calling wave, in a scalar register (all lanes active).
+__device__ __forceinline__ int32_t wave_base() {
+  return __builtin_amdgcn_readfirstlane((int32_t)threadIdx.x) & ~(kWave - 1);
+}
+
+// threadIdx.x rebuilt from the wave base and the lane's position in the
+// wave, computed where it is called: an output index formed from it after a
+// sweep needs no vector register kept live across the sweep. (The mbcnt pair
+// sits in a volatile asm so the compiler cannot hoist it above the sweep.)
+__device__ __forceinline__ int32_t thread_index(int32_t base) {
+  int32_t lane;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+  return base + lane;
+}
+
 // Block-local origin: centre of the bbox of the block's valid queries (the
 // same reduction as the face-record kernels). Begins and ends with a barrier.
 __device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double Oy, double Oz,
@@ -121,8 +136,8 @@ __device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double
   double lo0 = valid ? Ox : __builtin_inf(), hi0 = valid ? Ox : -__builtin_inf();
   double lo1 = valid ? Oy : __builtin_inf(), hi1 = valid ? Oy : -__builtin_inf();
   double lo2 = valid ? Oz : __builtin_inf(), hi2 = valid ? Oz : -__builtin_inf();
-  lo0 = wave_min_f64(lo0); lo1 = wave_min_f64(lo1); lo2 = wave_min_f64(lo2);
-  hi0 = wave_max_f64(hi0); hi1 = wave_max_f64(hi1); hi2 = wave_max_f64(hi2);
+  lo0 = wave_min_uniform_f64(lo0); lo1 = wave_min_uniform_f64(lo1); lo2 = wave_min_uniform_f64(lo2);
+  hi0 = wave_max_uniform_f64(hi0); hi1 = wave_max_uniform_f64(hi1); hi2 = wave_max_uniform_f64(hi2);
   const int wave = tid / kWave;
   __syncthreads();
   if ((tid & (kWave - 1)) == 0) {
@@ -135,9 +150,11 @@ __device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double
     lo0 = fmin(lo0, sm.red[0][w]); lo1 = fmin(lo1, sm.red[1][w]); lo2 = fmin(lo2, sm.red[2][w]);
     hi0 = fmax(hi0, sm.red[3][w]); hi1 = fmax(hi1, sm.red[4][w]); hi2 = fmax(hi2, sm.red[5][w]);
   }
-  gx = 0.5 * (lo0 + hi0);
-  gy = 0.5 * (lo1 + hi1);
-  gz = 0.5 * (lo2 + hi2);
+  // every lane combined the same values in the same order: the origin is
+  // block-uniform, so it lives in scalar registers for the rest of the sweep
+  gx = uniform_f64(0.5 * (lo0 + hi0));
+  gy = uniform_f64(0.5 * (lo1 + hi1));
+  gz = uniform_f64(0.5 * (lo2 + hi2));
   __syncthreads();
 }
 
@@ -170,29 +187,31 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
     const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nq,
     int64_t tiles_per_split, int splits, float kappa) {
   __shared__ StripSmem sm;
-  const int tid = threadIdx.x;
-  const int64_t slot = (int64_t)blockIdx.x * kBlockThreads + tid;
-  bool valid[1] = {slot < nq};
-  const int64_t qidx = valid[0] ? (perm ? (int64_t)perm[slot] : slot) : 0;
+  // 32-bit slot (nq < 2^31), and the output index read only after the sweep:
+  // nothing 64-bit stays live across the sweep (it would be spilled)
+  const int32_t base = wave_base();
+  const int32_t slot = (int32_t)blockIdx.x * kBlockThreads + (int32_t)threadIdx.x;
+  const bool valid = slot < nq;
   // q is in launch order (gathered through perm beforehand when sorted)
-  double Ox[1] = {valid[0] ? q[3 * slot] : 0.0};
-  double Oy[1] = {valid[0] ? q[3 * slot + 1] : 0.0};
-  double Oz[1] = {valid[0] ? q[3 * slot + 2] : 0.0};
+  const double Ox = valid ? q[3 * (int64_t)slot] : 0.0;
+  const double Oy = valid ? q[3 * (int64_t)slot + 1] : 0.0;
+  const double Oz = valid ? q[3 * (int64_t)slot + 2] : 0.0;
 
   double gx, gy, gz;
-  strip_block_origin(valid[0], Ox[0], Oy[0], Oz[0], sm, gx, gy, gz);
-  const float ox = valid[0] ? (float)(Ox[0] - gx) : 0.0f;
-  const float oy = valid[0] ? (float)(Oy[0] - gy) : 0.0f;
-  const float oz = valid[0] ? (float)(Oz[0] - gz) : 0.0f;
+  strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz);
+  const float ox = valid ? (float)(Ox - gx) : 0.0f;
+  const float oy = valid ? (float)(Oy - gy) : 0.0f;
+  const float oz = valid ? (float)(Oz - gz) : 0.0f;
 
   const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
   const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
   const double acc = strip_sweep(rec, t_begin, t_end, gx, gy, gz, ox, oy, oz, kappa, sm);
-  if (valid[0]) {
+  const int32_t oslot = (int32_t)blockIdx.x * kBlockThreads + thread_index(base);
+  if (oslot < nq) {
     if (splits == 1)
-      out[qidx] = acc * kInv2Pi;
+      out[perm ? (int64_t)perm[oslot] : (int64_t)oslot] = acc * kInv2Pi;
     else
-      out[(int64_t)blockIdx.y * nq + slot] = acc;
+      out[(int64_t)blockIdx.y * nq + oslot] = acc;
   }
 }
 
@@ -209,11 +228,14 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
   __shared__ StripSmem sm;
   __shared__ int32_t s_item;
   const int tid = threadIdx.x;
+  const int32_t base = wave_base();
   for (;;) {
     __syncthreads();  // every thread has read the previous s_item
     if (tid == 0) s_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int32_t it = s_item;
+    // block-uniform: scalar registers, and the item and mesh records are
+    // scalar loads (nothing per lane to keep live across the sweep)
+    const int32_t it = __builtin_amdgcn_readfirstlane(s_item);
     if (it >= n_items) break;  // block-uniform exit
     const BatchItem item = items[it];
     const BatchMesh m = meshes[item.mesh];
@@ -232,11 +254,13 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
     int64_t t1 = t0 + chunk_tiles;
     if (t1 > m.face_off + m.nf) t1 = m.face_off + m.nf;
     const double acc = strip_sweep(rec, t0, t1, gx, gy, gz, ox, oy, oz, kappa, sm);
-    if (valid) {
+    const int32_t local = item.qtile * kBlockThreads + thread_index(base);  // slot - query_off
+    if (local < m.nq) {
+      const int64_t oslot = m.query_off + local;
       if (m.chunks == 1)
-        dst[perm ? (int64_t)perm[slot] : slot] = acc * kInv2Pi;
+        dst[perm ? (int64_t)perm[oslot] : oslot] = acc * kInv2Pi;
       else  // chunk-major, slot order: a block's writes are contiguous
-        partial[m.pbase + (int64_t)item.chunk * m.nq + (slot - m.query_off)] = acc;
+        partial[m.pbase + (int64_t)item.chunk * m.nq + local] = acc;
     }
   }
 }

@@ -94,10 +94,10 @@ def test_device_tensors_and_reuse(require_gpu):
     tree = igl.AABB()
     tree.init(V, F)
     rng = np.random.default_rng(3)
-    for n in (4096, 1000):  # sorted, then unsorted by default; tree reused
+    for n, sort in ((4096, 1), (1000, -1)):  # Morton order, then the default; tree reused
         P = rng.uniform([-2, -2, -2], [32, 32, 2], size=(n, 3))
         Pd = torch.tensor(P, device="cuda")
-        s, i, c = tree.query_device(Pd)
+        s, i, c = tree.query_device(Pd, sort=sort)
         torch.cuda.synchronize()
         assert s.is_cuda and i.dtype == torch.int64 and c.shape == (n, 3)
         cpu = tree.squared_distance(V, F, P)
