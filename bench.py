@@ -26,6 +26,8 @@ Other workloads (one JSON line each, same timing method):
                      1,024 queries, end to end from host arrays
   --workload aabb    point-mesh squared distance: 243,602-face grid x 1M
                      points (and the README's 10k-point row end to end)
+  --workload fast    hierarchical winding number on the 1.31M-face sphere x
+                     1M queries, per accuracy parameter, vs all-pairs
   --workload copy    256 MiB device copy (calibrates the byte counters)
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
@@ -564,6 +566,57 @@ def run_aabb(args, h):
         tree=info)
 
 
+def run_fast(args, h):
+    """Hierarchical winding number on the stress mesh (1.31M faces x 1M
+    queries, half within 1e-7..1e-2 of the surface): time per beta, error
+    against the exact FP64 winding number on a 65,536-query subsample, and
+    the all-pairs FP32-compensated strip kernel on the same queries."""
+    import numpy as np
+    import torch
+
+    import igl
+    import wn_workloads as W
+    from igl.mesh import Mesh
+
+    V, F = W.config3_mesh()
+    Q, dist = W.config3_queries(V, F, args.queries)
+    t0 = time.perf_counter()
+    fw = igl.FastWindingNumber(V, F)
+    Qd = torch.tensor(Q, device="cuda")
+    fw.query(Qd[:64], device="gpu")  # device copy of the tree
+    build_ms = (time.perf_counter() - t0) * 1e3
+    sub = np.random.default_rng(9).choice(len(Q), 65536, replace=False)
+    exact = igl.winding_number(V, F, Q[sub], device="gpu")
+    rows = {}
+    for beta in (2.0, 4.0, 8.0):
+        ms, _ = timed(lambda: fw.query(Qd, beta=beta, device="gpu"), args.steps, args.warmup)
+        w = fw.query(Qd, beta=beta, device="gpu").cpu().numpy()
+        err = np.abs(w[sub] - exact)
+        near = ~np.isnan(dist[sub])
+        rows[f"beta_{beta:g}"] = {
+            "ms_per_step": ms, "queries_per_s": len(Q) / (ms * 1e-3),
+            "max_abs_dw": float(err.max()), "mean_abs_dw": float(err.mean()),
+            "max_abs_dw_near_surface": float(err[near].max()),
+            "max_abs_dw_uniform": float(err[~near].max())}
+        log(f"fast winding beta={beta}: {ms:.2f} ms, max err {err.max():.2e}")
+    m = Mesh(V, F)
+    out = torch.empty(len(Q), dtype=torch.float64, device="cuda")
+    ms_strip, _ = timed(lambda: m.winding_number(Qd, out=out), max(1, args.steps // 2), 1)
+    m.close()
+    info = fw.info()
+    fw.close()
+    best = rows["beta_2"]
+    return base_line(
+        args, "M winding-number queries/s, hierarchical (beta 2), 1 MI355X",
+        best["queries_per_s"] / 1e6, "M queries/s", best["ms_per_step"], "f64",
+        f"displaced icosphere level 8 ({len(F)} faces, {info['nodes']} tree nodes) x {len(Q)} "
+        "queries (half uniform, half at |d| in [1e-7, 1e-2]); second-order far-field "
+        "expansions, exact FP64 leaves",
+        by_beta=rows, tree_build_ms=build_ms,
+        all_pairs_fp32c_strip_ms=ms_strip,
+        speedup_beta2_vs_all_pairs_strip=ms_strip / best["ms_per_step"])
+
+
 def run_copy(args, h):
     """Device-to-device copy of a known size (256 MiB read + 256 MiB written
     per launch): the reference point for scaling the FETCH_SIZE / WRITE_SIZE
@@ -585,7 +638,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="torus",
-                    choices=("torus", "batch", "stress", "grad", "readme", "aabb", "copy"))
+                    choices=("torus", "batch", "stress", "grad", "readme", "aabb", "fast", "copy"))
     ap.add_argument("--queries", type=int, default=1_048_576)
     ap.add_argument("--qpl", type=int, default=0, help="queries per lane (0 = library default)")
     ap.add_argument("--splits", type=int, default=0, help="face splits (0 = library default)")
@@ -607,7 +660,8 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
     runner = {"torus": run_torus, "batch": run_batch, "stress": run_stress, "grad": run_grad,
-              "aabb": run_aabb, "readme": run_readme, "copy": run_copy}[args.workload]
+              "aabb": run_aabb, "fast": run_fast, "readme": run_readme,
+              "copy": run_copy}[args.workload]
     print(json.dumps(runner(args, h)), flush=True)
     return 0
 

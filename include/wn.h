@@ -232,6 +232,43 @@ int64_t wn_aabb_query(int64_t handle, const double *p, double *sqrd, int64_t *in
 /* Frees the device tree (synchronises the device first). */
 int64_t wn_aabb_destroy(int64_t handle);
 
+/* ---- Hierarchical ("fast") winding number ------------------------------- */
+
+/* No reference counterpart (README.md:29 lists it as not covered). Barill et
+ * al. 2018: per node of the median-split tree (the wn_aabb_build tree), the
+ * second-order expansion of its faces' solid angle about their area-weighted
+ * centroid; a query uses a node's expansion when farther than beta x the
+ * node's radius, else recurses, and sums leaf faces exactly
+ * (src/capi.mojo:218-235). V (nv x 3 f64), F (nf x 3, fdtype 0 = int64,
+ * 1 = int32) on the host; leaf_size 0 = 8; upload != 0 also copies the tree
+ * to the current device. Returns a handle (> 0) or a negative status. */
+int64_t wn_fwn_create(const double *V, const void *F, int64_t nv, int64_t nf,
+                      int64_t fdtype, int64_t leaf_size, int64_t upload);
+
+/* out4 = {faces, nodes, stack size, device bytes (0 without a device copy)}. */
+int64_t wn_fwn_info(int64_t handle, int64_t *out4);
+
+/* Per node: expansions (nodes x 34 doubles: centre xyz, radius, N (3),
+ * C (3 x 3), D (3 x 6 symmetric blocks 00 01 02 11 12 22)) and links
+ * (nodes x 4: left, right, start, count). Returns the node count; fills the
+ * outputs when both are given and capacity >= it. (host) */
+int64_t wn_fwn_export(int64_t handle, double *expansions, int64_t *links,
+                      int64_t capacity);
+
+/* CPU query: w[q0:q1] for q (nq x 3), accuracy parameter beta > 0 (2 is the
+ * published default; larger is more accurate and slower). (host) */
+int64_t wn_fwn_query_cpu(int64_t handle, const double *q, double *w, int64_t nq,
+                         double beta, int64_t q0, int64_t q1);
+
+/* GPU query over the device copy: (device) q (nq x 3), w (nq). Same walk and
+ * expansion arithmetic as the CPU query (exact faces agree to rounding).
+ * Asynchronous on `stream`; allocates nothing. */
+int64_t wn_fwn_query(int64_t handle, const double *q, double *w, int64_t nq,
+                     double beta, void *stream);
+
+/* Frees the tree (synchronises the device first when it has a device copy). */
+int64_t wn_fwn_destroy(int64_t handle);
+
 /* ---- Adjoint ------------------------------------------------------------ */
 
 /* No reference counterpart (the reference has no derivative of

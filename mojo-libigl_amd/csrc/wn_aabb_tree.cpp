@@ -13,6 +13,9 @@
 #include "wn_aabb.h"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
+#include <utility>
 
 #include "wn_closest.h"
 
@@ -20,54 +23,77 @@ namespace wn {
 
 namespace {
 
+// Preorder numbering is fixed by the subtree sizes alone (aabb_node_count),
+// and leaves take the face ids in the order the recursion leaves them, so
+// every subtree knows its node index and its slice of `order` up front and
+// large subtrees are built on their own threads with identical results.
 struct Builder {
   const std::vector<double>& lo;   // per face: corner minimum (3)
   const std::vector<double>& hi;   // per face: corner maximum (3)
   const std::vector<double>& mid;  // per face: 0.5 (lo + hi) (3)
   int64_t leaf;
   AabbTree& t;
-  int64_t deepest = 0;
+  int64_t* ids;                    // face ids, reordered in place; becomes `order`
+  std::atomic<int64_t> deepest{0};
 
-  int64_t node(int64_t* ids, int64_t n, int64_t depth) {
-    deepest = std::max(deepest, depth);
-    const int64_t me = (int64_t)t.nodes.size() / 4;
-    t.nodes.insert(t.nodes.end(), {-1, -1, 0, 0});
-    double box[6] = {lo[3 * ids[0]], lo[3 * ids[0] + 1], lo[3 * ids[0] + 2],
-                     hi[3 * ids[0]], hi[3 * ids[0] + 1], hi[3 * ids[0] + 2]};
+  void node(int64_t me, int64_t b, int64_t n, int64_t depth, int spawn_levels) {
+    int64_t seen = deepest.load(std::memory_order_relaxed);
+    while (depth > seen && !deepest.compare_exchange_weak(seen, depth)) {
+    }
+    const int64_t* id = ids + b;
+    double box[6] = {lo[3 * id[0]], lo[3 * id[0] + 1], lo[3 * id[0] + 2],
+                     hi[3 * id[0]], hi[3 * id[0] + 1], hi[3 * id[0] + 2]};
     for (int64_t k = 1; k < n; ++k)
       for (int d = 0; d < 3; ++d) {
-        box[d] = std::min(box[d], lo[3 * ids[k] + d]);
-        box[3 + d] = std::max(box[3 + d], hi[3 * ids[k] + d]);
+        box[d] = std::min(box[d], lo[3 * id[k] + d]);
+        box[3 + d] = std::max(box[3 + d], hi[3 * id[k] + d]);
       }
-    t.boxes.insert(t.boxes.end(), box, box + 6);
+    std::copy(box, box + 6, t.boxes.begin() + 6 * me);
     if (n <= leaf) {
-      t.nodes[4 * me + 2] = (int64_t)t.order.size();
+      t.nodes[4 * me] = -1;
+      t.nodes[4 * me + 1] = -1;
+      t.nodes[4 * me + 2] = b;
       t.nodes[4 * me + 3] = n;
-      t.order.insert(t.order.end(), ids, ids + n);
-      return me;
+      return;
     }
     // split axis: largest spread of the face-box centres, first on ties
     int axis = 0;
     double best = -1.0;
     for (int d = 0; d < 3; ++d) {
-      double a = mid[3 * ids[0] + d], b = a;
+      double a = mid[3 * id[0] + d], c = a;
       for (int64_t k = 1; k < n; ++k) {
-        a = std::min(a, mid[3 * ids[k] + d]);
-        b = std::max(b, mid[3 * ids[k] + d]);
+        a = std::min(a, mid[3 * id[k] + d]);
+        c = std::max(c, mid[3 * id[k] + d]);
       }
-      if (d == 0 || b - a > best) {
-        best = b - a;
+      if (d == 0 || c - a > best) {
+        best = c - a;
         axis = d;
       }
     }
-    const double* key = mid.data() + axis;
-    std::stable_sort(ids, ids + n, [key](int64_t x, int64_t y) { return key[3 * x] < key[3 * y]; });
+    // stable sort by centre along the axis: sort (key, position) pairs, which
+    // orders exactly as a stable sort of the ids does
+    std::vector<std::pair<double, int64_t>> kp((size_t)n);
+    for (int64_t k = 0; k < n; ++k) kp[(size_t)k] = {mid[3 * id[k] + axis], k};
+    std::sort(kp.begin(), kp.end());
+    std::vector<int64_t> tmp(id, id + n);
+    for (int64_t k = 0; k < n; ++k) ids[b + k] = tmp[(size_t)kp[(size_t)k].second];
     const int64_t half = n / 2;
-    const int64_t l = node(ids, half, depth + 1);
-    const int64_t r = node(ids + half, n - half, depth + 1);
+    const int64_t l = me + 1;
+    const int64_t r = me + 1 + aabb_node_count(half, leaf);
     t.nodes[4 * me] = l;
     t.nodes[4 * me + 1] = r;
-    return me;
+    t.nodes[4 * me + 2] = 0;
+    t.nodes[4 * me + 3] = 0;
+    if (spawn_levels > 0 && n > 65536) {
+      std::thread right([&, r, b, half, n, depth, spawn_levels] {
+        node(r, b + half, n - half, depth + 1, spawn_levels - 1);
+      });
+      node(l, b, half, depth + 1, spawn_levels - 1);
+      right.join();
+    } else {
+      node(l, b, half, depth + 1, 0);
+      node(r, b + half, n - half, depth + 1, 0);
+    }
   }
 };
 
@@ -90,9 +116,33 @@ void face_bounds(const double* V, const FT* F, int64_t nf, std::vector<double>& 
 }  // namespace
 
 int64_t aabb_node_count(int64_t nf, int64_t leaf) {
+  // leaves(n) = 1 for n <= leaf, else leaves(n / 2) + leaves(n - n / 2); at
+  // each depth the sizes take at most two adjacent values, so the recursion
+  // is walked level by level over (size, multiplicity) pairs
   if (nf <= 0) return 0;
-  if (nf <= leaf) return 1;
-  return 1 + aabb_node_count(nf / 2, leaf) + aabb_node_count(nf - nf / 2, leaf);
+  std::vector<std::pair<int64_t, int64_t>> level{{nf, 1}};
+  int64_t leaves = 0;
+  while (!level.empty()) {
+    std::vector<std::pair<int64_t, int64_t>> next;
+    auto add = [&next](int64_t size, int64_t times) {
+      for (auto& p : next)
+        if (p.first == size) {
+          p.second += times;
+          return;
+        }
+      next.push_back({size, times});
+    };
+    for (const auto& p : level) {
+      if (p.first <= leaf) {
+        leaves += p.second;
+      } else {
+        add(p.first / 2, p.second);
+        add(p.first - p.first / 2, p.second);
+      }
+    }
+    level.swap(next);
+  }
+  return 2 * leaves - 1;
 }
 
 void aabb_build(const double* V, const void* F, int fdtype, int64_t nf, int64_t leaf,
@@ -105,14 +155,13 @@ void aabb_build(const double* V, const void* F, int fdtype, int64_t nf, int64_t 
   else
     face_bounds(V, (const int32_t*)F, nf, lo, hi, mid);
   const int64_t count = aabb_node_count(nf, leaf);
-  out.boxes.reserve(6 * count);
-  out.nodes.reserve(4 * count);
-  out.order.reserve(nf);
-  std::vector<int64_t> ids(nf);
-  for (int64_t f = 0; f < nf; ++f) ids[f] = f;
-  Builder b{lo, hi, mid, leaf, out};
-  b.node(ids.data(), nf, 1);
-  out.stack_size = b.deepest + 1;
+  out.boxes.assign(6 * count, 0.0);
+  out.nodes.assign(4 * count, 0);
+  out.order.resize(nf);
+  for (int64_t f = 0; f < nf; ++f) out.order[f] = f;
+  Builder b{lo, hi, mid, leaf, out, out.order.data()};
+  b.node(0, 0, nf, 1, 3);  // up to 8 threads on the top three levels
+  out.stack_size = b.deepest.load() + 1;
 }
 
 }  // namespace wn

@@ -267,6 +267,31 @@ AabbHandle* lookup_aabb(int64_t handle) {
   return it == aabb_registry().end() ? nullptr : *it;
 }
 
+// A hierarchical winding-number tree (wn_fwn_*): host copy always, device
+// copy when created with upload != 0.
+struct FwnHandle {
+  wn::FwnTree tree;
+  int64_t nf = 0;
+  size_t bytes = 0;
+  void* dev = nullptr;
+};
+
+std::mutex& fwn_registry_mu() {
+  static std::mutex mu;
+  return mu;
+}
+
+std::set<FwnHandle*>& fwn_registry() {
+  static std::set<FwnHandle*> live;
+  return live;
+}
+
+FwnHandle* lookup_fwn(int64_t handle) {
+  std::lock_guard<std::mutex> lock(fwn_registry_mu());
+  auto it = fwn_registry().find((FwnHandle*)(intptr_t)handle);
+  return it == fwn_registry().end() ? nullptr : *it;
+}
+
 // Shared argument checks of the tree entry points; nullptr if fine.
 const char* aabb_args(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype,
                       int64_t leaf) {
@@ -677,6 +702,106 @@ int64_t wn_aabb_destroy(int64_t handle) {
     if (it == aabb_registry().end()) return fail(kErrArg, "unknown tree handle");
     h = *it;
     aabb_registry().erase(it);
+  }
+  if (h->dev) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(h->dev);
+  }
+  delete h;
+  return kOk;
+}
+
+int64_t wn_fwn_create(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype,
+                      int64_t leaf_size, int64_t upload) {
+  const int64_t leaf = leaf_size <= 0 ? 8 : leaf_size;
+  if (const char* why = aabb_args(V, F, nv, nf, fdtype, leaf)) return fail(kErrArg, why);
+  if (upload && device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  FwnHandle* h = new FwnHandle();
+  h->nf = nf;
+  wn::fwn_build(V, F, (int)fdtype, nf, leaf, h->tree);
+  if (h->tree.stack_size > wn::kAabbStack) {
+    delete h;
+    return fail(kErrArg, "tree deeper than the descent stack");
+  }
+  if (upload && !h->tree.nodes.empty()) {
+    h->bytes = wn::fwn_device_bytes((int64_t)h->tree.nodes.size(), nf);
+    hipStream_t s = nullptr;
+    hipError_t e = hipMalloc(&h->dev, h->bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = wn::fwn_upload(h->tree, h->dev, s);
+    if (s) (void)hipStreamDestroy(s);
+    if (e != hipSuccess) {
+      if (h->dev) (void)hipFree(h->dev);
+      delete h;
+      return fail(kErrHip, hip_msg("wn_fwn_create", e));
+    }
+  }
+  std::lock_guard<std::mutex> lock(fwn_registry_mu());
+  fwn_registry().insert(h);
+  return (int64_t)(intptr_t)h;
+}
+
+int64_t wn_fwn_info(int64_t handle, int64_t* out4) {
+  FwnHandle* h = lookup_fwn(handle);
+  if (!h) return fail(kErrArg, "unknown tree handle");
+  if (!out4) return fail(kErrArg, "null output");
+  out4[0] = h->nf;
+  out4[1] = (int64_t)h->tree.nodes.size();
+  out4[2] = h->tree.stack_size;
+  out4[3] = (int64_t)h->bytes;
+  return kOk;
+}
+
+int64_t wn_fwn_export(int64_t handle, double* expansions, int64_t* links, int64_t capacity) {
+  FwnHandle* h = lookup_fwn(handle);
+  if (!h) return fail(kErrArg, "unknown tree handle");
+  const int64_t n = (int64_t)h->tree.nodes.size();
+  if (!expansions || !links || capacity < n) return n;
+  for (int64_t i = 0; i < n; ++i) {
+    const wn::FwnNode& e = h->tree.nodes[(size_t)i];
+    std::memcpy(expansions + 34 * i, &e.cx, 34 * sizeof(double));
+    links[4 * i] = e.left;
+    links[4 * i + 1] = e.right;
+    links[4 * i + 2] = e.start;
+    links[4 * i + 3] = e.count;
+  }
+  return n;
+}
+
+int64_t wn_fwn_query_cpu(int64_t handle, const double* q, double* w, int64_t nq, double beta,
+                         int64_t q0, int64_t q1) {
+  FwnHandle* h = lookup_fwn(handle);
+  if (!h) return fail(kErrArg, "unknown tree handle");
+  if (!(beta > 0.0)) return fail(kErrArg, "beta must be positive");
+  if (nq < 0 || q0 < 0 || q1 < q0 || q1 > nq) return fail(kErrArg, "bad query span");
+  if (q1 > q0 && (!q || !w)) return fail(kErrArg, "null host buffer");
+  wn::fwn_query_cpu(h->tree, q, w, beta, q0, q1);
+  return kOk;
+}
+
+int64_t wn_fwn_query(int64_t handle, const double* q, double* w, int64_t nq, double beta,
+                     void* stream) {
+  FwnHandle* h = lookup_fwn(handle);
+  if (!h) return fail(kErrArg, "unknown tree handle");
+  if (!(beta > 0.0)) return fail(kErrArg, "beta must be positive");
+  if (nq < 0) return fail(kErrArg, "negative size");
+  if (nq == 0) return kOk;
+  if (!q || !w) return fail(kErrArg, "null device buffer");
+  if (!h->tree.nodes.empty() && !h->dev) return fail(kErrArg, "tree was created without a device copy");
+  hipError_t e = wn::fwn_query_launch(h->dev, (int64_t)h->tree.nodes.size(), q, w, nq, beta,
+                                      (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_fwn_query", e));
+  return kOk;
+}
+
+int64_t wn_fwn_destroy(int64_t handle) {
+  FwnHandle* h = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(fwn_registry_mu());
+    auto it = fwn_registry().find((FwnHandle*)(intptr_t)handle);
+    if (it == fwn_registry().end()) return fail(kErrArg, "unknown tree handle");
+    h = *it;
+    fwn_registry().erase(it);
   }
   if (h->dev) {
     (void)hipDeviceSynchronize();

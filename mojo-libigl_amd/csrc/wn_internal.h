@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "wn_aabb.h"
+#include "wn_fwn.h"
 
 namespace wn {
 
@@ -157,5 +158,17 @@ size_t aabb_workspace_bytes(int64_t np, int sort);
 hipError_t aabb_query_launch(const void* dev, int64_t nnodes, int64_t nf, const double* q,
                              double* sqrd, int64_t* index, double* closest, int64_t np, int sort,
                              void* ws, size_t ws_bytes, hipStream_t stream);
+
+}  // namespace wn
+
+namespace wn {
+
+// Hierarchical winding number on the GPU (wn_fwn.hip): the device copy of a
+// tree is one allocation of fwn_device_bytes() filled by fwn_upload().
+size_t fwn_device_bytes(int64_t nnodes, int64_t nf);
+hipError_t fwn_upload(const FwnTree& t, void* dev, hipStream_t stream);
+// q (nq x 3) and w (nq): device buffers.
+hipError_t fwn_query_launch(const void* dev, int64_t nnodes, const double* q, double* w,
+                            int64_t nq, double beta, hipStream_t stream);
 
 }  // namespace wn

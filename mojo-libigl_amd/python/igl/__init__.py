@@ -198,6 +198,11 @@ def winding_number_for_point(V, F, p):
 
 
 from .aabb import AABB, point_mesh_squared_distance  # noqa: E402
+from .fast_winding import (  # noqa: E402
+    FastWindingNumber,
+    fast_winding_number,
+    fast_winding_number_for_meshes,
+)
 
 
 def __getattr__(name):

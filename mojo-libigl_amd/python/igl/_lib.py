@@ -54,6 +54,12 @@ _SIGNATURES = {
     "wn_aabb_workspace_bytes": ([I] * 3, I),
     "wn_aabb_query": ([I] * 10, I),
     "wn_aabb_destroy": ([I], I),
+    "wn_fwn_create": ([I] * 7, I),
+    "wn_fwn_info": ([I] * 2, I),
+    "wn_fwn_export": ([I] * 4, I),
+    "wn_fwn_query_cpu": ([I] * 4 + [D] + [I] * 2, I),
+    "wn_fwn_query": ([I] * 4 + [D, I], I),
+    "wn_fwn_destroy": ([I], I),
     "wn_batch_run": ([I] * 6, I),
     "wn_batch_info": ([I] * 2, I),
     "wn_batch_destroy": ([I], I),

@@ -19,6 +19,10 @@ Contents, each following the reference file:line it restates:
   in NumPy; :func:`aabb_distance` -- the descent of ``src/capi.mojo:401-587``
   over it (``aabb_oracle.c``); :func:`brute_distance` -- the minimum over all
   faces, with no tree; :func:`closest_on_triangle` -- one point, one triangle.
+* :func:`fwn_expansion`, :func:`fwn_far_omega`, :func:`exact_omega`,
+  :func:`fast_winding` -- the hierarchical winding number (Barill et al.
+  2018, second order; not in the reference, README.md:29) restated in NumPy
+  over the same tree.
 """
 
 from __future__ import annotations
@@ -262,3 +266,90 @@ def closest_on_triangle(p, tri):
     out = np.empty(4)
     lib().wn_oracle_closest_on_triangle(p.ctypes.data, t.ctypes.data, out.ctypes.data)
     return float(out[0]), out[1:].copy()
+
+
+# ---------------------------------------------------------------------------
+# Hierarchical winding number (Barill et al. 2018, second order)
+# ---------------------------------------------------------------------------
+
+def fwn_expansion(tri_corners):
+    """Second-order far-field expansion of a set of triangles ((k, 3, 3)
+    corners): (centre, radius, N (3,), C (3, 3), D (3, 3, 3)) with centre =
+    area-weighted centroid, N = sum A n, C_ij = sum A n_i ybar_j,
+    D_ijk = sum A n_i (1/12)(sum_v y_vj y_vk + s_j s_k)."""
+    T = np.asarray(tri_corners, dtype=np.float64).reshape(-1, 3, 3)
+    an = 0.5 * np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+    area = np.linalg.norm(an, axis=1)
+    g = T.mean(axis=1)
+    c = (area[:, None] * g).sum(0) / area.sum() if area.sum() > 0 else g.mean(0)
+    y = T - c
+    s = y.sum(axis=1)
+    N = an.sum(0)
+    C = np.einsum("ti,tj->ij", an, s / 3.0)
+    M = (np.einsum("tvj,tvk->tjk", y, y) + np.einsum("tj,tk->tjk", s, s)) / 12.0
+    D = np.einsum("ti,tjk->ijk", an, M)
+    radius = np.sqrt((y ** 2).sum(-1).max())
+    return c, radius, N, C, D
+
+
+def fwn_far_omega(expansion, q):
+    """Solid angle of the expansion's triangles seen from q (second order):
+    N.f + sum J_ij C_ij + 1/2 sum H_ijk D_ijk at r = centre - q, for
+    f = r/|r|^3 and its first and second derivatives."""
+    c, _, N, C, D = expansion
+    r = np.asarray(c, dtype=np.float64) - np.asarray(q, dtype=np.float64)
+    n = np.sqrt(r @ r)
+    I = np.eye(3)
+    f = r / n ** 3
+    J = I / n ** 3 - 3.0 * np.outer(r, r) / n ** 5
+    H = (-3.0 * (np.einsum("ij,k->ijk", I, r) + np.einsum("ik,j->ijk", I, r)
+                 + np.einsum("jk,i->ijk", I, r)) / n ** 5
+         + 15.0 * np.einsum("i,j,k->ijk", r, r, r) / n ** 7)
+    return float(N @ f + (J * C).sum() + 0.5 * (H * D).sum())
+
+
+def exact_omega(tri_corners, q):
+    """Exact solid angle sum (2 atan2 per face, src/capi.mojo:218-235)."""
+    T = np.asarray(tri_corners, dtype=np.float64).reshape(-1, 3, 3) - np.asarray(q, dtype=np.float64)
+    a, b, c = T[:, 0], T[:, 1], T[:, 2]
+    la, lb, lc = (np.linalg.norm(x, axis=1) for x in (a, b, c))
+    det = np.einsum("ij,ij->i", a, np.cross(b, c))
+    den = la * lb * lc + (a * b).sum(1) * lc + (b * c).sum(1) * la + (c * a).sum(1) * lb
+    return float((2.0 * np.arctan2(det, den)).sum())
+
+
+def fast_winding(P, V, F, beta=2.0, leaf_size=8):
+    """Reference descent of the hierarchical winding number over build_aabb's
+    tree, in Python (small inputs): far nodes (|q - c| > beta R) by
+    expansion, near leaves exactly."""
+    V = np.asarray(V, dtype=np.float64)
+    F = np.asarray(F, dtype=np.int64)
+    P = np.asarray(P, dtype=np.float64).reshape(-1, 3)
+    if len(F) == 0:
+        return np.zeros(len(P))
+    boxes, nodes, order, _ = build_aabb(V, F, leaf_size)
+    corners = V[F[order]]
+
+    def span(n):
+        l, r, s, k = nodes[n]
+        if k > 0:
+            return s, s + k
+        return span(l)[0], span(r)[1]
+
+    exps = [fwn_expansion(corners[slice(*span(n))]) for n in range(len(nodes))]
+    out = np.empty(len(P))
+    for i, q in enumerate(P):
+        total, stack = 0.0, [0]
+        while stack:
+            n = stack.pop()
+            e = exps[n]
+            if np.sum((q - e[0]) ** 2) > beta * beta * e[1] * e[1]:
+                total += fwn_far_omega(e, q)
+            elif nodes[n][3] > 0:
+                s, k = nodes[n][2], nodes[n][3]
+                total += exact_omega(corners[s:s + k], q)
+            else:
+                stack.append(int(nodes[n][1]))
+                stack.append(int(nodes[n][0]))
+        out[i] = total / (4.0 * np.pi)
+    return out
