@@ -131,6 +131,18 @@ __device__ __forceinline__ float atan_small_f32(float r) {
   return r * p;
 }
 
+// atan(x / 4) for |x| <= 1: atan_small_f32 with its coefficients scaled by
+// powers of two, so atan_small_x4_f32(4 r) equals atan_small_f32(r) bit for
+// bit (every product and sum is the same value times an exact power of two).
+__device__ __forceinline__ float atan_small_x4_f32(float x) {
+  const float t = x * x;
+  float p = -0.12990966888801511f * (1.0f / 16384.0f);
+  p = __builtin_fmaf(p, t, 0.19950626010644533f * (1.0f / 1024.0f));
+  p = __builtin_fmaf(p, t, -0.33332723769728184f * (1.0f / 64.0f));
+  p = __builtin_fmaf(p, t, 0.99999998819005977f * 0.25f);
+  return x * p;
+}
+
 // Geometry of one pair in FP32 (see pair_rec32): doubled det and den and the
 // doubled scale 2 la lb lc.
 struct PairGeom32 {
@@ -218,6 +230,15 @@ __device__ __forceinline__ bool wave_all(bool pred) {
 __device__ __forceinline__ bool wave_all_far(float det2, float den2, float s2) {
   const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det2) < kFarTau * den2);
   const uint64_t m2 = __builtin_amdgcn_ballot_w64(den2 > kFarKappa * s2);
+  return (m1 & m2) == __builtin_amdgcn_read_exec();
+}
+
+// wave_all_far for det8 = 4 x det2: |det8| < den2 is |det2| < kFarTau den2
+// (kFarTau = 1/4) without the multiply.
+__device__ __forceinline__ bool wave_all_far_x4(float det8, float den2, float s2) {
+  static_assert(kFarTau == 0.25f && kFarKappa == 1.0f, "x4 form assumes tau 1/4, kappa 1");
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det8) < den2);
+  const uint64_t m2 = __builtin_amdgcn_ballot_w64(den2 > s2);
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }
 

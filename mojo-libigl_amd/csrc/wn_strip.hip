@@ -12,7 +12,7 @@
 // LDS tile (255 records, a multiple of 3 so the three register slots rotate
 // statically through a 3-way unrolled loop):
 //   R0[j] = (w.x, w.y, w.z, |w_j - w_{j-1}|^2)      (block-local FP32 coords)
-//   R1[j] = (|w_j - w_{j-2}|^2, N2.x, N2.y, N2.z)     (N2 = 2 x face normal)
+//   R1[j] = (|w_j - w_{j-2}|^2, N8.x, N8.y, N8.z)     (N8 = 8 x face normal)
 //   M[j]  = 0 (no face ends at j), +1 / -1 (face orientation vs strip order)
 // Squared lengths and N2 are computed in FP64 from the rounded FP32 corners.
 
@@ -52,8 +52,10 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   const double d1x = (double)cx - px, d1y = (double)cy - py, d1z = (double)cz - pz;
   const double d2x = (double)cx - qx, d2y = (double)cy - qy, d2z = (double)cz - qz;
   const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
-  // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag
-  const double f2 = 2.0 * c.w;
+  // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag; stored
+  // as 8 N so that the lane's dot product is 4 x the doubled det, and the
+  // far test |det2| < den2 / 4 needs no multiply (powers of two: exact)
+  const double f2 = 8.0 * c.w;
   sm.r0[j] = make_float4(cx, cy, cz, (float)(d1x * d1x + d1y * d1y + d1z * d1z));
   sm.r1[j] = make_float4((float)(d2x * d2x + d2y * d2y + d2z * d2z),
                          (float)(f2 * (ey * d2z - ez * d2y)), (float)(f2 * (ez * d2x - ex * d2z)),
@@ -90,19 +92,20 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, long long
   const float s_pr = (P.l2 + R.l2) - r1.x;  // 2 c.a
   if (flag_bits != 0) {
     // face (a, b, c) = (P, Q, R): s_ab = s_carry, s_bc = s_qr, s_ca = s_pr
-    const float det2 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
+    // det8 = 4 x the doubled det (exactly: the normal is stored scaled by 8)
+    const float det8 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
     const float u2 = 2.0f * (P.l * Q.l);
     const float s2 = u2 * R.l;
     const float den2 =
         __builtin_fmaf(R.l, u2 + s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
     float ang;
     bool risky = false;
-    if (wave_all_far(det2, den2, s2)) {
-      ang = atan_small_f32(det2 * __builtin_amdgcn_rcpf(den2));
+    if (wave_all_far_x4(det8, den2, s2)) {
+      ang = atan_small_x4_f32(det8 * __builtin_amdgcn_rcpf(den2));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       float mx;
-      ang = atan2_f32(det2, den2, mx);
+      ang = atan2_f32(0.25f * det8, den2, mx);
       risky = mx < kappa * s2;
     }
     double a64 = (double)ang;
