@@ -94,19 +94,20 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, long long
     // face (a, b, c) = (P, Q, R): s_ab = s_carry, s_bc = s_qr, s_ca = s_pr
     // det8 = 4 x the doubled det (exactly: the normal is stored scaled by 8)
     const float det8 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
-    const float u2 = 2.0f * (P.l * Q.l);
-    const float s2 = u2 * R.l;
-    const float den2 =
-        __builtin_fmaf(R.l, u2 + s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
+    // den2 = 2 la lb lc + dp with dp = lc s_ab + la s_bc + lb s_ca; the far
+    // test den2 > 2 la lb lc (kFarKappa = 1) is dp > 0
+    const float pq = P.l * Q.l;
+    const float dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
+    const float den2 = __builtin_fmaf(pq + pq, R.l, dp);
     float ang;
     bool risky = false;
-    if (wave_all_far_x4(det8, den2, s2)) {
+    if (wave_all_far_x4(det8, den2, dp)) {
       ang = atan_small_x4_f32(det8 * __builtin_amdgcn_rcpf(den2));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       float mx;
       ang = atan2_f32(0.25f * det8, den2, mx);
-      risky = mx < kappa * s2;
+      risky = mx < kappa * ((pq + pq) * R.l);  // kappa x 2 la lb lc
     }
     double a64 = (double)ang;
     if (risky) a64 = strip_pair_f64(sm, j, ox, oy, oz);
