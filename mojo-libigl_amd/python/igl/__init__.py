@@ -197,7 +197,7 @@ def winding_number_for_point(V, F, p):
     return float(winding_number(V, F, p))
 
 
-from .aabb import AABB, point_mesh_squared_distance  # noqa: E402
+from .aabb import AABB, point_mesh_squared_distance, signed_distance  # noqa: E402
 from .fast_winding import (  # noqa: E402
     FastWindingNumber,
     fast_winding_number,

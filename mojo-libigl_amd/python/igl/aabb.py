@@ -177,3 +177,34 @@ def point_mesh_squared_distance(P, V, Ele, device="cpu"):
     tree = AABB()
     tree.init(V, Ele)
     return tree.squared_distance(V, Ele, P, device=device)
+
+
+def signed_distance(P, V, F, device="cpu", winding="exact", beta=2.0):
+    """(S, face index, closest point): S = sign * sqrt(squared distance) with
+    sign -1 where the winding number is >= 1/2 (inside) and +1 elsewhere.
+
+    No counterpart in the reference (it has the distance and the winding
+    number separately, python/igl/__init__.py:232-283, 322-401). The
+    distance is point_mesh_squared_distance's; the winding number is the
+    exact one (``winding="exact"``; FP64) or the hierarchical one
+    (``winding="fast"``, accuracy ``beta``), on the same device."""
+    from . import _float_array, winding_number
+    from .fast_winding import FastWindingNumber
+
+    points = _float_array(P, "P")
+    if points.ndim != 2 or points.shape[1] != 3:
+        raise ValueError("P must have shape (p, 3)")
+    mode = str(winding).lower()
+    if mode not in ("exact", "fast"):
+        raise ValueError("winding must be 'exact' or 'fast'")
+    sqrd, index, closest = point_mesh_squared_distance(points, V, F, device=device)
+    if mode == "exact":
+        w = winding_number(V, F, points, device=device) if len(points) else np.zeros(0)
+    else:
+        fw = FastWindingNumber(V, F)
+        try:
+            w = fw.query(points, beta=beta, device=device)
+        finally:
+            fw.close()
+    sign = np.where(np.asarray(w) >= 0.5, -1.0, 1.0)
+    return sign * np.sqrt(sqrd), index, closest

@@ -258,3 +258,24 @@ def test_golden_fixture():
     V, F = g["V"], g["F"]
     ind = np.array([_independent_d2(p, *V[F[f]]) for p, f in zip(g["P"], g["index"])])
     np.testing.assert_allclose(g["sqrd"], ind, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("winding", ["exact", "fast"])
+def test_signed_distance_closed_mesh(winding):
+    """Sign from the winding number (inside negative), magnitude from the
+    distance query; against the oracle's brute-force distance and exact
+    winding number on a closed icosphere."""
+    V, F = W.icosphere(3)
+    rng = np.random.default_rng(21)
+    d = rng.normal(size=(600, 3))
+    P = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0.2, 1.8, size=(600, 1))
+    S, I, C = igl.signed_distance(P, V, F, winding=winding, beta=8.0)
+    sq, _, _ = oracle.brute_distance(P, V, F)
+    w = oracle.winding(V, F, P)
+    np.testing.assert_array_equal(np.abs(S), np.sqrt(sq))
+    np.testing.assert_array_equal(S < 0, w >= 0.5)
+    # radius 1 icosphere: points well inside or outside are signed by radius
+    r = np.linalg.norm(P, axis=1)
+    assert (S[r < 0.8] < 0).all() and (S[r > 1.2] > 0).all()
+    with pytest.raises(ValueError):
+        igl.signed_distance(P, V, F, winding="other")

@@ -117,3 +117,15 @@ def test_reference_benchmark_mesh(require_gpu, npts):
     sub = np.random.default_rng(2).choice(npts, 64, replace=False)
     np.testing.assert_array_equal(gpu[0][sub], oracle.brute_distance(P[sub], V, F)[0])
     print(f"grid(350) x {npts}: max sqrd {gpu[0].max():.3f}", flush=True)
+
+
+@pytest.mark.parametrize("winding", ["exact", "fast"])
+def test_signed_distance_gpu_equals_cpu(require_gpu, winding):
+    V, F = W.icosphere(4)
+    rng = np.random.default_rng(22)
+    d = rng.normal(size=(20000, 3))
+    P = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(0.2, 1.8, size=(20000, 1))
+    cpu = igl.signed_distance(P, V, F, winding=winding, beta=8.0)
+    gpu = igl.signed_distance(P, V, F, device="gpu", winding=winding, beta=8.0)
+    for a, b in zip(cpu, gpu):
+        np.testing.assert_array_equal(a, b)
