@@ -599,6 +599,8 @@ def run_fast(args, h):
             "max_abs_dw_near_surface": float(err[near].max()),
             "max_abs_dw_uniform": float(err[~near].max())}
         log(f"fast winding beta={beta}: {ms:.2f} ms, max err {err.max():.2e}")
+    ms_given, _ = timed(lambda: fw.query(Qd, beta=2.0, device="gpu", sort=0), args.steps,
+                        args.warmup)
     m = Mesh(V, F)
     out = torch.empty(len(Q), dtype=torch.float64, device="cuda")
     ms_strip, _ = timed(lambda: m.winding_number(Qd, out=out), max(1, args.steps // 2), 1)
@@ -611,8 +613,9 @@ def run_fast(args, h):
         best["queries_per_s"] / 1e6, "M queries/s", best["ms_per_step"], "f64",
         f"displaced icosphere level 8 ({len(F)} faces, {info['nodes']} tree nodes) x {len(Q)} "
         "queries (half uniform, half at |d| in [1e-7, 1e-2]); second-order far-field "
-        "expansions, exact FP64 leaves",
+        "expansions, exact FP64 leaves, queries walked in Morton order",
         by_beta=rows, tree_build_ms=build_ms,
+        beta_2_queries_in_given_order_ms=ms_given,
         all_pairs_fp32c_strip_ms=ms_strip,
         speedup_beta2_vs_all_pairs_strip=ms_strip / best["ms_per_step"])
 

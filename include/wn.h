@@ -260,11 +260,17 @@ int64_t wn_fwn_export(int64_t handle, double *expansions, int64_t *links,
 int64_t wn_fwn_query_cpu(int64_t handle, const double *q, double *w, int64_t nq,
                          double beta, int64_t q0, int64_t q1);
 
-/* GPU query over the device copy: (device) q (nq x 3), w (nq). Same walk and
- * expansion arithmetic as the CPU query (exact faces agree to rounding).
- * Asynchronous on `stream`; allocates nothing. */
+/* Workspace bytes for wn_fwn_query (sort -1 = Morton order when nq >= 4096,
+ * 1 = always, 0 = never). */
+int64_t wn_fwn_workspace_bytes(int64_t handle, int64_t nq, int64_t sort);
+
+/* GPU query over the device copy: (device) q (nq x 3), w (nq), workspace.
+ * Same walk and expansion arithmetic as the CPU query (exact faces agree to
+ * rounding); with sort, queries are walked in Morton order and w is written
+ * back in the given order. Asynchronous on `stream`; allocates nothing. */
 int64_t wn_fwn_query(int64_t handle, const double *q, double *w, int64_t nq,
-                     double beta, void *stream);
+                     double beta, int64_t sort, void *workspace,
+                     int64_t workspace_bytes, void *stream);
 
 /* Frees the tree (synchronises the device first when it has a device copy). */
 int64_t wn_fwn_destroy(int64_t handle);

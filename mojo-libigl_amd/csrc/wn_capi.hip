@@ -779,17 +779,29 @@ int64_t wn_fwn_query_cpu(int64_t handle, const double* q, double* w, int64_t nq,
   return kOk;
 }
 
+int64_t wn_fwn_workspace_bytes(int64_t handle, int64_t nq, int64_t sort) {
+  if (!lookup_fwn(handle)) return fail(kErrArg, "unknown tree handle");
+  if (nq < 0 || nq > (int64_t)0x7fffffff) return fail(kErrArg, "query count out of range");
+  const int srt = sort < 0 ? (nq >= 4096 ? 1 : 0) : (sort ? 1 : 0);
+  if (srt && device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  return (int64_t)wn::fwn_workspace_bytes(nq, srt);
+}
+
 int64_t wn_fwn_query(int64_t handle, const double* q, double* w, int64_t nq, double beta,
-                     void* stream) {
+                     int64_t sort, void* workspace, int64_t workspace_bytes, void* stream) {
   FwnHandle* h = lookup_fwn(handle);
   if (!h) return fail(kErrArg, "unknown tree handle");
   if (!(beta > 0.0)) return fail(kErrArg, "beta must be positive");
-  if (nq < 0) return fail(kErrArg, "negative size");
+  if (nq < 0 || nq > (int64_t)0x7fffffff) return fail(kErrArg, "query count out of range");
   if (nq == 0) return kOk;
   if (!q || !w) return fail(kErrArg, "null device buffer");
   if (!h->tree.nodes.empty() && !h->dev) return fail(kErrArg, "tree was created without a device copy");
-  hipError_t e = wn::fwn_query_launch(h->dev, (int64_t)h->tree.nodes.size(), q, w, nq, beta,
-                                      (hipStream_t)stream);
+  const int srt = sort < 0 ? (nq >= 4096 ? 1 : 0) : (sort ? 1 : 0);
+  const size_t need = wn::fwn_workspace_bytes(nq, srt);
+  if (workspace_bytes < 0 || (size_t)workspace_bytes < need || (need > 0 && !workspace))
+    return fail(kErrWorkspace, "workspace smaller than wn_fwn_workspace_bytes()");
+  hipError_t e = wn::fwn_query_launch(h->dev, (int64_t)h->tree.nodes.size(), q, w, nq, beta, srt,
+                                      workspace, (size_t)workspace_bytes, (hipStream_t)stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_fwn_query", e));
   return kOk;
 }

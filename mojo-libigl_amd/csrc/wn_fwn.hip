@@ -9,7 +9,8 @@
 // Layout in HBM (one allocation per tree): FwnNode[nnodes] (304 B: centre,
 // radius, 30 expansion coefficients, children and leaf range), then the face
 // corners in leaf order (72 B per face). One-wave blocks (walks diverge per
-// lane); descent stack in LDS, [depth][lane].
+// lane); descent stack in LDS, [depth][lane]. Queries are walked in Morton
+// order by default (1.4-1.5x faster on a 1.31M-face mesh x 1M queries).
 
 #include "wn_device.h"
 #include "wn_fwn.h"
@@ -24,7 +25,8 @@ constexpr int kFwnLanes = 64;
 
 __global__ __launch_bounds__(kFwnLanes) void fwn_query_kernel(
     const FwnNode* __restrict__ nodes, int64_t nnodes, const double* __restrict__ corners,
-    const double* __restrict__ q, double* __restrict__ w, int64_t nq, double beta2) {
+    const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ w,
+    int64_t nq, double beta2) {
   __shared__ int32_t stack[kAabbStack][kFwnLanes];
   const int lane = threadIdx.x;
   const int64_t i = (int64_t)blockIdx.x * kFwnLanes + lane;
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(kFwnLanes) void fwn_query_kernel(
       stack[top++][lane] = (int32_t)e->left;
     }
   }
-  w[i] = omega * (0.25 / 3.14159265358979323846264338327950288);
+  w[perm ? (int64_t)perm[i] : i] = omega * (0.25 / 3.14159265358979323846264338327950288);
 }
 
 }  // namespace
@@ -70,13 +72,34 @@ hipError_t fwn_upload(const FwnTree& t, void* dev, hipStream_t stream) {
   return e;
 }
 
+size_t fwn_workspace_bytes(int64_t nq, int sort) {
+  if (!sort) return 0;
+  return align_up((size_t)nq * sizeof(int32_t)) + align_up((size_t)nq * 3 * sizeof(double)) +
+         morton_workspace_bytes(nq);
+}
+
 hipError_t fwn_query_launch(const void* dev, int64_t nnodes, const double* q, double* w,
-                            int64_t nq, double beta, hipStream_t stream) {
+                            int64_t nq, double beta, int sort, void* ws, size_t ws_bytes,
+                            hipStream_t stream) {
   if (nq == 0) return hipSuccess;
+  if (ws_bytes < fwn_workspace_bytes(nq, sort)) return hipErrorInvalidValue;
   const FwnNode* nodes = (const FwnNode*)dev;
   const double* corners = (const double*)((const char*)dev + align_up(sizeof(FwnNode) * (size_t)nnodes));
+  int32_t* perm = nullptr;
+  if (sort) {
+    // queries in Morton order: neighbouring lanes walk the same nodes
+    char* c = (char*)ws;
+    perm = (int32_t*)c;
+    c += align_up((size_t)nq * sizeof(int32_t));
+    double* qs = (double*)c;
+    c += align_up((size_t)nq * 3 * sizeof(double));
+    hipError_t e = morton_order(q, nq, perm, c, morton_workspace_bytes(nq), stream);
+    if (e == hipSuccess) e = gather_queries_launch(q, perm, nq, qs, stream);
+    if (e != hipSuccess) return e;
+    q = qs;
+  }
   fwn_query_kernel<<<dim3((unsigned)((nq + kFwnLanes - 1) / kFwnLanes)), kFwnLanes, 0, stream>>>(
-      nodes, nnodes, corners, q, w, nq, beta * beta);
+      nodes, nnodes, corners, q, perm, w, nq, beta * beta);
   return hipGetLastError();
 }
 

@@ -167,8 +167,10 @@ namespace wn {
 // tree is one allocation of fwn_device_bytes() filled by fwn_upload().
 size_t fwn_device_bytes(int64_t nnodes, int64_t nf);
 hipError_t fwn_upload(const FwnTree& t, void* dev, hipStream_t stream);
-// q (nq x 3) and w (nq): device buffers.
+// q (nq x 3) and w (nq): device buffers; sort: Morton order (workspace).
+size_t fwn_workspace_bytes(int64_t nq, int sort);
 hipError_t fwn_query_launch(const void* dev, int64_t nnodes, const double* q, double* w,
-                            int64_t nq, double beta, hipStream_t stream);
+                            int64_t nq, double beta, int sort, void* ws, size_t ws_bytes,
+                            hipStream_t stream);
 
 }  // namespace wn

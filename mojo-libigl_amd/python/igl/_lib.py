@@ -58,7 +58,8 @@ _SIGNATURES = {
     "wn_fwn_info": ([I] * 2, I),
     "wn_fwn_export": ([I] * 4, I),
     "wn_fwn_query_cpu": ([I] * 4 + [D] + [I] * 2, I),
-    "wn_fwn_query": ([I] * 4 + [D, I], I),
+    "wn_fwn_workspace_bytes": ([I] * 3, I),
+    "wn_fwn_query": ([I] * 4 + [D] + [I] * 4, I),
     "wn_fwn_destroy": ([I], I),
     "wn_batch_run": ([I] * 6, I),
     "wn_batch_info": ([I] * 2, I),

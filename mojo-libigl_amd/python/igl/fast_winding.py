@@ -61,9 +61,11 @@ class FastWindingNumber:
             raise RuntimeError(last_error())
         return e, links
 
-    def query(self, Q, beta: float = BETA, device: str = "cpu"):
+    def query(self, Q, beta: float = BETA, device: str = "cpu", sort: int = -1):
         """w at the rows of Q ((q, 3) array-like -> (q,) float64), or at a
-        (q, 3) CUDA tensor (-> CUDA tensor, current stream) with device="gpu"."""
+        (q, 3) CUDA tensor (-> CUDA tensor, current stream) with device="gpu".
+        sort (GPU): -1 = walk queries in Morton order when q >= 4096, 1 =
+        always, 0 = in the given order; the result order never changes."""
         from . import _float_array, _gpu_memory_available
 
         beta = float(beta)
@@ -89,7 +91,12 @@ class FastWindingNumber:
             if not self._gpu:
                 self._gpu = self._create(1)
             w = torch.empty(P.shape[0], dtype=torch.float64, device=P.device)
+            nbytes = lib().wn_fwn_workspace_bytes(self._gpu, P.shape[0], int(sort))
+            if nbytes < 0:
+                raise RuntimeError(last_error())
+            ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=P.device)
             st = lib().wn_fwn_query(self._gpu, P.data_ptr(), w.data_ptr(), P.shape[0], beta,
+                                    int(sort), ws.data_ptr(), nbytes,
                                     torch.cuda.current_stream().cuda_stream)
             if st != 0:
                 raise RuntimeError(f"wn_fwn_query failed on the GPU: {last_error()}")

@@ -33,8 +33,10 @@ def test_gpu_equals_cpu(require_gpu, mesh, beta):
     Q = rng.uniform(lo - 0.3 * (hi - lo), hi + 0.3 * (hi - lo), size=(5000, 3))
     fw = igl.FastWindingNumber(V, F)
     cpu = fw.query(Q, beta=beta)
-    gpu = fw.query(Q, beta=beta, device="gpu")
+    gpu = fw.query(Q, beta=beta, device="gpu")  # 5000 queries: Morton order
     assert np.abs(gpu - cpu).max() <= 1e-12
+    unsorted = fw.query(Q, beta=beta, device="gpu", sort=0)
+    np.testing.assert_array_equal(unsorted, gpu)  # same walk per query
     if beta > 1e8:
         assert np.abs(gpu - oracle.winding(V, F, Q)).max() <= 1e-12
     assert fw.info()["device_bytes"] > 0
