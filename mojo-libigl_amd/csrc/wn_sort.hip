@@ -70,6 +70,15 @@ __global__ __launch_bounds__(kBlockThreads) void bbox_kernel(const double* __res
   }
 }
 
+// Blocks of the bbox pass: each ends in 6 global atomics on the same 6
+// words, which serialise, so the grid stays small (<= 256 blocks, >= 16
+// points per thread); the reads are a few microseconds of HBM either way.
+inline unsigned bbox_blocks(int64_t nq) {
+  int64_t blocks = (nq + 16 * kBlockThreads - 1) / (16 * kBlockThreads);
+  if (blocks > 256) blocks = 256;
+  return (unsigned)(blocks < 1 ? 1 : blocks);
+}
+
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
   v &= 0x3ffu;
   v = (v | (v << 16)) & 0x030000FFu;
@@ -150,9 +159,7 @@ hipError_t morton_order(const double* q, int64_t nq, int32_t* perm, void* ws,
   size_t temp = sort_temp_bytes(nq);
 
   bbox_init_kernel<<<1, kBlockThreads, 0, stream>>>(box);
-  int64_t blocks = (nq + kBlockThreads - 1) / kBlockThreads;
-  if (blocks > 1024) blocks = 1024;
-  bbox_kernel<<<(unsigned)blocks, kBlockThreads, 0, stream>>>(q, nq, box);
+  bbox_kernel<<<bbox_blocks(nq), kBlockThreads, 0, stream>>>(q, nq, box);
   morton_kernel<<<(unsigned)((nq + kBlockThreads - 1) / kBlockThreads), kBlockThreads, 0,
                   stream>>>(q, nq, box, keys_in, vals_in);
   hipError_t e = hipGetLastError();
@@ -234,9 +241,7 @@ hipError_t morton_order_segmented(const double* q, int64_t nq, const int64_t* se
   size_t temp = seg_sort_temp_bytes(nq);
 
   bbox_init_kernel<<<1, kBlockThreads, 0, stream>>>(box);
-  int64_t blocks = (nq + kBlockThreads - 1) / kBlockThreads;
-  if (blocks > 1024) blocks = 1024;
-  bbox_kernel<<<(unsigned)blocks, kBlockThreads, 0, stream>>>(q, nq, box);
+  bbox_kernel<<<bbox_blocks(nq), kBlockThreads, 0, stream>>>(q, nq, box);
   morton_seg_kernel<<<(unsigned)((nq + kBlockThreads - 1) / kBlockThreads), kBlockThreads, 0,
                       stream>>>(q, nq, box, seg_off, nseg, keys_in, vals_in);
   hipError_t e = hipGetLastError();
