@@ -82,6 +82,11 @@ def cpu_baseline(tri, Q, seconds: float):
     t0 = time.perf_counter()
     oracle.winding_threads(tri, sample, cores)
     dt = time.perf_counter() - t0
+    # the reference's own thread policy: at most 4 workers (python/igl/_lib.py:37)
+    n4 = max(4, nq // max(1, cores // 4) // 4)
+    t0 = time.perf_counter()
+    oracle.winding_threads(tri, Q[:n4], 4)
+    dt4 = time.perf_counter() - t0
     return {
         "value": nf * nq / dt / 1e9,
         "unit": "G query-face pairs/s",
@@ -89,6 +94,8 @@ def cpu_baseline(tri, Q, seconds: float):
         "kind": "port",
         "sample": f"first {nq} of the {len(Q)} queries x all {nf} faces, FP64, "
                   f"{cores} threads, one timed run of {dt:.1f} s after a warm-up",
+        "four_workers": {"value": nf * n4 / dt4 / 1e9,
+                         "sample": f"first {n4} queries, 4 threads, {dt4:.1f} s"},
     }
 
 

@@ -227,3 +227,42 @@ def test_config2_full_size_properties(require_gpu):
     sub = np.arange(0, len(Q), 4099)
     ref = oracle.winding_soa(tri, Q[sub])
     assert np.abs(w64[sub] - ref).max() <= FP64_TOL
+
+
+def test_host_entry_is_safe_under_concurrent_callers(require_gpu):
+    """mli_winding_gpu from 8 Python threads at once (the reference's callers
+    fan out over threads, python/igl/_lib.py:72-95): every call succeeds and
+    returns the single-threaded result, bit for bit."""
+    import threading
+
+    from igl import pack_faces
+    from igl._lib import addr, lib
+
+    V, F, Q = W.config1()
+    tri = pack_faces(V.astype(np.float64), F)
+    Qs = [np.ascontiguousarray(Q[i::8]) for i in range(8)]
+    ref = []
+    for q in Qs:
+        out = np.empty(len(q))
+        assert lib().mli_winding_gpu(addr(tri), addr(q), addr(out), tri.shape[1], len(q)) == 1
+        ref.append(out)
+    got = [None] * 8
+    errors = []
+
+    def run(i):
+        try:
+            for _ in range(5):
+                out = np.empty(len(Qs[i]))
+                ok = lib().mli_winding_gpu(addr(tri), addr(Qs[i]), addr(out), tri.shape[1], len(Qs[i]))
+                if ok != 1 or not np.array_equal(out, ref[i]):
+                    errors.append(i)
+            got[i] = True
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors and all(got)
