@@ -458,6 +458,17 @@ def run_grad(args, h):
         ((w - td) ** 2).sum().backward()
 
     ms, _ = timed(step, args.steps, args.warmup)
+    field = A.WindingField(Fd, len(V))
+
+    def step_field():
+        Vd.grad = None
+        Od.grad = None
+        w = field(Vd, Od)
+        ((w - td) ** 2).sum().backward()
+
+    ms_field, _ = timed(step_field, args.steps, args.warmup)
+    field_fwd_ms, _ = timed(lambda: field(Vd.detach(), Od.detach()), args.steps, 1)
+    field.close()
     pairs = len(F) * len(Q)
     tri = A.pack_faces(Vd.detach(), Fd)
     Qd = Od.detach().double()
@@ -466,10 +477,14 @@ def run_grad(args, h):
     bwd_ms, _ = timed(lambda: A.backward(tri, Qd, g), args.steps, 1)
     return base_line(
         args, "G query-face pairs/s, forward + adjoint (dL/dO, dL/dV), 1 MI355X",
-        pairs / (ms * 1e-3) / 1e9, "G query-face pairs/s", ms, "f32+f64",
+        pairs / (ms_field * 1e-3) / 1e9, "G query-face pairs/s", ms_field, "f32+f64",
         f"frequency-50 geodesic sphere ({len(F)} faces, float32) x {len(Q)} queries, "
-        "L = sum (w - t)^2, FP32-compensated forward + FP64 adjoint",
-        forward_ms=fwd_ms, backward_ms=bwd_ms,
+        "L = sum (w - t)^2; igl.autograd.WindingField: strip records gathered from V on "
+        "the device each step, FP32-compensated strip forward + FP64 adjoint",
+        forward_ms=field_fwd_ms, backward_ms=bwd_ms,
+        autograd_winding_number={"ms_per_step": ms, "forward_ms": fwd_ms,
+                                 "note": "igl.autograd.winding_number: device face pack, "
+                                         "FP32-compensated face-record forward + FP64 adjoint"},
         backward_pairs_per_s=pairs / (bwd_ms * 1e-3) / 1e9,
         gradient_max={"dL/dV": float(Vd.grad.abs().max()), "dL/dO": float(Od.grad.abs().max())})
 

@@ -125,11 +125,24 @@ int64_t wn_pack_soa(const void *V, const void *F, int64_t nv, int64_t nf,
 int64_t wn_mesh_records(const double *V, const void *F, int64_t nv, int64_t nf,
                         int64_t fdtype, double *out, int64_t capacity);
 
-/* Builds the strip records of (host) V, F and uploads them (the only
- * allocation of the mesh path). Returns a handle (> 0) or a negative status
- * (-1 also for a face index outside [0, nv)). */
+/* Builds the strip records of (host) V, F and uploads them with the strip
+ * topology (vertex id and flag per record; the only allocation of the mesh
+ * path). nv < 2^31. Returns a handle (> 0) or a negative status (-1 also for
+ * a face index outside [0, nv)). */
 int64_t wn_mesh_create(const double *V, const void *F, int64_t nv, int64_t nf,
                        int64_t fdtype);
+
+/* No reference counterpart. A mesh from its faces alone (host F, nv
+ * vertices): the strip topology is built and uploaded, the records are set
+ * by wn_mesh_set_vertices. Returns a handle (> 0) or a negative status. */
+int64_t wn_mesh_create_topology(const void *F, int64_t nv, int64_t nf, int64_t fdtype);
+
+/* No reference counterpart. Re-gathers a mesh's records from (device) V
+ * (nv x 3, vdtype 0 = float64, 1 = float32; nv as at creation) on `stream`:
+ * new vertex positions without re-stripifying or host copies. Asynchronous;
+ * allocates nothing. */
+int64_t wn_mesh_set_vertices(int64_t mesh, const void *V, int64_t nv, int64_t vdtype,
+                             void *stream);
 
 /* out3 = {faces, records, tiles}. */
 int64_t wn_mesh_info(int64_t mesh, int64_t *out3);

@@ -222,9 +222,52 @@ wn::BatchPlan* lookup_batch(int64_t handle) {
 // A device-resident mesh in strip-record form (wn_mesh_*).
 struct MeshHandle {
   int64_t nf = 0;
+  int64_t nv = 0;
   int64_t nrec = 0;
-  void* d_rec = nullptr;
+  void* d_rec = nullptr;     // one allocation: records, then vertex ids, then flags
+  int32_t* d_idx = nullptr;
+  int8_t* d_flag = nullptr;
 };
+
+// Device copy of a mesh's strip topology (and its records when V is given on
+// the host): one allocation, records first.
+int64_t mesh_upload(MeshHandle* m, const double* V, const void* F, int fdtype) {
+  std::vector<int32_t> idx;
+  std::vector<int8_t> flag;
+  m->nrec = wn::strip_topology(F, m->nf, fdtype, idx, flag);
+  if (m->nrec == 0) return kOk;
+  const size_t rec_b = wn::align_up((size_t)m->nrec * 4 * sizeof(double));
+  const size_t idx_b = wn::align_up((size_t)m->nrec * sizeof(int32_t));
+  hipError_t e = hipMalloc(&m->d_rec, rec_b + idx_b + wn::align_up((size_t)m->nrec));
+  if (e != hipSuccess) {
+    m->d_rec = nullptr;
+    return fail(kErrHip, hip_msg("wn_mesh_create", e));
+  }
+  m->d_idx = (int32_t*)((char*)m->d_rec + rec_b);
+  m->d_flag = (int8_t*)((char*)m->d_rec + rec_b + idx_b);
+  e = hipMemcpy(m->d_idx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(m->d_flag, flag.data(), flag.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && V) {
+    std::vector<double> rec((size_t)(4 * m->nrec));
+    for (int64_t k = 0; k < m->nrec; ++k) {
+      const int64_t v = idx[(size_t)k];
+      rec[(size_t)(4 * k)] = V[3 * v];
+      rec[(size_t)(4 * k + 1)] = V[3 * v + 1];
+      rec[(size_t)(4 * k + 2)] = V[3 * v + 2];
+      rec[(size_t)(4 * k + 3)] = (double)flag[(size_t)k];
+    }
+    e = hipMemcpy(m->d_rec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice);
+  } else if (e == hipSuccess) {
+    e = hipMemset(m->d_rec, 0, (size_t)m->nrec * 4 * sizeof(double));
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(m->d_rec);
+    m->d_rec = nullptr;
+    return fail(kErrHip, hip_msg("wn_mesh_create", e));
+  }
+  return kOk;
+}
 
 std::mutex& mesh_registry_mu() {
   static std::mutex mu;
@@ -485,6 +528,7 @@ int64_t wn_batch_destroy(int64_t handle) {
 int64_t wn_mesh_records(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype,
                         double* out, int64_t capacity) {
   if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (nv > (int64_t)0x7fffffff) return fail(kErrArg, "more than 2^31-1 vertices");
   if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
   if (nf > 0 && (!V || !F)) return fail(kErrArg, "null host buffer");
   for (int64_t i = 0; i < 3 * nf; ++i) {
@@ -499,6 +543,7 @@ int64_t wn_mesh_records(const double* V, const void* F, int64_t nv, int64_t nf, 
 
 int64_t wn_mesh_create(const double* V, const void* F, int64_t nv, int64_t nf, int64_t fdtype) {
   if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (nv > (int64_t)0x7fffffff) return fail(kErrArg, "more than 2^31-1 vertices");
   if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
   if (nf > 0 && (!V || !F)) return fail(kErrArg, "null host buffer");
   for (int64_t i = 0; i < 3 * nf; ++i) {
@@ -506,23 +551,54 @@ int64_t wn_mesh_create(const double* V, const void* F, int64_t nv, int64_t nf, i
     if (v < 0 || v >= nv) return fail(kErrArg, "face index is outside V");
   }
   if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
-  std::vector<double> rec;
   MeshHandle* m = new MeshHandle();
   m->nf = nf;
-  m->nrec = wn::strip_records(V, F, nf, (int)fdtype, rec);
-  if (m->nrec > 0) {
-    hipError_t e = hipMalloc(&m->d_rec, rec.size() * sizeof(double));
-    if (e == hipSuccess)
-      e = hipMemcpy(m->d_rec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      if (m->d_rec) (void)hipFree(m->d_rec);
-      delete m;
-      return fail(kErrHip, hip_msg("wn_mesh_create", e));
-    }
+  m->nv = nv;
+  const int64_t st = mesh_upload(m, V, F, (int)fdtype);
+  if (st != kOk) {
+    delete m;
+    return st;
   }
   std::lock_guard<std::mutex> lock(mesh_registry_mu());
   mesh_registry().insert(m);
   return (int64_t)(intptr_t)m;
+}
+
+int64_t wn_mesh_create_topology(const void* F, int64_t nv, int64_t nf, int64_t fdtype) {
+  if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (nv > (int64_t)0x7fffffff) return fail(kErrArg, "more than 2^31-1 vertices");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if (nf > 0 && !F) return fail(kErrArg, "null host buffer");
+  for (int64_t i = 0; i < 3 * nf; ++i) {
+    const int64_t v = fdtype == 0 ? ((const int64_t*)F)[i] : (int64_t)((const int32_t*)F)[i];
+    if (v < 0 || v >= nv) return fail(kErrArg, "face index is outside V");
+  }
+  if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  MeshHandle* m = new MeshHandle();
+  m->nf = nf;
+  m->nv = nv;
+  const int64_t st = mesh_upload(m, nullptr, F, (int)fdtype);
+  if (st != kOk) {
+    delete m;
+    return st;
+  }
+  std::lock_guard<std::mutex> lock(mesh_registry_mu());
+  mesh_registry().insert(m);
+  return (int64_t)(intptr_t)m;
+}
+
+int64_t wn_mesh_set_vertices(int64_t mesh, const void* V, int64_t nv, int64_t vdtype,
+                             void* stream) {
+  MeshHandle* m = lookup_mesh(mesh);
+  if (!m) return fail(kErrArg, "unknown mesh handle");
+  if (nv != m->nv) return fail(kErrArg, "vertex count differs from the mesh's");
+  if (vdtype != 0 && vdtype != 1) return fail(kErrArg, "vdtype must be 0 (float64) or 1 (float32)");
+  if (m->nrec == 0) return kOk;
+  if (!V) return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::strip_gather_launch(V, (int)vdtype, nv, m->d_idx, m->d_flag, m->nrec,
+                                         (double4*)m->d_rec, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_set_vertices", e));
+  return kOk;
 }
 
 int64_t wn_mesh_info(int64_t mesh, int64_t* out3) {

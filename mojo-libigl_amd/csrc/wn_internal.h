@@ -38,6 +38,14 @@ hipError_t reduce_splits_launch(const double* partial, const int32_t* perm, doub
 // them). Records: 4 doubles each (x, y, z, flag), a multiple of 255 per mesh.
 int64_t strip_records(const double* V, const void* F, int64_t nf, int fdtype,
                       std::vector<double>& out);
+// The same records as vertex ids and flags (host), and the device gather
+// that turns them into records for device vertices V (nv x 3; vdtype 0 =
+// float64, 1 = float32): rec[k] = (V[idx[k]], flag[k]).
+int64_t strip_topology(const void* F, int64_t nf, int fdtype, std::vector<int32_t>& idx,
+                       std::vector<int8_t>& flag);
+hipError_t strip_gather_launch(const void* V, int vdtype, int64_t nv, const int32_t* idx,
+                               const int8_t* flag, int64_t nrec, double4* rec,
+                               hipStream_t stream);
 int strip_resident_blocks();
 size_t strip_workspace_bytes(int64_t nq, int splits, int sort);
 hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, double* dst,

@@ -327,6 +327,36 @@ hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, doubl
   return reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
 }
 
+namespace {
+
+template <typename VT>
+__global__ __launch_bounds__(kBlockThreads) void strip_gather_kernel(
+    const VT* __restrict__ V, int64_t nv, const int32_t* __restrict__ idx,
+    const int8_t* __restrict__ flag, int64_t nrec, double4* __restrict__ rec) {
+  const int64_t k = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (k >= nrec) return;
+  const int64_t v = idx[k];
+  if (v < 0 || v >= nv) return;  // ids come from the host stripifier: in range
+  rec[k] = make_double4((double)V[3 * v], (double)V[3 * v + 1], (double)V[3 * v + 2],
+                        (double)flag[k]);
+}
+
+}  // namespace
+
+hipError_t strip_gather_launch(const void* V, int vdtype, int64_t nv, const int32_t* idx,
+                               const int8_t* flag, int64_t nrec, double4* rec,
+                               hipStream_t stream) {
+  if (nrec == 0) return hipSuccess;
+  const dim3 grid((unsigned)((nrec + kBlockThreads - 1) / kBlockThreads));
+  if (vdtype == 0)
+    strip_gather_kernel<double><<<grid, kBlockThreads, 0, stream>>>((const double*)V, nv, idx,
+                                                                     flag, nrec, rec);
+  else
+    strip_gather_kernel<float><<<grid, kBlockThreads, 0, stream>>>((const float*)V, nv, idx,
+                                                                    flag, nrec, rec);
+  return hipGetLastError();
+}
+
 int batch_strip_resident_blocks() {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||

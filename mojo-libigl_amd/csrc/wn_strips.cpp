@@ -125,29 +125,28 @@ struct Stripifier {
   }
 };
 
+// Appends (vertex id, flag) records, tracking the position in the tile.
 struct Emitter {
-  std::vector<double>* out;  // 4 doubles per record
-  const double* V;
-  int vstride;
+  std::vector<int32_t>* idx;
+  std::vector<int8_t>* flag;
   int64_t in_tile = 0;
 
-  void record(int64_t v, double flag) {
-    out->push_back(V[3 * v]);
-    out->push_back(V[3 * v + 1]);
-    out->push_back(V[3 * v + 2]);
-    out->push_back(flag);
+  void record(int64_t v, int f) {
+    idx->push_back((int32_t)v);
+    flag->push_back((int8_t)f);
     if (++in_tile == kStripTile) in_tile = 0;
   }
   void pad_tile(int64_t v) {
-    while (in_tile != 0) record(v, 0.0);
+    while (in_tile != 0) record(v, 0);
   }
 };
 
 template <typename FT>
-int64_t build_records(const double* V, const FT* F, int64_t nf, std::vector<double>& out) {
+int64_t build_topology(const FT* F, int64_t nf, std::vector<int32_t>& idx,
+                       std::vector<int8_t>& flag) {
   Stripifier<FT> s(F, nf);
-  Emitter em{&out, V, 3};
-  std::vector<int64_t> faces, verts, best_f, best_v;
+  Emitter em{&idx, &flag};
+  std::vector<int64_t> faces, verts;
   int64_t last_v = nf > 0 ? corner(F, 0, 0) : 0;
   for (int64_t f = 0; f < nf; ++f) {
     if (s.used[f]) continue;
@@ -173,32 +172,52 @@ int64_t build_records(const double* V, const FT* F, int64_t nf, std::vector<doub
         space = kStripTile;
       }
       const size_t m = std::min<size_t>(faces.size() - i, (size_t)(space - 2));
-      em.record(verts[i], 0.0);
-      em.record(verts[i + 1], 0.0);
+      em.record(verts[i], 0);
+      em.record(verts[i + 1], 0);
       for (size_t k = 0; k < m; ++k) {
         const int64_t face = faces[i + k];
-        em.record(verts[i + k + 2], (double)s.orientation(face, verts[i + k], verts[i + k + 1]));
+        em.record(verts[i + k + 2], s.orientation(face, verts[i + k], verts[i + k + 1]));
       }
       last_v = verts[i + m + 1];
       i += m;
     }
   }
   em.pad_tile(last_v);
-  return (int64_t)(out.size() / 4);
+  return (int64_t)idx.size();
 }
 
 }  // namespace
 
-// Builds the records of mesh (V: nv x 3 float64, F: nf x 3 of fdtype
-// 0 = int64, 1 = int32). Indices must be in [0, nv). Returns the record count
-// (a multiple of kStripTile; 0 for an empty mesh).
+// Strip topology of mesh F (nf x 3 of fdtype 0 = int64, 1 = int32; indices
+// in [0, nv) with nv < 2^31): per record the vertex id and the flag (0, +1,
+// -1). Returns the record count (a multiple of kStripTile; 0 for no faces).
+int64_t strip_topology(const void* F, int64_t nf, int fdtype, std::vector<int32_t>& idx,
+                       std::vector<int8_t>& flag) {
+  idx.clear();
+  flag.clear();
+  if (nf <= 0) return 0;
+  idx.reserve((size_t)(nf + nf / 4 + kStripTile));
+  flag.reserve((size_t)(nf + nf / 4 + kStripTile));
+  return fdtype == 0 ? build_topology((const int64_t*)F, nf, idx, flag)
+                     : build_topology((const int32_t*)F, nf, idx, flag);
+}
+
+// Records (x, y, z, flag) of mesh (V: nv x 3 float64, F as above): the
+// topology's vertex ids gathered from V. Returns the record count.
 int64_t strip_records(const double* V, const void* F, int64_t nf, int fdtype,
                       std::vector<double>& out) {
-  out.clear();
-  if (nf <= 0) return 0;
-  out.reserve((size_t)(4 * (nf + nf / 4 + kStripTile)));
-  return fdtype == 0 ? build_records(V, (const int64_t*)F, nf, out)
-                     : build_records(V, (const int32_t*)F, nf, out);
+  std::vector<int32_t> idx;
+  std::vector<int8_t> flag;
+  const int64_t n = strip_topology(F, nf, fdtype, idx, flag);
+  out.resize((size_t)(4 * n));
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t v = idx[(size_t)k];
+    out[(size_t)(4 * k)] = V[3 * v];
+    out[(size_t)(4 * k + 1)] = V[3 * v + 1];
+    out[(size_t)(4 * k + 2)] = V[3 * v + 2];
+    out[(size_t)(4 * k + 3)] = (double)flag[(size_t)k];
+  }
+  return n;
 }
 
 }  // namespace wn

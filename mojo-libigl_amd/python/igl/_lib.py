@@ -40,6 +40,8 @@ _SIGNATURES = {
     "wn_pack_soa": ([I] * 9, I),
     "wn_mesh_records": ([I] * 7, I),
     "wn_mesh_create": ([I] * 5, I),
+    "wn_mesh_create_topology": ([I] * 4, I),
+    "wn_mesh_set_vertices": ([I] * 5, I),
     "wn_mesh_info": ([I] * 2, I),
     "wn_mesh_resolve": ([I] * 5, I),
     "wn_mesh_workspace_bytes": ([I] * 4, I),

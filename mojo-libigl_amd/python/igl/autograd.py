@@ -9,7 +9,8 @@ PyTorch's current stream, with PyTorch only allocating device memory:
   for dL/dO and dL/dV (FP64 adjoint, face tiles recomputed).
 
 ``winding_number(V, F, O, precision="fp64")`` is the differentiable entry
-point. V may be float32 or float64, F int32 or int64, O float32 or float64;
+point; ``WindingField(F, nv)`` serves repeated calls on one topology with the
+FP32-compensated strip kernel forward. V may be float32 or float64, F int32 or int64, O float32 or float64;
 w is float64 and gradients come back in the dtype of their input.
 """
 
@@ -137,3 +138,57 @@ class WindingNumber(torch.autograd.Function):
 def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision="fp64"):
     """Differentiable generalized winding number of (V, F) at the rows of O."""
     return WindingNumber.apply(V, F, O, precision)
+
+
+class _StripWinding(torch.autograd.Function):
+    """Forward on a resident strip mesh (FP32-compensated), FP64 adjoint."""
+
+    @staticmethod
+    def forward(ctx, V, O, field):
+        field.mesh.set_vertices(V)
+        Q = O.to(torch.float64).contiguous()
+        w = field.mesh.winding_number(Q)
+        ctx.save_for_backward(V, Q)
+        ctx.field = field
+        ctx.odtype = O.dtype
+        return w
+
+    @staticmethod
+    def backward(ctx, grad_w):
+        V, Q = ctx.saved_tensors
+        field = ctx.field
+        need_v, need_o = ctx.needs_input_grad[:2]
+        tri = pack_faces(V, field.F, validate=False)
+        gq, gtri = backward(tri, Q, grad_w)
+        gv = scatter_corners(gtri, field.F, field.nv).to(V.dtype) if need_v else None
+        go = gq.to(ctx.odtype) if need_o else None
+        return gv, go, None
+
+
+class WindingField:
+    """Differentiable winding number of a fixed-topology mesh whose vertices
+    change between calls (an optimisation loop): the strip topology is built
+    once on the host, every call gathers the strip records from V on the
+    device and runs the FP32-compensated strip kernel; backward is the FP64
+    adjoint (as winding_number's).
+
+        field = WindingField(F, n_vertices)
+        w = field(V, O)            # V (nv, 3), O (q, 3) CUDA tensors
+    """
+
+    def __init__(self, F, n_vertices: int):
+        from .mesh import Mesh
+
+        Fh = F.detach().cpu().numpy() if isinstance(F, torch.Tensor) else np.asarray(F)
+        self.mesh = Mesh.from_faces(Fh, int(n_vertices))
+        self.F = torch.as_tensor(np.ascontiguousarray(Fh, dtype=np.int64), device="cuda")
+        self.nv = int(n_vertices)
+
+    def __call__(self, V: torch.Tensor, O: torch.Tensor) -> torch.Tensor:
+        _require_cuda(V, O)
+        if V.shape != (self.nv, 3):
+            raise ValueError("V must have shape (n_vertices, 3)")
+        return _StripWinding.apply(V, O, self)
+
+    def close(self) -> None:
+        self.mesh.close()

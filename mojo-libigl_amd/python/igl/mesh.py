@@ -50,6 +50,44 @@ class Mesh:
             raise RuntimeError(f"wn_mesh_create failed: {last_error()}")
         self.handle = handle
         self.n_faces = len(faces)
+        self.n_vertices = len(vertices)
+
+    @classmethod
+    def from_faces(cls, F, n_vertices: int):
+        """A mesh from its faces alone; give it vertex positions with
+        set_vertices (device tensors) before querying."""
+        from . import _faces, _integer
+
+        faces = _faces(F)
+        nv = _integer(n_vertices, "n_vertices")
+        if faces.size and (faces.min() < 0 or faces.max() >= nv):
+            raise IndexError("face index is outside V")
+        self = cls.__new__(cls)
+        handle = lib().wn_mesh_create_topology(addr(faces), nv, len(faces), 0)
+        if handle <= 0:
+            raise RuntimeError(f"wn_mesh_create_topology failed: {last_error()}")
+        self.handle = handle
+        self.n_faces = len(faces)
+        self.n_vertices = nv
+        return self
+
+    def set_vertices(self, V) -> None:
+        """New vertex positions from a (nv, 3) float32/float64 CUDA tensor, on
+        the current stream (device gather of the strip records; no
+        re-stripification, no host copy)."""
+        import torch
+
+        if not (isinstance(V, torch.Tensor) and V.is_cuda):
+            raise TypeError("V must be a CUDA tensor")
+        if V.shape != (self.n_vertices, 3) or V.dtype not in (torch.float32, torch.float64):
+            raise ValueError("V must be a (n_vertices, 3) float32 or float64 tensor")
+        V = V.detach().contiguous()
+        st = lib().wn_mesh_set_vertices(self.handle, V.data_ptr(), self.n_vertices,
+                                        0 if V.dtype == torch.float64 else 1,
+                                        torch.cuda.current_stream().cuda_stream)
+        if st != 0:
+            raise RuntimeError(f"wn_mesh_set_vertices failed: {last_error()}")
+        self._keep = V  # the gather may still be reading it on the stream
 
     def info(self) -> dict:
         out = np.zeros(3, dtype=np.int64)

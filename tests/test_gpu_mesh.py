@@ -83,3 +83,65 @@ def test_mesh_rejects_bad_faces(require_gpu):
     V, F = W.tetrahedron()
     with pytest.raises(IndexError):
         igl.Mesh(V, F + 4)
+
+
+def test_topology_mesh_with_device_vertices(require_gpu):
+    """Mesh.from_faces + set_vertices gathers the same records on the device
+    as Mesh(V, F) builds on the host: identical w; moving the vertices and
+    re-gathering equals a fresh mesh of the moved vertices."""
+    from igl.mesh import Mesh
+
+    V, F, Q = W.config2(4096)
+    Qd = torch.tensor(Q, device="cuda")
+    ref = _mesh(V, F)
+    topo = Mesh.from_faces(F, len(V))
+    try:
+        topo.set_vertices(torch.tensor(V, device="cuda"))
+        np.testing.assert_array_equal(topo.winding_number(Qd).cpu().numpy(),
+                                      ref.winding_number(Qd).cpu().numpy())
+        V2 = V * 1.3 + [0.1, -0.2, 0.05]
+        topo.set_vertices(torch.tensor(V2, device="cuda"))
+        fresh = _mesh(V2, F)
+        np.testing.assert_array_equal(topo.winding_number(Qd).cpu().numpy(),
+                                      fresh.winding_number(Qd).cpu().numpy())
+        fresh.close()
+        V3 = V.astype(np.float32)
+        topo.set_vertices(torch.tensor(V3, device="cuda"))
+        f32 = _mesh(V3.astype(np.float64), F)
+        np.testing.assert_array_equal(topo.winding_number(Qd).cpu().numpy(),
+                                      f32.winding_number(Qd).cpu().numpy())
+        f32.close()
+        with pytest.raises(ValueError):
+            topo.set_vertices(torch.zeros((len(V) + 1, 3), device="cuda"))
+        with pytest.raises(TypeError):
+            topo.set_vertices(V)
+    finally:
+        topo.close()
+        ref.close()
+    with pytest.raises(IndexError):
+        Mesh.from_faces(F, len(V) - 1)
+
+
+def test_winding_field_forward_and_gradients(require_gpu):
+    """WindingField (strip forward, FP64 adjoint) against winding_number:
+    w within the FP32-compensated bar of FP64, gradients equal to the FP64
+    path's (same adjoint kernel on the same faces) to rounding."""
+    import igl.autograd as A
+
+    V, F = W.icosphere(3)
+    Q = np.random.default_rng(3).uniform(-1.4, 1.4, size=(3000, 3))
+    t = np.random.default_rng(4).uniform(size=3000)
+    field = A.WindingField(F, len(V))
+    out = []
+    for use_field in (True, False):
+        Vd = torch.tensor(V, device="cuda", requires_grad=True)
+        Od = torch.tensor(Q, device="cuda", requires_grad=True)
+        w = field(Vd, Od) if use_field else A.winding_number(Vd, torch.tensor(F, device="cuda"), Od)
+        ((w - torch.tensor(t, device="cuda")) ** 2).sum().backward()
+        out.append((w.detach().cpu().numpy(), Vd.grad.cpu().numpy(), Od.grad.cpu().numpy()))
+    field.close()
+    (w1, gv1, go1), (w0, gv0, go0) = out
+    assert np.abs(w1 - w0).max() <= 1e-6
+    # dL/dw differs by 2 (w1 - w0) <= 2e-6 per query, so gradients agree to that scale
+    assert np.abs(gv1 - gv0).max() <= 1e-4 * max(1.0, np.abs(gv0).max())
+    assert np.abs(go1 - go0).max() <= 1e-4 * max(1.0, np.abs(go0).max())
