@@ -660,8 +660,8 @@ def run_copy(args, h):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="torus",
                     choices=("torus", "batch", "stress", "grad", "readme", "aabb", "fast", "copy"))
     ap.add_argument("--queries", type=int, default=1_048_576)
