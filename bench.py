@@ -459,33 +459,64 @@ def run_grad(args, h):
 
     ms, _ = timed(step, args.steps, args.warmup)
     field = A.WindingField(Fd, len(V))
+    field_faces = A.WindingField(Fd, len(V), adjoint="faces")
 
-    def step_field():
-        Vd.grad = None
-        Od.grad = None
-        w = field(Vd, Od)
-        ((w - td) ** 2).sum().backward()
+    def step_with(fld):
+        def run():
+            Vd.grad = None
+            Od.grad = None
+            w = fld(Vd, Od)
+            ((w - td) ** 2).sum().backward()
+        return run
 
-    ms_field, _ = timed(step_field, args.steps, args.warmup)
+    ms_faces, _ = timed(step_with(field_faces), args.steps, args.warmup)
+    ms_field, _ = timed(step_with(field), args.steps, args.warmup)
     field_fwd_ms, _ = timed(lambda: field(Vd.detach(), Od.detach()), args.steps, 1)
+    n_edges = int(field.weight.shape[0])
     field.close()
+    field_faces.close()
     pairs = len(F) * len(Q)
     tri = A.pack_faces(Vd.detach(), Fd)
     Qd = Od.detach().double()
     fwd_ms, _ = timed(lambda: A.forward(tri, Qd, "fp32c"), args.steps, 1)
     g = torch.ones(len(Q), dtype=torch.float64, device="cuda")
     bwd_ms, _ = timed(lambda: A.backward(tri, Qd, g), args.steps, 1)
+    edge_ms, _ = timed(lambda: A.backward_edges(Vd.detach(), field.edges, field.weight, Qd, g),
+                       args.steps, 1)
+
+    # the same sphere with a polar cap cut away (an open surface): the edge
+    # adjoint's work is its boundary rim
+    cz = V[F].mean(axis=1)[:, 2]
+    Fo = F[cz < 0.6 * np.abs(V[:, 2]).max()]
+    Fod = torch.tensor(Fo, device="cuda")
+    eo, mo = A.boundary_edges(Fo, len(V))
+    tri_o = A.pack_faces(Vd.detach(), Fod)
+    gr = torch.tensor(np.random.default_rng(504).standard_normal(len(Q)), device="cuda")
+    open_face_ms, _ = timed(lambda: A.backward(tri_o, Qd, gr), args.steps, 1)
+    open_edge_ms, _ = timed(lambda: A.backward_edges(Vd.detach(), eo, mo, Qd, gr), args.steps, 1)
+    fq, ftri = A.backward(tri_o, Qd, gr)
+    fV = A.scatter_corners(ftri, Fod, len(V))
+    eq, eV = A.backward_edges(Vd.detach(), eo, mo, Qd, gr)
+    diff = max(float((eq - fq).abs().max() / fq.abs().max()),
+               float((eV - fV).abs().max() / fV.abs().max()))
     return base_line(
         args, "G query-face pairs/s, forward + adjoint (dL/dO, dL/dV), 1 MI355X",
         pairs / (ms_field * 1e-3) / 1e9, "G query-face pairs/s", ms_field, "f32+f64",
         f"frequency-50 geodesic sphere ({len(F)} faces, float32) x {len(Q)} queries, "
         "L = sum (w - t)^2; igl.autograd.WindingField: strip records gathered from V on "
-        "the device each step, FP32-compensated strip forward + FP64 adjoint",
-        forward_ms=field_fwd_ms, backward_ms=bwd_ms,
+        "the device each step, FP32-compensated strip forward + FP64 edge adjoint (the "
+        f"closed sphere has {n_edges} boundary edges: exact zero gradient)",
+        forward_ms=field_fwd_ms, edge_adjoint_ms=edge_ms, face_adjoint_ms=bwd_ms,
+        face_adjoint_step_ms=ms_faces,
+        face_adjoint_pairs_per_s=pairs / (bwd_ms * 1e-3) / 1e9,
         autograd_winding_number={"ms_per_step": ms, "forward_ms": fwd_ms,
                                  "note": "igl.autograd.winding_number: device face pack, "
-                                         "FP32-compensated face-record forward + FP64 adjoint"},
-        backward_pairs_per_s=pairs / (bwd_ms * 1e-3) / 1e9,
+                                         "FP32-compensated face-record forward + FP64 edge "
+                                         "adjoint (boundary edges from F on the host)"},
+        open_cap={"faces": len(Fo), "boundary_edges": int(mo.shape[0]),
+                  "face_adjoint_ms": open_face_ms, "edge_adjoint_ms": open_edge_ms,
+                  "edge_pairs_per_s": int(mo.shape[0]) * len(Q) / (open_edge_ms * 1e-3) / 1e9,
+                  "max_rel_diff_edges_vs_faces": diff},
         gradient_max={"dL/dV": float(Vd.grad.abs().max()), "dL/dO": float(Od.grad.abs().max())})
 
 

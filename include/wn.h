@@ -208,6 +208,21 @@ int64_t wn_batch_info(int64_t handle, int64_t *out6);
 /* Frees a plan (synchronises the device first). */
 int64_t wn_batch_destroy(int64_t handle);
 
+/* No reference counterpart. The strip topology of (host) F (nv vertices,
+ * nv < 2^31): per record the vertex id and the flag (0, +1, -1) of
+ * wn_mesh_records' format. Fills (host) idx and flag when capacity suffices;
+ * returns the record count or a negative status. */
+int64_t wn_strip_topology(const void *F, int64_t nv, int64_t nf, int64_t fdtype,
+                          int32_t *idx, int8_t *flag, int64_t capacity);
+
+/* No reference counterpart. Device gather of strip records: rec[k] =
+ * (V[idx[k]], flag[k]) for (device) V (nv x 3, vdtype 0 = float64,
+ * 1 = float32), idx, flag and rec (nrec x 4 float64). Ids outside [0, nv)
+ * leave their record untouched. Asynchronous on `stream`; allocates
+ * nothing. */
+int64_t wn_strip_gather(const void *V, int64_t nv, int64_t vdtype, const int32_t *idx,
+                        const int8_t *flag, int64_t nrec, double *rec, void *stream);
+
 /* ---- Point-mesh squared distance -------------------------------------- */
 
 /* Replaces the reference's NumPy tree build, python/igl/__init__.py:286-319
@@ -300,6 +315,31 @@ int64_t wn_fwn_destroy(int64_t handle);
 int64_t wn_backward(const double *tri, const double *q, const double *grad_w,
                     double *grad_q, double *grad_tri, int64_t nf, int64_t nq,
                     int64_t precision, int64_t qpl, int64_t splits, void *stream);
+
+/* No reference counterpart. Net directed boundary edges of (host) F over nv
+ * vertices (nv < 2^31): an edge {u, v} used n times as u->v and m times as
+ * v->u with n != m is one edge, directed u->v with weight n - m when n > m
+ * and v->u with weight m - n otherwise; a closed, consistently oriented mesh
+ * has none. Fills (host) edges (2 x count int32, source then target) and
+ * weight (count float64) when capacity suffices; returns the count or a
+ * negative status. */
+int64_t wn_boundary_edges(const void *F, int64_t nv, int64_t nf, int64_t fdtype,
+                          int32_t *edges, double *weight, int64_t capacity);
+
+/* No reference counterpart. dL/dO and dL/dV of the winding number from the
+ * boundary edges alone: per face the gradient of atan2(det, den) is a sum
+ * over its three directed edges, whose terms cancel on edges shared by
+ * oppositely directed faces, so this equals wn_backward followed by
+ * wn_scatter_corners (to rounding). (device) V (nv x 3; vdtype 0 = float64,
+ * 1 = float32), edges/weight from wn_boundary_edges, q, grad_w (nq);
+ * grad_q (nq x 3) and grad_V (nv x 3) overwritten, in FP64. Edge ids outside
+ * [0, nv) contribute nothing. Asynchronous on `stream`; allocates nothing.
+ * qpl 1, 2 or 4 (0 = 4); splits 0 = default. grad_V is accumulated with FP64
+ * atomics; grad_q is exact-order when one split runs. */
+int64_t wn_backward_edges(const void *V, int64_t nv, int64_t vdtype, const int32_t *edges,
+                          const double *weight, int64_t ne, const double *q,
+                          const double *grad_w, double *grad_q, double *grad_V, int64_t nq,
+                          int64_t qpl, int64_t splits, void *stream);
 
 /* grad_V (nv x 3, overwritten) = per-vertex sum of the corner gradients of
  * grad_tri through F (nf x 3; fdtype 0 = int64, 1 = int32). (device);

@@ -24,7 +24,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIP_SOURCES = ["wn_forward.hip", "wn_strip.hip", "wn_sort.hip", "wn_batch.hip", "wn_backward.hip",
                "wn_aabb.hip", "wn_fwn.hip", "wn_capi.hip"]
-CPU_SOURCES = ["wn_cpu.cpp", "wn_strips.cpp", "wn_aabb_tree.cpp", "wn_fwn_tree.cpp"]
+CPU_SOURCES = ["wn_cpu.cpp", "wn_strips.cpp", "wn_edges.cpp", "wn_aabb_tree.cpp", "wn_fwn_tree.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

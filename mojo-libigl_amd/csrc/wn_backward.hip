@@ -25,6 +25,10 @@
 // atomic per face-corner component per block -- so grad_tri and grad_V are
 // sums in an unspecified order (not bitwise reproducible), grad_q with
 // splits == 1 is.
+//
+// bwd_edge_kernel computes dL/dO and dL/dV from the mesh's net boundary
+// edges instead (the derivation is above the kernel); same tiling, queries
+// per lane and reduction, six components per edge.
 
 #include "wn_device.h"
 #include "wn_internal.h"
@@ -228,6 +232,154 @@ __global__ __launch_bounds__(kBlockThreads) void bwd_f64_kernel(
   }
 }
 
+// Edge form of the same adjoint. theta_f is half the solid angle of face f,
+// a function of its boundary loop only, and the variation of a loop's solid
+// angle under a motion of one straight edge P1 -> P2 integrates in closed
+// form: with a = P1 - O, b = P2 - O and t = la lb + a.b,
+//   d theta / dP1 = -(a x b) / (2 la t),  d theta / dP2 = -(a x b) / (2 lb t),
+// and d theta / dO = -(d/dP1 + d/dP2) summed over the face's three edges.
+// Summed over a mesh, the terms of an edge shared by two faces with opposite
+// directions cancel identically, so dL/dO and dL/dV are sums over the net
+// directed boundary edges alone (weight = net count) -- zero for a closed,
+// consistently oriented mesh. t is formed as |a x b|^2 / (la lb - a.b) when
+// a.b < 0 (no cancellation near the edge); t == 0 (query on the edge) and
+// zero distances contribute 0.
+
+// Reduce-scatter of six per-lane values over each 16-lane row, with the
+// exchanges of row_reduce_scatter9: components {0,1,2 | 3,4,5} by lane bit 3,
+// then {first two | third} by bit 2, then {first | second} by bit 1 (lanes
+// with bit 2 set keep their one component), then the lane^1 sum. Lane l
+// returns the row sum of component edge_slot(l) (6 = none: the duplicates).
+__device__ __forceinline__ int edge_slot(int lane) {
+  // Nibble l: {0, -, 1, -, 2, -, -, -, 3, -, 4, -, 5, -, -, -}.
+  constexpr uint64_t kSlots = 0x6665646366626160ull;
+  return (int)((kSlots >> (4 * (lane & 15))) & 0xF);
+}
+
+__device__ __forceinline__ double row_reduce_scatter6(const double (&v)[6], int lane) {
+  const bool b3 = lane & 8, b2 = lane & 4, b1 = lane & 2;
+  double u[3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+    u[p] = (b3 ? v[3 + p] : v[p]) + dpp_move_f64<0x140>(b3 ? v[p] : v[3 + p]);
+  const double x0 = (b2 ? u[2] : u[0]) + dpp_move_f64<0x141>(b2 ? u[0] : u[2]);
+  const double x1 = u[1] + dpp_move_f64<0x141>(u[1]);
+  const double keep = b2 ? x0 : (b1 ? x1 : x0);
+  const double give = b2 ? x0 : (b1 ? x0 : x1);
+  const double y = keep + dpp_move_f64<0x4E>(give);
+  return y + dpp_move_f64<0xB1>(y);
+}
+
+template <int QPL, typename VT>
+__global__ __launch_bounds__(kBlockThreads) void bwd_edge_kernel(
+    const VT* __restrict__ V, const int32_t* __restrict__ edges,
+    const double* __restrict__ weight, const double* __restrict__ q,
+    const double* __restrict__ gw, double* __restrict__ gq, double* __restrict__ gv, int64_t nv,
+    int64_t ne, int64_t nq, int64_t edges_per_split, int splits) {
+  __shared__ double2 sedge[kFaceTile][4];
+  __shared__ double sacc[kFaceTile][6];
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int slot = edge_slot(lane);
+  const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
+
+  double ox[QPL], oy[QPL], oz[QPL], g[QPL], dx[QPL], dy[QPL], dz[QPL];
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    const bool ok = qi < nq;
+    ox[k] = ok ? q[3 * qi] : 0.0;
+    oy[k] = ok ? q[3 * qi + 1] : 0.0;
+    oz[k] = ok ? q[3 * qi + 2] : 0.0;
+    g[k] = ok ? gw[qi] * kInv2Pi : 0.0;
+    dx[k] = dy[k] = dz[k] = 0.0;
+  }
+
+  const int64_t e0 = (int64_t)blockIdx.y * edges_per_split;
+  const int64_t e1 = e0 + edges_per_split < ne ? e0 + edges_per_split : ne;
+  for (int64_t t0 = e0; t0 < e1; t0 += kFaceTile) {
+    const int cnt = (int)(e1 - t0 < kFaceTile ? e1 - t0 : kFaceTile);
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t e = t0 + tid;
+      const int64_t i = edges[2 * e], j = edges[2 * e + 1];
+      // an id outside V makes a zero-weight edge at the origin (contributes 0)
+      const bool in = i >= 0 && i < nv && j >= 0 && j < nv;
+      const int64_t ii = in ? i : 0, jj = in ? j : 0;
+      const double s = in ? 1.0 : 0.0;
+      sedge[tid][0] = make_double2(s * (double)V[3 * ii], s * (double)V[3 * ii + 1]);
+      sedge[tid][1] = make_double2(s * (double)V[3 * ii + 2], s * (double)V[3 * jj]);
+      sedge[tid][2] = make_double2(s * (double)V[3 * jj + 1], s * (double)V[3 * jj + 2]);
+      sedge[tid][3] = make_double2(in ? -0.5 * weight[e] : 0.0, 0.0);
+    }
+    for (int i = tid; i < kFaceTile * 6; i += kBlockThreads) (&sacc[0][0])[i] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const double2 p0 = sedge[j][0], p1 = sedge[j][1], p2 = sedge[j][2];
+      const double hw = sedge[j][3].x;
+      double fa[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) fa[i] = 0.0;
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        const double ax = p0.x - ox[k], ay = p0.y - oy[k], az = p1.x - oz[k];
+        const double bx = p1.y - ox[k], by = p2.x - oy[k], bz = p2.y - oz[k];
+        const double ra = rsqrt_nr(ax * ax + ay * ay + az * az);
+        const double rb = rsqrt_nr(bx * bx + by * by + bz * bz);
+        const double la = (ax * ax + ay * ay + az * az) * ra;
+        const double lb = (bx * bx + by * by + bz * bz) * rb;
+        const double ab = ax * bx + ay * by + az * bz;
+        const double xx = ay * bz - az * by, xy = az * bx - ax * bz, xz = ax * by - ay * bx;
+        const double lalb = la * lb;
+        const bool pos = ab >= 0.0;
+        const double x2 = xx * xx + xy * xy + xz * xz;
+        const double dd = pos ? lalb + ab : x2;  // t, or |a x b|^2 = t (la lb - a.b)
+        const double num = pos ? 1.0 : lalb - ab;
+        // a zero distance makes ra (and so dd) NaN; dd == 0 is a query on the edge
+        const bool ok = dd > 0.0 && dd < __builtin_inf();
+        const double kk = hw * g[k] * num * rcp_nr(ok ? dd : 1.0);
+        const double k1 = ok ? kk * ra : 0.0, k2 = ok ? kk * rb : 0.0;
+        fa[0] += k1 * xx; fa[1] += k1 * xy; fa[2] += k1 * xz;
+        fa[3] += k2 * xx; fa[4] += k2 * xy; fa[5] += k2 * xz;
+        const double k12 = k1 + k2;
+        dx[k] -= k12 * xx;
+        dy[k] -= k12 * xy;
+        dz[k] -= k12 * xz;
+      }
+      const double rs = row_reduce_scatter6(fa, lane);
+      if (slot < 6) atomicAdd(&sacc[j][slot], rs);
+    }
+    __syncthreads();
+    if (tid < cnt) {
+      const int64_t e = t0 + tid;
+      const int64_t i = edges[2 * e], j = edges[2 * e + 1];
+      if (i >= 0 && i < nv && j >= 0 && j < nv) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          unsafeAtomicAdd(&gv[3 * i + d], sacc[tid][d]);
+          unsafeAtomicAdd(&gv[3 * j + d], sacc[tid][3 + d]);
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    if (qi < nq) {
+      if (splits == 1) {
+        gq[3 * qi] = dx[k];
+        gq[3 * qi + 1] = dy[k];
+        gq[3 * qi + 2] = dz[k];
+      } else {
+        unsafeAtomicAdd(&gq[3 * qi], dx[k]);
+        unsafeAtomicAdd(&gq[3 * qi + 1], dy[k]);
+        unsafeAtomicAdd(&gq[3 * qi + 2], dz[k]);
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlockThreads) void zero_kernel(double* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -276,6 +428,50 @@ hipError_t bwd_launch(const double* tri, const double* q, const double* gw, doub
     bwd_f64_kernel<2><<<grid, kBlockThreads, 0, stream>>>(tri, q, gw, gq, gtri, nf, nq, per, splits);
   else
     bwd_f64_kernel<4><<<grid, kBlockThreads, 0, stream>>>(tri, q, gw, gq, gtri, nf, nq, per, splits);
+  return hipGetLastError();
+}
+
+const char* bwd_edge_check(int64_t ne, int64_t nq, int qpl, int splits) {
+  if (ne < 0 || nq < 0) return "negative size";
+  if (qpl != 1 && qpl != 2 && qpl != 4) return "backward queries_per_lane must be 1, 2 or 4";
+  if (splits < 1 || splits > 65535) return "splits must be in [1, 65535]";
+  if (nq > (int64_t)0x7fffffff) return "more than 2^31-1 queries per call";
+  return nullptr;
+}
+
+template <typename VT>
+static void edge_dispatch(dim3 grid, const VT* V, int64_t nv, const int32_t* edges,
+                          const double* weight, int64_t ne, const double* q, const double* gw, double* gq, double* gv,
+                          int64_t nq, int qpl, int64_t per, int splits, hipStream_t stream) {
+  if (qpl == 1)
+    bwd_edge_kernel<1, VT><<<grid, kBlockThreads, 0, stream>>>(V, edges, weight, q, gw, gq, gv,
+                                                               nv, ne, nq, per, splits);
+  else if (qpl == 2)
+    bwd_edge_kernel<2, VT><<<grid, kBlockThreads, 0, stream>>>(V, edges, weight, q, gw, gq, gv,
+                                                               nv, ne, nq, per, splits);
+  else
+    bwd_edge_kernel<4, VT><<<grid, kBlockThreads, 0, stream>>>(V, edges, weight, q, gw, gq, gv,
+                                                               nv, ne, nq, per, splits);
+}
+
+hipError_t bwd_edge_launch(const void* V, int vdtype, int64_t nv, const int32_t* edges,
+                           const double* weight, int64_t ne, const double* q, const double* gw,
+                           double* gq, double* gv, int64_t nq, int qpl, int splits,
+                           hipStream_t stream) {
+  if (nq > 0) zero_kernel<<<grid1(3 * nq), kBlockThreads, 0, stream>>>(gq, 3 * nq);
+  if (nv > 0) zero_kernel<<<grid1(3 * nv), kBlockThreads, 0, stream>>>(gv, 3 * nv);
+  // with no vertices every edge id is outside V: nothing to add
+  if (nq == 0 || ne == 0 || nv == 0) return hipGetLastError();
+  int64_t per = (ne + splits - 1) / splits;
+  per = (per + kFaceTile - 1) / kFaceTile * kFaceTile;
+  const int64_t qblock = (int64_t)kBlockThreads * qpl;
+  const dim3 grid((unsigned)((nq + qblock - 1) / qblock), (unsigned)((ne + per - 1) / per));
+  if (vdtype == 0)
+    edge_dispatch(grid, (const double*)V, nv, edges, weight, ne, q, gw, gq, gv, nq, qpl, per,
+                  (int)grid.y, stream);
+  else
+    edge_dispatch(grid, (const float*)V, nv, edges, weight, ne, q, gw, gq, gv, nq, qpl, per,
+                  (int)grid.y, stream);
   return hipGetLastError();
 }
 

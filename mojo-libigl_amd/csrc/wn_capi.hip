@@ -457,6 +457,54 @@ int64_t wn_backward(const double* tri, const double* q, const double* grad_w, do
   return kOk;
 }
 
+int64_t wn_boundary_edges(const void* F, int64_t nv, int64_t nf, int64_t fdtype, int32_t* edges,
+                          double* weight, int64_t capacity) {
+  if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (nv > (int64_t)0x7fffffff) return fail(kErrArg, "more than 2^31-1 vertices");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if (nf > 0 && !F) return fail(kErrArg, "null host buffer");
+  for (int64_t i = 0; i < 3 * nf; ++i) {
+    const int64_t v = fdtype == 0 ? ((const int64_t*)F)[i] : (int64_t)((const int32_t*)F)[i];
+    if (v < 0 || v >= nv) return fail(kErrArg, "face index is outside V");
+  }
+  std::vector<int32_t> e;
+  std::vector<double> m;
+  const int64_t n = wn::boundary_edges(F, nf, (int)fdtype, e, m);
+  if (edges && weight && capacity >= n) {
+    std::memcpy(edges, e.data(), e.size() * sizeof(int32_t));
+    std::memcpy(weight, m.data(), m.size() * sizeof(double));
+  }
+  return n;
+}
+
+int64_t wn_backward_edges(const void* V, int64_t nv, int64_t vdtype, const int32_t* edges,
+                          const double* weight, int64_t ne, const double* q,
+                          const double* grad_w, double* grad_q, double* grad_V, int64_t nq,
+                          int64_t qpl, int64_t splits, void* stream) {
+  if (vdtype != 0 && vdtype != 1) return fail(kErrArg, "vdtype must be 0 (float64) or 1 (float32)");
+  if (nv < 0) return fail(kErrArg, "negative size");
+  const int lanes = (int)(qpl <= 0 ? 4 : qpl);
+  int s = (int)splits;
+  if (s <= 0) {
+    // about 2048 blocks, at least one edge tile per split
+    const int64_t qblocks = (nq + 256 * lanes - 1) / (256 * lanes);
+    int64_t t = (2048 + qblocks - 1) / (qblocks > 0 ? qblocks : 1);
+    const int64_t max_s = (ne + wn::kFaceTile - 1) / wn::kFaceTile;
+    if (t > max_s) t = max_s;
+    if (t < 1) t = 1;
+    if (t > 4096) t = 4096;
+    s = (int)t;
+  }
+  if (const char* why = wn::bwd_edge_check(ne, nq, lanes, s)) return fail(kErrArg, why);
+  if ((nq > 0 && (!q || !grad_w || !grad_q)) || (nv > 0 && !grad_V) ||
+      (ne > 0 && nq > 0 && (!V || !edges || !weight)))
+    return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::bwd_edge_launch(V, (int)vdtype, nv, edges, weight, ne, q, grad_w, grad_q,
+                                     grad_V, nq, lanes, s, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_backward_edges", e));
+  return kOk;
+}
+
 int64_t wn_batch_create(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
                         int64_t precision, int64_t qpl, int64_t chunk_faces, int64_t sort,
                         double kappa) {
@@ -896,6 +944,38 @@ int64_t wn_fwn_destroy(int64_t handle) {
     (void)hipFree(h->dev);
   }
   delete h;
+  return kOk;
+}
+
+int64_t wn_strip_topology(const void* F, int64_t nv, int64_t nf, int64_t fdtype, int32_t* idx,
+                          int8_t* flag, int64_t capacity) {
+  if (nv < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (nv > (int64_t)0x7fffffff) return fail(kErrArg, "more than 2^31-1 vertices");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if (nf > 0 && !F) return fail(kErrArg, "null host buffer");
+  for (int64_t i = 0; i < 3 * nf; ++i) {
+    const int64_t v = fdtype == 0 ? ((const int64_t*)F)[i] : (int64_t)((const int32_t*)F)[i];
+    if (v < 0 || v >= nv) return fail(kErrArg, "face index is outside V");
+  }
+  std::vector<int32_t> ids;
+  std::vector<int8_t> flags;
+  const int64_t n = wn::strip_topology(F, nf, (int)fdtype, ids, flags);
+  if (idx && flag && capacity >= n) {
+    std::memcpy(idx, ids.data(), ids.size() * sizeof(int32_t));
+    std::memcpy(flag, flags.data(), flags.size());
+  }
+  return n;
+}
+
+int64_t wn_strip_gather(const void* V, int64_t nv, int64_t vdtype, const int32_t* idx,
+                        const int8_t* flag, int64_t nrec, double* rec, void* stream) {
+  if (nv < 0 || nrec < 0) return fail(kErrArg, "negative size");
+  if (vdtype != 0 && vdtype != 1) return fail(kErrArg, "vdtype must be 0 (float64) or 1 (float32)");
+  if (nrec == 0) return kOk;
+  if (!V || !idx || !flag || !rec) return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::strip_gather_launch(V, (int)vdtype, nv, idx, flag, nrec, (double4*)rec,
+                                         (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_strip_gather", e));
   return kOk;
 }
 

@@ -90,6 +90,17 @@ hipError_t bwd_launch(const double* tri, const double* q, const double* gw, doub
 // grad_V[F[f, c]] += grad_tri corner c of face f; grad_V (nv x 3) overwritten.
 hipError_t scatter_corners_launch(const double* gtri, const void* F, int fdtype, int64_t nf,
                                   int64_t nv, double* gv, hipStream_t stream);
+// Net directed boundary edges of F (wn_edges.cpp): edges (2 x ne, directed
+// vertex ids) and their positive weights (net counts); returns ne.
+int64_t boundary_edges(const void* F, int64_t nf, int fdtype, std::vector<int32_t>& edges,
+                       std::vector<double>& weight);
+// Edge adjoint (wn_backward.hip): grad_q (nq x 3) and grad_V (nv x 3, vdtype
+// 0 = float64, 1 = float32 V) overwritten from the ne weighted edges.
+const char* bwd_edge_check(int64_t ne, int64_t nq, int qpl, int splits);
+hipError_t bwd_edge_launch(const void* V, int vdtype, int64_t nv, const int32_t* edges,
+                           const double* weight, int64_t ne, const double* q, const double* gw,
+                           double* gq, double* gv, int64_t nq, int qpl, int splits,
+                           hipStream_t stream);
 
 }  // namespace wn
 

@@ -42,6 +42,10 @@ _SIGNATURES = {
     "wn_mesh_create": ([I] * 5, I),
     "wn_mesh_create_topology": ([I] * 4, I),
     "wn_mesh_set_vertices": ([I] * 5, I),
+    "wn_strip_topology": ([I] * 7, I),
+    "wn_strip_gather": ([I] * 8, I),
+    "wn_boundary_edges": ([I] * 7, I),
+    "wn_backward_edges": ([I] * 14, I),
     "wn_mesh_info": ([I] * 2, I),
     "wn_mesh_resolve": ([I] * 5, I),
     "wn_mesh_workspace_bytes": ([I] * 4, I),

@@ -5,8 +5,12 @@ PyTorch's current stream, with PyTorch only allocating device memory:
 
 * ``pack_faces(V, F)``      -> wn_pack_soa (device gather of V[F], 9 x m f64)
 * ``forward(tri, O)``       -> wn_forward (FP64 plain or FP32-compensated)
-* ``WindingNumber.apply``   -> forward, then wn_backward + wn_scatter_corners
-  for dL/dO and dL/dV (FP64 adjoint, face tiles recomputed).
+* ``WindingNumber.apply``   -> forward, then the FP64 adjoint for dL/dO and
+  dL/dV: by default wn_backward_edges over the mesh's net boundary edges
+  (wn_boundary_edges; interior edges cancel, a closed mesh has none), or
+  with ``adjoint="faces"`` wn_backward + wn_scatter_corners over every face
+  (face tiles recomputed). Both are the exact gradient; they agree to
+  rounding.
 
 ``winding_number(V, F, O, precision="fp64")`` is the differentiable entry
 point; ``WindingField(F, nv)`` serves repeated calls on one topology with the
@@ -20,7 +24,7 @@ import numpy as np
 import torch
 
 from . import PRECISIONS
-from ._lib import last_error, lib
+from ._lib import addr, last_error, lib
 
 _INT32_MAX = 2**31 - 1
 
@@ -111,15 +115,63 @@ def scatter_corners(grad_tri: torch.Tensor, F: torch.Tensor, nv: int) -> torch.T
     return gv
 
 
+def boundary_edges(F, n_vertices: int, device="cuda"):
+    """Net directed boundary edges of F (host, wn_boundary_edges) as device
+    tensors: ((ne, 2) int32 source/target ids, (ne,) float64 weights)."""
+    Fh = F.detach().cpu().numpy() if isinstance(F, torch.Tensor) else np.asarray(F)
+    if Fh.ndim != 2 or Fh.shape[1] != 3:
+        raise ValueError("F must have shape (m, 3)")
+    Fh = np.ascontiguousarray(Fh, dtype=np.int64)
+    h = lib()
+    n = h.wn_boundary_edges(addr(Fh), int(n_vertices), len(Fh), 0, 0, 0, 0)
+    if n < 0:
+        raise (IndexError if "outside V" in last_error() else ValueError)(last_error())
+    e = np.zeros((n, 2), dtype=np.int32)
+    m = np.zeros(n, dtype=np.float64)
+    if n and h.wn_boundary_edges(addr(Fh), int(n_vertices), len(Fh), 0, e.ctypes.data,
+                                 m.ctypes.data, n) != n:
+        raise RuntimeError(last_error())
+    return torch.as_tensor(e, device=device), torch.as_tensor(m, device=device)
+
+
+def backward_edges(V: torch.Tensor, edges: torch.Tensor, weight: torch.Tensor, Q: torch.Tensor,
+                   grad_w: torch.Tensor, qpl=0, splits=0):
+    """(dL/dO (nq, 3), dL/dV (nv, 3)) in float64 from the boundary edges."""
+    if V.dtype not in (torch.float32, torch.float64):
+        raise TypeError("V must be float32 or float64")
+    V = V.detach().contiguous()
+    Q = Q.to(torch.float64).contiguous()
+    gw = grad_w.to(torch.float64).contiguous()
+    e = edges.to(torch.int32).contiguous()
+    m = weight.to(torch.float64).contiguous()
+    nq, nv, ne = Q.shape[0], V.shape[0], e.shape[0]
+    gq = torch.empty((nq, 3), dtype=torch.float64, device=Q.device)
+    gv = torch.empty((nv, 3), dtype=torch.float64, device=Q.device)
+    _check(lib().wn_backward_edges(V.data_ptr(), nv, 0 if V.dtype == torch.float64 else 1,
+                                   e.data_ptr(), m.data_ptr(), ne, Q.data_ptr(), gw.data_ptr(),
+                                   gq.data_ptr(), gv.data_ptr(), nq, qpl, splits, _stream()),
+           "wn_backward_edges")
+    return gq, gv
+
+
+ADJOINTS = ("edges", "faces")
+
+
 class WindingNumber(torch.autograd.Function):
     """w = winding_number(V, F, O); backward gives dL/dV and dL/dO."""
 
     @staticmethod
-    def forward(ctx, V, F, O, precision="fp64"):
+    def forward(ctx, V, F, O, precision="fp64", adjoint="edges"):
+        if adjoint not in ADJOINTS:
+            raise ValueError(f"adjoint must be one of {ADJOINTS}")
         tri = pack_faces(V, F)
         Q = O.to(torch.float64).contiguous()
         w = forward(tri, Q, precision)
-        ctx.save_for_backward(tri, Q, F)
+        if adjoint == "faces":
+            ctx.save_for_backward(tri, Q, F)
+        else:
+            ctx.save_for_backward(V.detach(), Q, F)
+        ctx.adjoint = adjoint
         ctx.nv = V.shape[0]
         ctx.vdtype = V.dtype
         ctx.odtype = O.dtype
@@ -127,17 +179,25 @@ class WindingNumber(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_w):
-        tri, Q, F = ctx.saved_tensors
         need_v, _, need_o = ctx.needs_input_grad[:3]
-        gq, gtri = backward(tri, Q, grad_w)
-        gv = scatter_corners(gtri, F, ctx.nv).to(ctx.vdtype) if need_v else None
+        if ctx.adjoint == "faces":
+            tri, Q, F = ctx.saved_tensors
+            gq, gtri = backward(tri, Q, grad_w)
+            gv = scatter_corners(gtri, F, ctx.nv) if need_v else None
+        else:
+            V, Q, F = ctx.saved_tensors
+            edges, weight = boundary_edges(F, ctx.nv, device=Q.device)
+            gq, gv = backward_edges(V, edges, weight, Q, grad_w)
+        gv = gv.to(ctx.vdtype) if need_v else None
         go = gq.to(ctx.odtype) if need_o else None
-        return gv, None, go, None
+        return gv, None, go, None, None
 
 
-def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision="fp64"):
-    """Differentiable generalized winding number of (V, F) at the rows of O."""
-    return WindingNumber.apply(V, F, O, precision)
+def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision="fp64",
+                   adjoint="edges"):
+    """Differentiable generalized winding number of (V, F) at the rows of O.
+    ``adjoint`` picks the backward: "edges" (boundary edges only) or "faces"."""
+    return WindingNumber.apply(V, F, O, precision, adjoint)
 
 
 class _StripWinding(torch.autograd.Function):
@@ -158,31 +218,39 @@ class _StripWinding(torch.autograd.Function):
         V, Q = ctx.saved_tensors
         field = ctx.field
         need_v, need_o = ctx.needs_input_grad[:2]
-        tri = pack_faces(V, field.F, validate=False)
-        gq, gtri = backward(tri, Q, grad_w)
-        gv = scatter_corners(gtri, field.F, field.nv).to(V.dtype) if need_v else None
+        if field.adjoint == "faces":
+            tri = pack_faces(V, field.F, validate=False)
+            gq, gtri = backward(tri, Q, grad_w)
+            gv = scatter_corners(gtri, field.F, field.nv) if need_v else None
+        else:
+            gq, gv = backward_edges(V, field.edges, field.weight, Q, grad_w)
+        gv = gv.to(V.dtype) if need_v else None
         go = gq.to(ctx.odtype) if need_o else None
         return gv, go, None
 
 
 class WindingField:
     """Differentiable winding number of a fixed-topology mesh whose vertices
-    change between calls (an optimisation loop): the strip topology is built
-    once on the host, every call gathers the strip records from V on the
-    device and runs the FP32-compensated strip kernel; backward is the FP64
-    adjoint (as winding_number's).
+    change between calls (an optimisation loop): the strip topology and the
+    boundary edges are built once on the host, every call gathers the strip
+    records from V on the device and runs the FP32-compensated strip kernel;
+    backward is the FP64 adjoint (as winding_number's, "edges" by default).
 
         field = WindingField(F, n_vertices)
         w = field(V, O)            # V (nv, 3), O (q, 3) CUDA tensors
     """
 
-    def __init__(self, F, n_vertices: int):
+    def __init__(self, F, n_vertices: int, adjoint="edges"):
         from .mesh import Mesh
 
+        if adjoint not in ADJOINTS:
+            raise ValueError(f"adjoint must be one of {ADJOINTS}")
         Fh = F.detach().cpu().numpy() if isinstance(F, torch.Tensor) else np.asarray(F)
         self.mesh = Mesh.from_faces(Fh, int(n_vertices))
         self.F = torch.as_tensor(np.ascontiguousarray(Fh, dtype=np.int64), device="cuda")
         self.nv = int(n_vertices)
+        self.adjoint = adjoint
+        self.edges, self.weight = boundary_edges(Fh, self.nv)
 
     def __call__(self, V: torch.Tensor, O: torch.Tensor) -> torch.Tensor:
         _require_cuda(V, O)

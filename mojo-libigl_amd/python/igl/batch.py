@@ -25,7 +25,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import PRECISIONS
-from ._lib import last_error, lib
+from ._lib import addr, last_error, lib
 
 
 def _mode(precision) -> int:
@@ -146,6 +146,67 @@ def pack_batch_strips(vertices, faces, device="cuda"):
     counts = np.array([len(r) for r in recs], dtype=np.int64)
     rec = np.concatenate(recs) if recs else np.zeros((0, 4))
     return torch.as_tensor(np.ascontiguousarray(rec), device=device), counts
+
+
+class BatchTopology:
+    """Strip topology of many meshes, built once on the host and kept on the
+    device: ``records(V_all)`` gathers a StripBatch's records from all
+    meshes' vertices concatenated on the device (new positions, same
+    topology) without re-stripifying.
+
+        topo = BatchTopology([F0, F1, ...], [nv0, nv1, ...])
+        plan = StripBatch(topo.record_counts, query_counts)
+        w = plan.run(topo.records(V_all), Q_all)
+    """
+
+    def __init__(self, faces, vertex_counts, device="cuda"):
+        import torch
+
+        from . import _faces
+
+        h = lib()
+        ids, flags, counts, off = [], [], [], 0
+        for F, nv in zip(faces, vertex_counts):
+            Fa = _faces(F)
+            nv = int(nv)
+            n = h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, 0, 0, 0)
+            if n < 0:
+                raise (IndexError if "outside V" in last_error() else ValueError)(last_error())
+            idx = np.empty(n, dtype=np.int32)
+            flag = np.empty(n, dtype=np.int8)
+            if n and h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, idx.ctypes.data,
+                                         flag.ctypes.data, n) != n:
+                raise RuntimeError(last_error())
+            ids.append(idx.astype(np.int64) + off)
+            flags.append(flag)
+            counts.append(n)
+            off += nv
+        if off > np.iinfo(np.int32).max:
+            raise ValueError("more than 2^31-1 vertices in the batch")
+        self.n_vertices = off
+        self.record_counts = np.asarray(counts, dtype=np.int64)
+        cat = np.concatenate(ids).astype(np.int32) if ids else np.zeros(0, np.int32)
+        self.idx = torch.as_tensor(cat, device=device)
+        self.flag = torch.as_tensor(np.concatenate(flags) if flags else np.zeros(0, np.int8),
+                                    device=device)
+
+    def records(self, V_all, out=None):
+        """(R_total, 4) float64 records for (n_vertices, 3) float32/float64
+        device vertices, gathered on the current stream."""
+        import torch
+
+        if V_all.shape != (self.n_vertices, 3) or V_all.dtype not in (torch.float32, torch.float64):
+            raise ValueError("V_all must be a (n_vertices, 3) float32 or float64 tensor")
+        V = V_all.detach().contiguous()
+        n = int(self.idx.shape[0])
+        rec = out if out is not None else torch.empty((n, 4), dtype=torch.float64, device=V.device)
+        st = lib().wn_strip_gather(V.data_ptr(), self.n_vertices,
+                                   0 if V.dtype == torch.float64 else 1, self.idx.data_ptr(),
+                                   self.flag.data_ptr(), n, rec.data_ptr(),
+                                   torch.cuda.current_stream().cuda_stream)
+        if st != 0:
+            raise RuntimeError(f"wn_strip_gather failed: {last_error()}")
+        return rec
 
 
 def pack_batch(vertices, faces, device="cuda"):

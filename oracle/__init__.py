@@ -13,6 +13,11 @@ Contents, each following the reference file:line it restates:
 * :func:`winding_mpmath` -- the same formula at 40+ digits (brute force pin).
 * :func:`winding_torch` -- the formula in float64 PyTorch ops, differentiable,
   the FP64 reference gradient for the adjoint kernel.
+* :func:`boundary_edges` -- net directed boundary edges by counting (pure
+  Python); :func:`edge_adjoint` -- dL/dO, dL/dV of sum g w as a sum over them
+  in NumPy (no reference counterpart: the reference has no derivative). Pinned
+  against the autograd gradient of :func:`winding_torch` and against hand
+  counts of edges in the CPU tests.
 * :func:`ref_gpu_atan2` -- the reference GPU kernel's own arctangent,
   ``src/capi.mojo:326-352``.
 * :func:`build_aabb` -- the median-split tree of ``python/igl/__init__.py:286-319``
@@ -177,6 +182,56 @@ def winding_torch(V, F, Q):
     det = (a * torch.linalg.cross(b, c, dim=-1)).sum(-1)
     den = la * lb * lc + (a * b).sum(-1) * lc + (b * c).sum(-1) * la + (c * a).sum(-1) * lb
     return torch.atan2(det, den).sum(-1) / (2.0 * torch.pi)
+
+
+def boundary_edges(F):
+    """{(u, v): n} for every undirected edge used n > 0 more times as u -> v
+    than as v -> u by the faces' directed edges (i->j, j->k, k->i)."""
+    net = {}
+    for f in np.asarray(F).tolist():
+        for s, t in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0])):
+            if s == t:
+                continue
+            key, sign = ((s, t), 1) if s < t else ((t, s), -1)
+            net[key] = net.get(key, 0) + sign
+    out = {}
+    for (u, v), n in net.items():
+        if n > 0:
+            out[(u, v)] = n
+        elif n < 0:
+            out[(v, u)] = -n
+    return out
+
+
+def edge_adjoint(V, F, Q, g):
+    """(dL/dV (n, 3), dL/dO (q, 3)) of L = sum_q g_q w_q, as a sum over the
+    net boundary edges: per directed edge P1 -> P2 with a = P1 - O,
+    b = P2 - O, t = |a||b| + a.b, d theta/dP1 = -(a x b)/(2|a| t),
+    d theta/dP2 = -(a x b)/(2|b| t), d theta/dO = -(both). Float64 NumPy."""
+    V = np.asarray(V, dtype=np.float64)
+    Q = np.asarray(Q, dtype=np.float64)
+    g = np.asarray(g, dtype=np.float64) / (2.0 * np.pi)
+    gV = np.zeros_like(V)
+    gO = np.zeros_like(Q)
+    E = boundary_edges(F)
+    if not E:
+        return gV, gO
+    src = np.array([k[0] for k in E])
+    dst = np.array([k[1] for k in E])
+    wt = np.array(list(E.values()), dtype=np.float64)
+    a = V[src][None, :, :] - Q[:, None, :]
+    b = V[dst][None, :, :] - Q[:, None, :]
+    la = np.linalg.norm(a, axis=-1)
+    lb = np.linalg.norm(b, axis=-1)
+    x = np.cross(a, b)
+    t = la * lb + (a * b).sum(-1)
+    k = -0.5 * wt[None, :] * g[:, None] / t
+    d1 = (k / la)[..., None] * x
+    d2 = (k / lb)[..., None] * x
+    np.add.at(gV, src, d1.sum(0))
+    np.add.at(gV, dst, d2.sum(0))
+    gO = -(d1 + d2).sum(1)
+    return gV, gO
 
 
 # ---------------------------------------------------------------------------

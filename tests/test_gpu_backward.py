@@ -181,3 +181,126 @@ def test_lane_and_split_variants_match_fp64_torch_open_subset(require_gpu, c5, q
         print(f"qpl={qpl} splits={splits}: max|grad| {scale:.3e}, max err {err:.3e}", flush=True)
         assert scale > 1e-3
         assert err <= 1e-9 * scale
+
+
+# ---- edge form of the adjoint (wn_boundary_edges + wn_backward_edges) ----
+
+def _torch_grads(V, F, O, g):
+    Vt = torch.tensor(V, dtype=torch.float64, requires_grad=True)
+    Ot = torch.tensor(O, dtype=torch.float64, requires_grad=True)
+    w = oracle.winding_torch(Vt, torch.tensor(F), Ot)
+    (w * torch.tensor(g)).sum().backward()
+    return Vt.grad.numpy(), Ot.grad.numpy()
+
+
+@pytest.mark.parametrize("qpl,splits,vdtype", [(1, 1, "f64"), (2, 3, "f64"), (4, 1, "f32"),
+                                                (4, 5, "f64"), (0, 0, "f32")])
+def test_edge_adjoint_matches_fp64_torch_open_subset(require_gpu, c5, qpl, splits, vdtype):
+    """wn_backward_edges (every lane / split variant, float32 and float64 V)
+    against the FP64 torch gradient of sum_q g_q w_q on the open cap."""
+    import igl.autograd as A
+
+    V, F, Q, t = c5
+    V = V.astype(np.float64 if vdtype == "f64" else np.float32)
+    Fs = _open_subset(V, F)
+    rng = np.random.default_rng(78)
+    sel = rng.choice(len(Q), 1000, replace=False)
+    O = Q[sel].astype(np.float64)
+    g = rng.standard_normal(len(O))
+    dev = torch.device("cuda")
+    edges, weight = A.boundary_edges(Fs, len(V))
+    assert 0 < len(weight) < len(Fs)
+    gq, gV = A.backward_edges(torch.tensor(V, device=dev), edges, weight,
+                              torch.tensor(O, device=dev), torch.tensor(g, device=dev),
+                              qpl=qpl, splits=splits)
+    rV, rO = _torch_grads(V.astype(np.float64), Fs, O, g)
+    for got, ref in ((gV.cpu().numpy(), rV), (gq.cpu().numpy(), rO)):
+        scale = np.abs(ref).max()
+        err = np.abs(got - ref).max()
+        print(f"edges qpl={qpl} splits={splits} {vdtype}: {len(weight)} edges, "
+              f"max|grad| {scale:.3e}, max err {err:.3e}", flush=True)
+        assert scale > 1e-3
+        assert err <= 1e-9 * scale
+
+
+def test_edge_and_face_adjoints_agree(require_gpu, c5):
+    """Both autograd adjoints on the open cap and on an open cap with a
+    flipped face (a weight-2 boundary), through winding_number."""
+    V, F, Q, t = c5
+    V = V.astype(np.float64)
+    Fs = _open_subset(V, F)
+    Ff = Fs.copy()
+    Ff[7] = Ff[7][::-1]
+    rng = np.random.default_rng(79)
+    O = Q[rng.choice(len(Q), 700, replace=False)].astype(np.float64)
+    ts = rng.uniform(size=len(O))
+    import igl.autograd as A
+
+    for faces in (Fs, Ff):
+        out = {}
+        for adjoint in ("edges", "faces"):
+            Vd = torch.tensor(V, device="cuda", requires_grad=True)
+            Od = torch.tensor(O, device="cuda", requires_grad=True)
+            w = A.winding_number(Vd, torch.tensor(faces, device="cuda"), Od, "fp64", adjoint)
+            ((w - torch.tensor(ts, device="cuda")) ** 2).sum().backward()
+            out[adjoint] = (Vd.grad.cpu().numpy(), Od.grad.cpu().numpy())
+        for i in range(2):
+            scale = np.abs(out["faces"][i]).max()
+            err = np.abs(out["edges"][i] - out["faces"][i]).max()
+            print(f"edges vs faces: max|grad| {scale:.3e}, max diff {err:.3e}", flush=True)
+            assert err <= 1e-9 * scale
+
+
+def test_edge_adjoint_closed_sphere_is_exactly_zero(require_gpu, c5):
+    """The closed sphere has no boundary edges: the edge adjoint returns exact
+    zeros at full size, which the face adjoint reaches only to rounding."""
+    import igl.autograd as A
+
+    V, F, Q, t = c5
+    edges, weight = A.boundary_edges(F, len(V))
+    assert len(weight) == 0
+    dev = torch.device("cuda")
+    Vd = torch.tensor(V, device=dev)
+    Od = torch.tensor(Q, device=dev, dtype=torch.float64)
+    g = torch.tensor(np.random.default_rng(2).normal(size=len(Q)), device=dev)
+    gq, gV = A.backward_edges(Vd, edges, weight, Od, g)
+    assert not gq.any() and not gV.any()
+    fq, ftri = A.backward(A.pack_faces(Vd, torch.tensor(F, device=dev)), Od, g)
+    fV = A.scatter_corners(ftri, torch.tensor(F, device=dev), len(V))
+    print(f"closed sphere: face adjoint max|dL/dO| {float(fq.abs().max()):.3e}, "
+          f"max|dL/dV| {float(fV.abs().max()):.3e} (exact 0)", flush=True)
+    assert float(fq.abs().max()) < 1e-6 and float(fV.abs().max()) < 1e-6
+
+
+def test_edge_adjoint_skips_ids_outside_V(require_gpu, c5):
+    """An edge whose id is outside [0, nv) contributes nothing (checked in the
+    kernel); the rest of the result is unchanged."""
+    import igl.autograd as A
+
+    V, F, Q, t = c5
+    Fs = _open_subset(V, F)
+    edges, weight = A.boundary_edges(Fs, len(V))
+    bad = torch.tensor([[0, len(V)], [len(V) + 5, 1], [-1, 2]], dtype=torch.int32, device="cuda")
+    e2 = torch.cat([edges, bad])
+    w2 = torch.cat([weight, torch.ones(3, dtype=torch.float64, device="cuda")])
+    dev = torch.device("cuda")
+    Vd = torch.tensor(V, device=dev)
+    Od = torch.tensor(Q[:3000], device=dev, dtype=torch.float64)
+    g = torch.tensor(np.random.default_rng(3).normal(size=3000), device=dev)
+    q1, v1 = A.backward_edges(Vd, edges, weight, Od, g, splits=1)
+    q2, v2 = A.backward_edges(Vd, e2, w2, Od, g, splits=1)
+    assert torch.equal(q1, q2)
+    # grad_V is summed with atomics (block order varies): equal to rounding
+    assert float((v1 - v2).abs().max()) <= 1e-12 * float(v1.abs().max())
+
+
+def test_edge_adjoint_empty_inputs(require_gpu):
+    import igl.autograd as A
+
+    dev = torch.device("cuda")
+    V = torch.zeros((4, 3), dtype=torch.float64, device=dev)
+    e = torch.zeros((0, 2), dtype=torch.int32, device=dev)
+    m = torch.zeros(0, dtype=torch.float64, device=dev)
+    O = torch.zeros((0, 3), dtype=torch.float64, device=dev)
+    gq, gV = A.backward_edges(V, e, m, O, torch.zeros(0, dtype=torch.float64, device=dev))
+    assert gq.shape == (0, 3) and gV.shape == (4, 3) and not gV.any()

@@ -178,3 +178,26 @@ def test_strip_batch_equals_single_mesh_strip_launches(require_gpu, small_batch)
         finally:
             m.close()
         assert np.abs(single - a).max() <= 1e-12
+
+
+def test_batch_topology_device_records(require_gpu, small_batch):
+    """BatchTopology gathers on the device exactly the records the host
+    packer builds, and again for moved vertices (same topology)."""
+    from igl.batch import BatchTopology, StripBatch, pack_batch_strips
+
+    B = small_batch
+    topo = BatchTopology([b[1] for b in B], [len(b[0]) for b in B])
+    V_all = torch.tensor(np.concatenate([b[0] for b in B]), device="cuda")
+    host_rec, counts = pack_batch_strips([b[0] for b in B], [b[1] for b in B])
+    np.testing.assert_array_equal(topo.record_counts, counts)
+    assert torch.equal(topo.records(V_all), host_rec)
+    moved = [b[0] * 1.1 + 0.05 for b in B]
+    host2, _ = pack_batch_strips(moved, [b[1] for b in B])
+    rec2 = topo.records(torch.tensor(np.concatenate(moved), device="cuda"))
+    assert torch.equal(rec2, host2)
+    Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
+    plan = StripBatch(counts, [len(b[2]) for b in B])
+    try:
+        assert torch.equal(plan.run(rec2, Q, use_graph=False), plan.run(host2, Q, use_graph=False))
+    finally:
+        plan.close()

@@ -87,3 +87,24 @@ def test_bad_index_is_rejected():
     V, F = W.tetrahedron()
     with pytest.raises(IndexError):
         strip_records(V, F + 10)
+
+
+@pytest.mark.parametrize("name", list(_meshes()))
+def test_topology_gathers_to_the_records(name):
+    """wn_strip_topology's (vertex id, flag) per record, gathered on the host,
+    is exactly strip_records' output: the device gather only needs these."""
+    from igl._lib import addr, lib
+
+    V, F = _meshes()[name]
+    F = np.ascontiguousarray(F, dtype=np.int64)
+    h = lib()
+    n = h.wn_strip_topology(addr(F), len(V), len(F), 0, 0, 0, 0)
+    assert n >= 0 and n % TILE == 0
+    idx = np.empty(n, np.int32)
+    flag = np.empty(n, np.int8)
+    assert h.wn_strip_topology(addr(F), len(V), len(F), 0, idx.ctypes.data, flag.ctypes.data, n) == n
+    R = strip_records(V, F)
+    assert len(R) == n
+    np.testing.assert_array_equal(R[:, :3], np.asarray(V, np.float64)[idx])
+    np.testing.assert_array_equal(R[:, 3], flag.astype(np.float64))
+    assert h.wn_strip_topology(addr(F), int(F.max()), len(F), 0, 0, 0, 0) < 0
