@@ -13,7 +13,9 @@
 // statically through a 3-way unrolled loop):
 //   R0[j] = (w.x, w.y, w.z, |w_j - w_{j-1}|^2)      (block-local FP32 coords)
 //   R1[j] = (|w_j - w_{j-2}|^2, N8.x, N8.y, N8.z)     (N8 = 8 x face normal)
-//   M[j]  = 0 (no face ends at j), +1 / -1 (face orientation vs strip order)
+// The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
+// orientation vs strip order) is read from the records themselves, at a
+// wave-uniform address (scalar loads), not staged.
 // Squared lengths and N2 are computed in FP64 from the rounded FP32 corners.
 
 #include "wn_device.h"
@@ -27,7 +29,6 @@ constexpr int kStripTileRecords = 255;
 struct StripSmem {
   float4 r0[kStripTileRecords];
   float4 r1[kStripTileRecords];
-  float m[kStripTileRecords];
   double red[6][kBlockThreads / kWave];
 };
 
@@ -60,34 +61,34 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   sm.r1[j] = make_float4((float)(d2x * d2x + d2y * d2y + d2z * d2z),
                          (float)(f2 * (ey * d2z - ez * d2y)), (float)(f2 * (ez * d2x - ex * d2z)),
                          (float)(f2 * (ex * d2y - ey * d2x)));
-  sm.m[j] = (float)c.w;
 }
 
 // FP64 re-evaluation of face (j-2, j-1, j) from the LDS corners, oriented.
-__device__ __noinline__ double strip_pair_f64(const StripSmem& sm, int j, float ox, float oy,
-                                              float oz) {
+__device__ __noinline__ double strip_pair_f64(const StripSmem& sm, int j, bool flip, float ox,
+                                              float oy, float oz) {
   const float4 a = sm.r0[j - 2], b = sm.r0[j - 1], c = sm.r0[j];
   const double qx = ox, qy = oy, qz = oz;
   const double v = pair_f64((double)a.x - qx, (double)a.y - qy, (double)a.z - qz,
                             (double)b.x - qx, (double)b.y - qy, (double)b.z - qz,
                             (double)c.x - qx, (double)c.y - qy, (double)c.z - qz);
-  return sm.m[j] > 0.0f ? v : -v;
+  return flip ? -v : v;
 }
 
 // flag_bits: the record's flag as raw bits (0 = no face ends here), read
 // from global memory at a wave-uniform address, so it arrives in SGPRs and
 // the emit test below is a scalar branch.
-__device__ __forceinline__ void strip_step(const StripSmem& sm, int j, long long flag_bits,
-                                           VSlot& P, VSlot& Q, VSlot& R, float& s_carry,
-                                           double& acc, float ox, float oy, float oz,
-                                           float kappa) {
-  const float4 r0 = sm.r0[j];
+// boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
+__device__ __forceinline__ void strip_step(const StripSmem& sm, int j, int boff,
+                                           long long flag_bits, VSlot& P, VSlot& Q, VSlot& R,
+                                           float& s_carry, double& acc, float ox, float oy,
+                                           float oz, float kappa) {
+  const float4 r0 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r0) + boff);
   R.ax = r0.x - ox;
   R.ay = r0.y - oy;
   R.az = r0.z - oz;
   R.l2 = __builtin_fmaf(R.az, R.az, __builtin_fmaf(R.ay, R.ay, R.ax * R.ax));
   R.l = __builtin_amdgcn_sqrtf(R.l2);
-  const float4 r1 = sm.r1[j];
+  const float4 r1 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r1) + boff);
   const float s_qr = (Q.l2 + R.l2) - r0.w;  // 2 b.c
   const float s_pr = (P.l2 + R.l2) - r1.x;  // 2 c.a
   if (flag_bits != 0) {
@@ -99,18 +100,17 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, long long
     const float pq = P.l * Q.l;
     const float dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
     const float den2 = __builtin_fmaf(pq + pq, R.l, dp);
-    float ang;
-    bool risky = false;
-    if (wave_all_far_x4(det8, den2, dp)) {
-      ang = atan_small_x4_f32(det8 * __builtin_amdgcn_rcpf(den2));
+    double a64;
+    if (__builtin_expect(wave_all_far_x4(det8, den2, dp), 1)) {
+      a64 = (double)atan_small_x4_f32(det8 * __builtin_amdgcn_rcpf(den2));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       float mx;
-      ang = atan2_f32(0.25f * det8, den2, mx);
-      risky = mx < kappa * ((pq + pq) * R.l);  // kappa x 2 la lb lc
+      a64 = (double)atan2_f32(0.25f * det8, den2, mx);
+      // flag -1.0 has the sign bit set: the face runs against the strip order
+      if (mx < kappa * ((pq + pq) * R.l))  // kappa x 2 la lb lc: ill-conditioned
+        a64 = strip_pair_f64(sm, j, flag_bits < 0, ox, oy, oz);
     }
-    double a64 = (double)ang;
-    if (risky) a64 = strip_pair_f64(sm, j, ox, oy, oz);
     acc += a64;
   }
   s_carry = s_qr;
@@ -177,9 +177,16 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     __syncthreads();
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     for (int j = 0; j < kStripTileRecords; j += 3) {
-      strip_step(sm, j, flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz, kappa);
-      strip_step(sm, j + 1, flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz, kappa);
-      strip_step(sm, j + 2, flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz, kappa);
+      // The record offset is wave-uniform, but an LDS read takes its address
+      // from a VGPR: one VGPR copy per three records, read with immediate
+      // offsets, instead of a copy from the scalar loop counter per read.
+      int boff = j * (int)sizeof(float4);
+      asm volatile("" : "+v"(boff));
+      strip_step(sm, j, boff, flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz, kappa);
+      strip_step(sm, j + 1, boff + 16, flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz,
+                 kappa);
+      strip_step(sm, j + 2, boff + 32, flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz,
+                 kappa);
     }
   }
   __syncthreads();
