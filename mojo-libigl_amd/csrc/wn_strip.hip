@@ -3,7 +3,11 @@
 //
 // Same arithmetic as fwd_f32c_kernel (block-local origin, FP32 pairs, FP64
 // accumulation, far-pair fast path, FP64 re-evaluation of ill-conditioned
-// pairs), but faces arrive as a strip: record k brings vertex w_k and closes
+// pairs), except that the angles of three consecutive records are summed in
+// FP32 before each FP64 add (error at most 2 eps times the three angles'
+// magnitudes, the bound of a compensated sum, with a third of the FP64
+// instructions), and the re-evaluated angle is rounded to FP32 once before
+// that sum. Faces arrive as a strip: record k brings vertex w_k and closes
 // face (w_{k-2}, w_{k-1}, w_k). Per face the lane computes one offset, one
 // squared length and one square root (the other two vertices are kept in
 // registers from the previous records) and two law-of-cosines dot products
@@ -78,9 +82,11 @@ __device__ __noinline__ double strip_pair_f64(const StripSmem& sm, int j, bool f
 // from global memory at a wave-uniform address, so it arrives in SGPRs and
 // the emit test below is a scalar branch.
 // boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
+// The face's angle is added to part, the FP32 sum of at most three angles
+// that the caller adds to the FP64 sum.
 __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, int boff,
                                            long long flag_bits, VSlot& P, VSlot& Q, VSlot& R,
-                                           float& s_carry, double& acc, float ox, float oy,
+                                           float& s_carry, float& part, float ox, float oy,
                                            float oz, float kappa) {
   const float4 r0 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r0) + boff);
   R.ax = r0.x - ox;
@@ -100,18 +106,19 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, int boff,
     const float pq = P.l * Q.l;
     const float dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
     const float den2 = __builtin_fmaf(pq + pq, R.l, dp);
-    double a64;
     if (__builtin_expect(wave_all_far_x4(det8, den2, dp), 1)) {
-      a64 = (double)atan_small_x4_f32(det8 * __builtin_amdgcn_rcpf(den2));
+      part += atan_small_x4_f32(det8 * __builtin_amdgcn_rcpf(den2));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       float mx;
-      a64 = (double)atan2_f32(0.25f * det8, den2, mx);
-      // flag -1.0 has the sign bit set: the face runs against the strip order
-      if (mx < kappa * ((pq + pq) * R.l))  // kappa x 2 la lb lc: ill-conditioned
-        a64 = strip_pair_f64(sm, j, flag_bits < 0, ox, oy, oz);
+      float ang = atan2_f32(0.25f * det8, den2, mx);
+      // ill-conditioned (mx < kappa x 2 la lb lc): the FP64 angle, rounded
+      // once, replaces the FP32 one. Flag -1.0 has the sign bit set: the
+      // face runs against the strip order.
+      if (mx < kappa * ((pq + pq) * R.l))
+        ang = (float)strip_pair_f64(sm, j, flag_bits < 0, ox, oy, oz);
+      part += ang;
     }
-    acc += a64;
   }
   s_carry = s_qr;
 }
@@ -182,11 +189,15 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
       // offsets, instead of a copy from the scalar loop counter per read.
       int boff = j * (int)sizeof(float4);
       asm volatile("" : "+v"(boff));
-      strip_step(sm, j, boff, flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz, kappa);
-      strip_step(sm, j + 1, boff + 16, flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz,
+      // the three records' FP32 angles are summed in FP32 (at most three
+      // terms, each |angle| <= pi) and added to the FP64 sum once
+      float part = 0.0f;
+      strip_step(sm, j, boff, flags[4 * j], A, B, C, s_carry, part, ox, oy, oz, kappa);
+      strip_step(sm, j + 1, boff + 16, flags[4 * (j + 1)], B, C, A, s_carry, part, ox, oy, oz,
                  kappa);
-      strip_step(sm, j + 2, boff + 32, flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz,
+      strip_step(sm, j + 2, boff + 32, flags[4 * (j + 2)], C, A, B, s_carry, part, ox, oy, oz,
                  kappa);
+      acc += (double)part;
     }
   }
   __syncthreads();
