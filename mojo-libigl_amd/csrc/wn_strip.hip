@@ -191,7 +191,7 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
       asm volatile("" : "+v"(boff));
       // the three records' FP32 angles are summed in FP32 (at most three
       // terms, each |angle| <= pi) and added to the FP64 sum once
-      float part = 0.0f;
+      float part = -0.0f;  // -0 + x == x for every x: the first add folds away
       strip_step(sm, j, boff, flags[4 * j], A, B, C, s_carry, part, ox, oy, oz, kappa);
       strip_step(sm, j + 1, boff + 16, flags[4 * (j + 1)], B, C, A, s_carry, part, ox, oy, oz,
                  kappa);
