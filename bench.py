@@ -194,9 +194,10 @@ def roofline(pairs, ms, peak=FP32_VECTOR_PEAK_TFLOPS, kernel=None):
 
 
 class MeshForward:
-    """wn_mesh_forward (strip kernel) on a resident mesh handle."""
+    """wn_mesh_forward (FP32c strip kernel) or wn_mesh_forward64 (FP64 strip
+    kernel) on a resident mesh handle."""
 
-    def __init__(self, h, mesh, Qd, splits=0):
+    def __init__(self, h, mesh, Qd, splits=0, fp64=False):
         import torch
 
         import igl
@@ -209,13 +210,19 @@ class MeshForward:
         self.ws = torch.empty(max(self.nbytes, 1), dtype=torch.uint8, device=Qd.device)
         self.out = torch.empty(self.nq, dtype=torch.float64, device=Qd.device)
         self.stream = torch.cuda.current_stream()
+        self.fp64 = fp64
 
     def __call__(self):
         import igl
 
-        st = self.h.wn_mesh_forward(self.mesh.handle, self.Qd.data_ptr(), self.out.data_ptr(),
-                                    self.nq, self.splits, self.sort, -1.0, self.ws.data_ptr(),
-                                    self.nbytes, self.stream.cuda_stream)
+        if self.fp64:
+            st = self.h.wn_mesh_forward64(self.mesh.handle, self.Qd.data_ptr(), self.out.data_ptr(),
+                                          self.nq, self.splits, self.sort, self.ws.data_ptr(),
+                                          self.nbytes, self.stream.cuda_stream)
+        else:
+            st = self.h.wn_mesh_forward(self.mesh.handle, self.Qd.data_ptr(), self.out.data_ptr(),
+                                        self.nq, self.splits, self.sort, -1.0, self.ws.data_ptr(),
+                                        self.nbytes, self.stream.cuda_stream)
         if st != 0:
             raise RuntimeError(igl._lib.last_error())
 
@@ -269,6 +276,16 @@ def run_torus(args, h):
         ref = oracle.winding_soa(tri, Q[sub])
         acc["max_abs_dw_fp64_vs_oracle_sub512"] = float(np.abs(w64[sub] - ref).max())
         acc["max_abs_dw_fp32c_vs_oracle_sub512"] = float(np.abs(w_strip[sub] - ref).max())
+        s64 = MeshForward(h, mesh, Qd, args.splits, fp64=True)
+        ms_s64, _ = timed(s64, args.steps, args.warmup)
+        log(f"fp64 strip {ms_s64:.3f} ms/step (device)")
+        w_s64 = s64.out.cpu().numpy()
+        acc["max_abs_dw_fp64_strip_vs_fp64"] = float(np.abs(w_s64 - w64).max())
+        acc["max_abs_dw_fp64_strip_vs_oracle_sub512"] = float(np.abs(w_s64[sub] - ref).max())
+        result_fp64["strip"] = {"value": nf * nq / (ms_s64 * 1e-3) / 1e9, "ms_per_step": ms_s64,
+                                "speedup_vs_fp64_plain": ms64 / ms_s64,
+                                "roofline_frac_fp64_peak":
+                                    roofline(nf * nq, ms_s64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
     mesh.close()
 
     line = base_line(

@@ -164,6 +164,15 @@ int64_t wn_mesh_forward(int64_t mesh, const double *q, double *dst, int64_t nq,
                         void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Frees a mesh (synchronises the device first). */
+/* The same query in FP64 (no reference counterpart beyond
+ * src/capi.mojo:218-235,286 itself): FP64 over the mesh's strip records, one
+ * square root per face, far pairs from a polynomial accurate to FP64
+ * rounding, every other pair with the plain FP64 formula; agrees with the
+ * plain FP64 kernel to ~1e-15. Workspace as wn_mesh_workspace_bytes. */
+int64_t wn_mesh_forward64(int64_t mesh, const double *q, double *dst, int64_t nq,
+                          int64_t splits, int64_t sort, void *workspace,
+                          int64_t workspace_bytes, void *stream);
+
 int64_t wn_mesh_destroy(int64_t mesh);
 
 /* ---- Batched forward ------------------------------------------------------ */

@@ -224,7 +224,8 @@ struct MeshHandle {
   int64_t nf = 0;
   int64_t nv = 0;
   int64_t nrec = 0;
-  void* d_rec = nullptr;     // one allocation: records, then vertex ids, then flags
+  void* d_rec = nullptr;     // one allocation: records, FP64 sweep records, vertex ids, flags
+  wn::StripRec64* d_rec64 = nullptr;
   int32_t* d_idx = nullptr;
   int8_t* d_flag = nullptr;
 };
@@ -237,14 +238,16 @@ int64_t mesh_upload(MeshHandle* m, const double* V, const void* F, int fdtype) {
   m->nrec = wn::strip_topology(F, m->nf, fdtype, idx, flag);
   if (m->nrec == 0) return kOk;
   const size_t rec_b = wn::align_up((size_t)m->nrec * 4 * sizeof(double));
+  const size_t rec64_b = wn::align_up((size_t)m->nrec * sizeof(wn::StripRec64));
   const size_t idx_b = wn::align_up((size_t)m->nrec * sizeof(int32_t));
-  hipError_t e = hipMalloc(&m->d_rec, rec_b + idx_b + wn::align_up((size_t)m->nrec));
+  hipError_t e = hipMalloc(&m->d_rec, rec_b + rec64_b + idx_b + wn::align_up((size_t)m->nrec));
   if (e != hipSuccess) {
     m->d_rec = nullptr;
     return fail(kErrHip, hip_msg("wn_mesh_create", e));
   }
-  m->d_idx = (int32_t*)((char*)m->d_rec + rec_b);
-  m->d_flag = (int8_t*)((char*)m->d_rec + rec_b + idx_b);
+  m->d_rec64 = (wn::StripRec64*)((char*)m->d_rec + rec_b);
+  m->d_idx = (int32_t*)((char*)m->d_rec + rec_b + rec64_b);
+  m->d_flag = (int8_t*)((char*)m->d_rec + rec_b + rec64_b + idx_b);
   e = hipMemcpy(m->d_idx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(m->d_flag, flag.data(), flag.size(), hipMemcpyHostToDevice);
@@ -261,6 +264,9 @@ int64_t mesh_upload(MeshHandle* m, const double* V, const void* F, int fdtype) {
   } else if (e == hipSuccess) {
     e = hipMemset(m->d_rec, 0, (size_t)m->nrec * 4 * sizeof(double));
   }
+  // FP64 sweep records of the records (zero records give zero sweep records)
+  if (e == hipSuccess) e = wn::strip64_prep_launch((const double4*)m->d_rec, m->nrec, m->d_rec64, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
   if (e != hipSuccess) {
     (void)hipFree(m->d_rec);
     m->d_rec = nullptr;
@@ -645,6 +651,8 @@ int64_t wn_mesh_set_vertices(int64_t mesh, const void* V, int64_t nv, int64_t vd
   if (!V) return fail(kErrArg, "null device buffer");
   hipError_t e = wn::strip_gather_launch(V, (int)vdtype, nv, m->d_idx, m->d_flag, m->nrec,
                                          (double4*)m->d_rec, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = wn::strip64_prep_launch((const double4*)m->d_rec, m->nrec, m->d_rec64, (hipStream_t)stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_set_vertices", e));
   return kOk;
 }
@@ -703,6 +711,25 @@ int64_t wn_mesh_forward(int64_t mesh, const double* q, double* dst, int64_t nq, 
                                   (int)cfg[1], kappa < 0.0 ? 1.0f : (float)kappa, workspace,
                                   (size_t)workspace_bytes, (hipStream_t)stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_forward", e));
+  return kOk;
+}
+
+int64_t wn_mesh_forward64(int64_t mesh, const double* q, double* dst, int64_t nq,
+                          int64_t splits, int64_t sort, void* workspace, int64_t workspace_bytes,
+                          void* stream) {
+  int64_t cfg[2];
+  const int64_t st = wn_mesh_resolve(mesh, nq, splits, sort, cfg);
+  if (st != kOk) return st;
+  if (nq == 0) return kOk;
+  if (!q || !dst) return fail(kErrArg, "null device buffer");
+  MeshHandle* m = lookup_mesh(mesh);
+  const size_t need = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+  if (workspace_bytes < 0 || (size_t)workspace_bytes < need || (need > 0 && !workspace))
+    return fail(kErrWorkspace, "workspace smaller than wn_mesh_workspace_bytes()");
+  hipError_t e = wn::strip64_launch(m->d_rec64, (const double4*)m->d_rec, m->nrec, q, dst, nq,
+                                    (int)cfg[0], (int)cfg[1], workspace, (size_t)workspace_bytes,
+                                    (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_forward64", e));
   return kOk;
 }
 

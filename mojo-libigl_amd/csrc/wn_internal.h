@@ -16,6 +16,15 @@ enum Precision : int { kFP64 = 0, kFP32C = 1 };
 
 constexpr int kBlockThreads = 256;  // 4 waves
 constexpr int kFaceTile = 256;      // faces staged in LDS per tile (one per thread)
+constexpr int kStripTileRecords = 255;  // strip records per tile (strips cut at tile ends)
+
+// FP64 sweep record of strip record k (wn_strip64.hip), built on the device
+// from the records: its vertex, the squared lengths to the previous two
+// vertices of its tile, and 8 x the normal of the face it closes, oriented
+// by its flag (zero when it closes none).
+struct alignas(64) StripRec64 {
+  double x, y, z, e1, e2, nx, ny, nz;
+};
 
 struct FwdConfig {
   int precision = kFP64;  // kFP64: plain FP64; kFP32C: FP32 + FP64 accumulation/fallback
@@ -48,6 +57,13 @@ hipError_t strip_gather_launch(const void* V, int vdtype, int64_t nv, const int3
                                hipStream_t stream);
 int strip_resident_blocks();
 size_t strip_workspace_bytes(int64_t nq, int splits, int sort);
+// FP64 strip forward (wn_strip64.hip): sweep records from the records, and
+// the launch (same workspace layout and size as strip_launch).
+hipError_t strip64_prep_launch(const double4* rec, int64_t nrec, StripRec64* out,
+                               hipStream_t stream);
+hipError_t strip64_launch(const StripRec64* rec64, const double4* rec, int64_t nrec,
+                          const double* q, double* dst, int64_t nq, int splits, int sort,
+                          void* ws, size_t ws_bytes, hipStream_t stream);
 hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, double* dst,
                         int64_t nq, int splits, int sort, float kappa, void* ws, size_t ws_bytes,
                         hipStream_t stream);

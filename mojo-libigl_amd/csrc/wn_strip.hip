@@ -28,8 +28,6 @@
 
 namespace wn {
 
-constexpr int kStripTileRecords = 255;
-
 struct StripSmem {
   float4 r0[kStripTileRecords];
   float4 r1[kStripTileRecords];

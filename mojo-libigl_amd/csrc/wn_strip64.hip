@@ -1,0 +1,231 @@
+// FP64 forward over triangle-strip records: the winding number of
+// src/capi.mojo:218-235,286 in FP64 at the strip kernel's per-face cost.
+//
+// Per face the lane computes one vertex offset, one squared length and one
+// square root (the other two vertices are kept from the previous records),
+// the doubled dot products from the law of cosines with the squared edge
+// lengths of the sweep record, and det = N . c with the face normal of the
+// sweep record (exact: N . (a - c) = 0). A wave whose 64 pairs are all far
+// (|det| < den / 4 and den > 2 la lb lc, the FP32 strip kernel's test) takes
+// the angle from a 9-term Remez polynomial of det / den (relative error
+// 4.5e-18 on |x| <= 1/4, below FP64 rounding; gen_atan_coeffs.py, n = 9);
+// any other wave evaluates every lane's pair with the plain FP64 formula
+// (pair_f64, the plain kernel's arithmetic). Square roots and the quotient
+// are refined from the hardware estimates to FP64 rounding (Goldschmidt for
+// the root, one Newton step and a remainder correction for the quotient).
+//
+// The sweep record is read at a wave-uniform address, so it arrives in
+// scalar registers through scalar loads: no LDS, no tile staging, no
+// barriers -- every wave streams the records on its own. Queries are used as
+// given (no block-local origin: FP64 offsets V - O are the plain kernel's
+// own). Splits, Morton order and the split reduction are strip_launch's.
+
+#include "wn_device.h"
+#include "wn_internal.h"
+
+namespace wn {
+
+namespace {
+
+// Builds the sweep records of one tile-aligned record array. Neighbours are
+// taken inside the record's tile (the first two records of a tile close no
+// face: strips are cut at tile ends), as the FP32 staging does.
+__global__ __launch_bounds__(kBlockThreads) void strip64_prep_kernel(
+    const double4* __restrict__ rec, int64_t nrec, StripRec64* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (k >= nrec) return;
+  const int64_t j = k % kStripTileRecords;
+  const int64_t base = k - j;
+  const double4 c = rec[k];
+  const double4 p = rec[base + (j >= 1 ? j - 1 : j)];
+  const double4 pp = rec[base + (j >= 2 ? j - 2 : j)];
+  const double d1x = c.x - p.x, d1y = c.y - p.y, d1z = c.z - p.z;
+  const double d2x = c.x - pp.x, d2y = c.y - pp.y, d2z = c.z - pp.z;
+  const double ex = p.x - pp.x, ey = p.y - pp.y, ez = p.z - pp.z;
+  const double f8 = 8.0 * c.w;  // orientation flag (0 = no face), times 8
+  StripRec64 r;
+  r.x = c.x;
+  r.y = c.y;
+  r.z = c.z;
+  r.e1 = d1x * d1x + d1y * d1y + d1z * d1z;
+  r.e2 = d2x * d2x + d2y * d2y + d2z * d2z;
+  r.nx = f8 * (ey * d2z - ez * d2y);
+  r.ny = f8 * (ez * d2x - ex * d2z);
+  r.nz = f8 * (ex * d2y - ey * d2x);
+  out[k] = r;
+}
+
+struct Slot64 {
+  double ax, ay, az, l2, l;
+};
+
+// sqrt(x) for x > 0 finite: Goldschmidt iteration from v_rsq_f64 (two
+// steps: the estimate's ~2^-23 relative error squares twice). x == 0 gives
+// NaN (0 * inf), which fails the far test and sends the wave to pair_f64.
+__device__ __forceinline__ double sqrt_gs_f64(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  double g = x * r, h = 0.5 * r;
+  double e = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  e = __builtin_fma(-g, h, 0.5);
+  return __builtin_fma(g, e, g);
+}
+
+// n / d for d > 0 finite: v_rcp_f64, one Newton step, then the quotient
+// corrected by its remainder.
+__device__ __forceinline__ double div_f64(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+  const double q = n * r;
+  return __builtin_fma(__builtin_fma(-d, q, n), r, q);
+}
+
+// c + a * b with c in scalar registers (a wave-uniform constant): the VOP3
+// v_fma_f64 form. Without it the compiler keeps the polynomial's constants
+// in VGPRs and copies one into the destination of a v_fmac_f64 per step.
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+}
+
+// atan(x / 4) for |x| <= 1: x * P(x^2) with the n = 9 relative-error
+// minimax coefficients of atan on (0, 1/4], each divided by 4^(2k+1)
+// (powers of two: exact).
+__device__ __forceinline__ double atan_small_x4_f64(double x) {
+  const double t = x * x;
+  double p = 0.0459998373611164223 / 17179869184.0;                   // x^17: 4^17
+  p = fma_sc(p, t, -0.065237123716060834312 / 1073741824.0);   // x^15: 4^15
+  p = fma_sc(p, t, 0.076834985777981783012 / 67108864.0);      // x^13: 4^13
+  p = fma_sc(p, t, -0.090905895478190080677 / 4194304.0);      // x^11: 4^11
+  p = fma_sc(p, t, 0.11111104268448408897 / 262144.0);         // x^9: 4^9
+  p = fma_sc(p, t, -0.14285714203175044227 / 16384.0);         // x^7: 4^7
+  p = fma_sc(p, t, 0.19999999999500416733 / 1024.0);           // x^5: 4^5
+  p = fma_sc(p, t, -0.33333333333332167915 / 64.0);            // x^3: 4^3
+  p = fma_sc(p, t, 0.99999999999999999552 / 4.0);              // x^1: 4
+  return x * p;
+}
+
+// The plain FP64 pair, oriented; out of line so that its registers are not
+// reserved for the whole sweep.
+__device__ __noinline__ double strip64_pair(double ax, double ay, double az, double bx,
+                                            double by, double bz, double cx, double cy,
+                                            double cz, bool flip) {
+  const double v = pair_f64(ax, ay, az, bx, by, bz, cx, cy, cz);
+  return flip ? -v : v;
+}
+
+__device__ __forceinline__ bool wave_all_far_x4_f64(double det8, double den2, double dp) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabs(det8) < den2);
+  const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp > 0.0);
+  return (m1 & m2) == __builtin_amdgcn_read_exec();
+}
+
+// One record: R takes vertex j; when the record closes face (P, Q, R) its
+// angle joins acc. s_carry is the doubled dot product of the previous
+// record's two newest vertices (Q, R of the last step = P, Q here).
+__device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag_bits,
+                                             const Slot64& P, const Slot64& Q, Slot64& R,
+                                             double& s_carry, double& acc, double ox,
+                                             double oy, double oz) {
+  R.ax = r.x - ox;
+  R.ay = r.y - oy;
+  R.az = r.z - oz;
+  R.l2 = __builtin_fma(R.az, R.az, __builtin_fma(R.ay, R.ay, R.ax * R.ax));
+  R.l = sqrt_gs_f64(R.l2);
+  const double s_qr = (Q.l2 + R.l2) - r.e1;  // 2 b.c
+  if (flag_bits != 0) {
+    const double s_pr = (P.l2 + R.l2) - r.e2;  // 2 c.a
+    const double det8 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
+    const double pq = P.l * Q.l;
+    const double dp = __builtin_fma(R.l, s_carry, __builtin_fma(P.l, s_qr, Q.l * s_pr));
+    const double den2 = __builtin_fma(pq + pq, R.l, dp);
+    if (__builtin_expect(wave_all_far_x4_f64(det8, den2, dp), 1)) {
+      acc += atan_small_x4_f64(div_f64(det8, den2));
+    } else {
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      // flag -1.0 has the sign bit set: the face runs against the strip order
+      acc += strip64_pair(P.ax, P.ay, P.az, Q.ax, Q.ay, Q.az, R.ax, R.ay, R.az, flag_bits < 0);
+    }
+  }
+  s_carry = s_qr;
+}
+
+// 6 waves per SIMD: 80 VGPRs, no spills
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6))) void fwd_strip64_kernel(
+    const StripRec64* __restrict__ rec64, const double4* __restrict__ rec, int64_t ntiles,
+    const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ out,
+    int64_t nq, int64_t tiles_per_split, int splits) {
+  const int64_t slot = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  const bool valid = slot < nq;
+  // q is in launch order (gathered through perm beforehand when sorted)
+  const double ox = valid ? q[3 * slot] : 0.0;
+  const double oy = valid ? q[3 * slot + 1] : 0.0;
+  const double oz = valid ? q[3 * slot + 2] : 0.0;
+  const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
+  const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
+  double acc = 0.0, s_carry = 0.0;
+  Slot64 A{0.0, 0.0, 0.0, 0.0, 0.0}, B = A, C = A;
+  for (int64_t t = t_begin; t < t_end; ++t) {
+    const StripRec64* r = rec64 + t * kStripTileRecords;
+    const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
+    for (int j = 0; j < kStripTileRecords; j += 3) {
+      strip64_step(r[j], flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz);
+      strip64_step(r[j + 1], flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz);
+      strip64_step(r[j + 2], flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz);
+    }
+  }
+  if (valid) {
+    if (splits == 1)
+      out[perm ? (int64_t)perm[slot] : slot] = acc * kInv2Pi;
+    else
+      out[(int64_t)blockIdx.y * nq + slot] = acc;
+  }
+}
+
+}  // namespace
+
+hipError_t strip64_prep_launch(const double4* rec, int64_t nrec, StripRec64* out,
+                               hipStream_t stream) {
+  if (nrec == 0) return hipSuccess;
+  strip64_prep_kernel<<<(unsigned)((nrec + kBlockThreads - 1) / kBlockThreads), kBlockThreads,
+                        0, stream>>>(rec, nrec, out);
+  return hipGetLastError();
+}
+
+hipError_t strip64_launch(const StripRec64* rec64, const double4* rec, int64_t nrec,
+                          const double* q, double* dst, int64_t nq, int splits, int sort,
+                          void* ws, size_t ws_bytes, hipStream_t stream) {
+  if (nq == 0) return hipSuccess;
+  if (ws_bytes < strip_workspace_bytes(nq, splits, sort)) return hipErrorInvalidValue;
+  const int64_t ntiles = nrec / kStripTileRecords;
+  char* cursor = (char*)ws;
+  double* partial = nullptr;
+  if (splits > 1) {
+    partial = (double*)cursor;
+    cursor += align_up((size_t)nq * (size_t)splits * sizeof(double));
+  }
+  int32_t* perm = nullptr;
+  if (sort) {
+    perm = (int32_t*)cursor;
+    cursor += align_up((size_t)nq * sizeof(int32_t));
+    double* qs = (double*)cursor;
+    cursor += align_up((size_t)nq * 3 * sizeof(double));
+    hipError_t e = morton_order(q, nq, perm, cursor, morton_workspace_bytes(nq), stream);
+    if (e == hipSuccess) e = gather_queries_launch(q, perm, nq, qs, stream);
+    if (e != hipSuccess) return e;
+    q = qs;
+  }
+  if (ntiles == 0) return fill_launch(dst, nq, 0.0, stream);
+  const int64_t tps = (ntiles + splits - 1) / splits;
+  const int used_splits = (int)((ntiles + tps - 1) / tps);
+  const dim3 grid((unsigned)((nq + kBlockThreads - 1) / kBlockThreads), (unsigned)used_splits);
+  fwd_strip64_kernel<<<grid, kBlockThreads, 0, stream>>>(
+      rec64, rec, ntiles, q, perm, used_splits > 1 ? partial : dst, nq, tps, used_splits);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || used_splits == 1) return e;
+  return reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
+}
+
+}  // namespace wn

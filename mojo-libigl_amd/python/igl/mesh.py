@@ -101,11 +101,16 @@ class Mesh:
             raise RuntimeError(last_error())
         return int(out[0]), int(out[1])
 
-    def winding_number(self, O, splits=0, sort=-1, kappa=-1.0, out=None):
+    def winding_number(self, O, splits=0, sort=-1, kappa=-1.0, out=None, precision="fp32c"):
         """w at the rows of O. O: (q, 3) array-like (result numpy float64) or
-        a CUDA tensor (result a float64 CUDA tensor on the current stream)."""
+        a CUDA tensor (result a float64 CUDA tensor on the current stream).
+        precision "fp32c" (FP32-compensated strip kernel) or "fp64" (FP64
+        strip kernel, wn_mesh_forward64; kappa does not apply)."""
         import torch
 
+        precision = str(precision).lower()
+        if precision not in ("fp32c", "fp64"):
+            raise ValueError("precision must be 'fp32c' or 'fp64'")
         host = not (isinstance(O, torch.Tensor) and O.is_cuda)
         Q = torch.as_tensor(np.asarray(O, dtype=np.float64).reshape(-1, 3) if host else O)
         Q = Q.to(device="cuda", dtype=torch.float64).contiguous()
@@ -116,9 +121,13 @@ class Mesh:
             raise RuntimeError(last_error())
         ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=Q.device)
         w = out if out is not None else torch.empty(nq, dtype=torch.float64, device=Q.device)
-        st = lib().wn_mesh_forward(self.handle, Q.data_ptr(), w.data_ptr(), nq, splits, sort,
-                                   float(kappa), ws.data_ptr(), nbytes,
-                                   torch.cuda.current_stream().cuda_stream)
+        stream = torch.cuda.current_stream().cuda_stream
+        if precision == "fp64":
+            st = lib().wn_mesh_forward64(self.handle, Q.data_ptr(), w.data_ptr(), nq, splits, sort,
+                                         ws.data_ptr(), nbytes, stream)
+        else:
+            st = lib().wn_mesh_forward(self.handle, Q.data_ptr(), w.data_ptr(), nq, splits, sort,
+                                       float(kappa), ws.data_ptr(), nbytes, stream)
         if st != 0:
             raise RuntimeError(f"wn_mesh_forward failed on the GPU: {last_error()}")
         return w.cpu().numpy() if host else w
