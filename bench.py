@@ -286,6 +286,15 @@ def run_torus(args, h):
                                 "speedup_vs_fp64_plain": ms64 / ms_s64,
                                 "roofline_frac_fp64_peak":
                                     roofline(nf * nq, ms_s64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
+        # the drop-in call from host arrays, end to end (large calls take the
+        # strip route: host stripification, upload, FP64 strip kernel, download)
+        e2e = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            w_e2e = igl.winding_number(V, F, Q, device="gpu")
+            e2e.append((time.perf_counter() - t0) * 1e3)
+        result_fp64["host_call_end_to_end_ms"] = min(e2e)
+        acc["max_abs_dw_host_call_vs_fp64"] = float(np.abs(w_e2e - w64).max())
     mesh.close()
 
     line = base_line(

@@ -175,6 +175,16 @@ int64_t wn_mesh_forward64(int64_t mesh, const double *q, double *dst, int64_t nq
 
 int64_t wn_mesh_destroy(int64_t mesh);
 
+/* Host-buffer strip path (no reference counterpart; the reference's GPU entry
+ * is mli_winding_gpu above): host V (nv x 3 float64) and F (nf x 3, fdtype 0
+ * = int64, 1 = int32) are stripified and uploaded, (host) q (nq x 3) runs
+ * through the FP64 (precision 0) or FP32-compensated (1) strip kernel into
+ * (host) dst (nq). Synchronous; allocates and frees its own device memory.
+ * Returns 0 or a negative status. */
+int64_t wn_winding_gpu_mesh(const double *V, const void *F, int64_t nv, int64_t nf,
+                            int64_t fdtype, const double *q, double *dst, int64_t nq,
+                            int64_t precision);
+
 /* ---- Batched forward ------------------------------------------------------ */
 
 /* No reference counterpart (the reference evaluates one mesh per call,

@@ -750,6 +750,78 @@ int64_t wn_mesh_destroy(int64_t mesh) {
   return kOk;
 }
 
+// Host-buffer strip path: a temporary mesh (host stripifier, upload), then
+// the host path's stream and grow-only buffer for queries, output and
+// workspace. Same contract as wn_winding_gpu_ex: synchronous, returns 0 or a
+// negative status.
+int64_t wn_winding_gpu_mesh(const double* V, const void* F, int64_t nv, int64_t nf,
+                            int64_t fdtype, const double* q, double* dst, int64_t nq,
+                            int64_t precision) {
+  if (precision != wn::kFP64 && precision != wn::kFP32C)
+    return fail(kErrArg, "precision must be 0 (FP64) or 1 (FP32-compensated)");
+  if (nq < 0) return fail(kErrArg, "negative size");
+  if (nq == 0) return kOk;
+  if (!q || !dst) return fail(kErrArg, "null query or output buffer");
+  int64_t cfg[2];
+  const int64_t mesh = wn_mesh_create(V, F, nv, nf, fdtype);
+  if (mesh < 0) return mesh;
+  int64_t st = wn_mesh_resolve(mesh, nq, 0, -1, cfg);
+  if (st == kOk) {
+    MeshHandle* m = lookup_mesh(mesh);
+    HostPathContext& ctx = host_ctx();
+    std::lock_guard<std::mutex> lock(ctx.mu);
+    hipError_t e = hipSuccess;
+    if (!ctx.ready) {
+      e = hipGetDevice(&ctx.device);
+      if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking);
+      if (e == hipSuccess) ctx.ready = true;
+    }
+    const size_t q_b = wn::align_up((size_t)nq * 3 * sizeof(double));
+    const size_t dst_b = wn::align_up((size_t)nq * sizeof(double));
+    const size_t ws_b = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+    const size_t need = q_b + dst_b + ws_b;
+    if (e == hipSuccess && need > ctx.cap) {
+      if (ctx.buf) {
+        (void)hipStreamSynchronize(ctx.stream);
+        (void)hipFree(ctx.buf);
+        ctx.buf = nullptr;
+        ctx.cap = 0;
+      }
+      e = hipMalloc(&ctx.buf, need);
+      if (e == hipSuccess)
+        ctx.cap = need;
+      else
+        ctx.buf = nullptr;
+    }
+    if (e == hipSuccess) {
+      char* base = (char*)ctx.buf;
+      double* d_q = (double*)base;
+      double* d_dst = (double*)(base + q_b);
+      void* d_ws = base + q_b + dst_b;
+      e = hipMemcpyAsync(d_q, q, (size_t)nq * 3 * sizeof(double), hipMemcpyHostToDevice,
+                         ctx.stream);
+      if (e == hipSuccess) {
+        if (precision == wn::kFP64)
+          e = wn::strip64_launch(m->d_rec64, (const double4*)m->d_rec, m->nrec, d_q, d_dst, nq,
+                                 (int)cfg[0], (int)cfg[1], d_ws, ws_b, ctx.stream);
+        else
+          e = wn::strip_launch((const double4*)m->d_rec, m->nrec, d_q, d_dst, nq, (int)cfg[0],
+                               (int)cfg[1], 1.0f, d_ws, ws_b, ctx.stream);
+      }
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(dst, d_dst, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx.stream);
+      const hipError_t es = hipStreamSynchronize(ctx.stream);
+      if (e == hipSuccess) e = es;
+    }
+    if (e != hipSuccess) st = fail(kErrHip, hip_msg("wn_winding_gpu_mesh", e));
+  }
+  const std::string why = st == kOk ? std::string() : g_error;
+  (void)wn_mesh_destroy(mesh);
+  if (st != kOk) set_error(why);
+  return st;
+}
+
 int64_t wn_scatter_corners(const double* grad_tri, const void* F, int64_t fdtype, int64_t nf,
                            int64_t nv, double* grad_V, void* stream) {
   if (nf < 0 || nv < 0) return fail(kErrArg, "negative size");

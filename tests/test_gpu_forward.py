@@ -266,3 +266,30 @@ def test_host_entry_is_safe_under_concurrent_callers(require_gpu):
     for t in threads:
         t.join(timeout=120)
     assert not errors and all(got)
+
+
+def test_large_calls_use_strip_kernels(require_gpu, monkeypatch):
+    """igl.winding_number sends calls with many queries through the strip
+    kernels (wn_winding_gpu_mesh): FP64 agrees with the plain FP64 kernel to
+    1e-12 and with the oracle to the FP64 bar; FP32c stays within its bar;
+    the input contract is unchanged."""
+    from igl import pack_faces
+    from igl._lib import addr, lib
+
+    V, F, Q = W.config2(4096)
+    monkeypatch.setattr(igl, "STRIP_MIN_QUERIES", {0: 1000, 1: 1000})
+    w64 = igl.winding_number(V, F, Q, device="gpu")
+    w32 = igl.winding_number(V, F, Q, device="gpu", precision="fp32c")
+    tri = pack_faces(V, F)
+    plain = np.empty(len(Q))
+    assert lib().mli_winding_gpu(addr(tri), addr(Q), addr(plain), tri.shape[1], len(Q)) == 1
+    print(f"strip route: fp64 vs plain {np.abs(w64 - plain).max():.3e}, "
+          f"fp32c vs plain {np.abs(w32 - plain).max():.3e}", flush=True)
+    assert np.abs(w64 - plain).max() <= 1e-12
+    assert np.abs(w32 - plain).max() <= FP32C_TOL
+    sub = np.arange(0, len(Q), 17)
+    assert np.abs(w64[sub] - oracle.winding(V, F, Q[sub])).max() <= FP64_TOL
+    with pytest.raises(IndexError):
+        igl.winding_number(V, F + len(V), Q, device="gpu")
+    F32 = F.astype(np.int32)
+    assert np.array_equal(igl.winding_number(V, F32, Q, device="gpu"), w64)
