@@ -201,13 +201,14 @@ def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision=
 
 
 class _StripWinding(torch.autograd.Function):
-    """Forward on a resident strip mesh (FP32-compensated), FP64 adjoint."""
+    """Forward on a resident strip mesh (FP32-compensated or FP64), FP64
+    adjoint."""
 
     @staticmethod
     def forward(ctx, V, O, field):
         field.mesh.set_vertices(V)
         Q = O.to(torch.float64).contiguous()
-        w = field.mesh.winding_number(Q)
+        w = field.mesh.winding_number(Q, precision=field.precision)
         ctx.save_for_backward(V, Q)
         ctx.field = field
         ctx.odtype = O.dtype
@@ -233,18 +234,22 @@ class WindingField:
     """Differentiable winding number of a fixed-topology mesh whose vertices
     change between calls (an optimisation loop): the strip topology and the
     boundary edges are built once on the host, every call gathers the strip
-    records from V on the device and runs the FP32-compensated strip kernel;
-    backward is the FP64 adjoint (as winding_number's, "edges" by default).
+    records from V on the device and runs the strip kernel (FP32-compensated,
+    or FP64 with precision="fp64"); backward is the FP64 adjoint (as
+    winding_number's, "edges" by default).
 
         field = WindingField(F, n_vertices)
         w = field(V, O)            # V (nv, 3), O (q, 3) CUDA tensors
     """
 
-    def __init__(self, F, n_vertices: int, adjoint="edges"):
+    def __init__(self, F, n_vertices: int, adjoint="edges", precision="fp32c"):
         from .mesh import Mesh
 
         if adjoint not in ADJOINTS:
             raise ValueError(f"adjoint must be one of {ADJOINTS}")
+        if str(precision).lower() not in ("fp32c", "fp64"):
+            raise ValueError("precision must be 'fp32c' or 'fp64'")
+        self.precision = str(precision).lower()
         Fh = F.detach().cpu().numpy() if isinstance(F, torch.Tensor) else np.asarray(F)
         self.mesh = Mesh.from_faces(Fh, int(n_vertices))
         self.F = torch.as_tensor(np.ascontiguousarray(Fh, dtype=np.int64), device="cuda")

@@ -158,3 +158,31 @@ def test_winding_field_forward_and_gradients(require_gpu):
     # dL/dw differs by 2 (w1 - w0) <= 2e-6 per query, so gradients agree to that scale
     assert np.abs(gv1 - gv0).max() <= 1e-4 * max(1.0, np.abs(gv0).max())
     assert np.abs(go1 - go0).max() <= 1e-4 * max(1.0, np.abs(go0).max())
+
+
+def test_winding_field_fp64_forward(require_gpu):
+    """WindingField(precision="fp64"): the FP64 strip forward, within the FP64
+    bar of the oracle, with the same (edge) adjoint as the default field."""
+    import igl.autograd as A
+
+    V, F = W.icosphere(3)
+    F = F[: len(F) - 40]  # an open surface: nonzero gradients
+    Q = np.random.default_rng(5).uniform(-1.4, 1.4, size=(2000, 3))
+    out = []
+    for precision in ("fp64", "fp32c"):
+        field = A.WindingField(F, len(V), precision=precision)
+        Vd = torch.tensor(V, device="cuda", requires_grad=True)
+        Od = torch.tensor(Q, device="cuda", requires_grad=True)
+        w = field(Vd, Od)
+        w.sum().backward()
+        out.append((w.detach().cpu().numpy(), Vd.grad.cpu().numpy(), Od.grad.cpu().numpy()))
+        field.close()
+    (w64, gv64, go64), (w32, gv32, go32) = out
+    assert np.abs(w64 - oracle.winding(V, F, Q)).max() <= 1e-10
+    assert np.abs(w32 - w64).max() <= 1e-5
+    # dL/dw = 1 in both: the same adjoint call (grad_V summed with atomics,
+    # so equal to rounding)
+    assert np.abs(gv64 - gv32).max() <= 1e-12 * np.abs(gv64).max()
+    assert np.abs(go64 - go32).max() <= 1e-12 * np.abs(go64).max()
+    with pytest.raises(ValueError):
+        A.WindingField(F, len(V), precision="fp16")
