@@ -13,6 +13,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "wn_radix.h"
+
 namespace wn {
 
 int64_t boundary_edges(const void* F, int64_t nf, int fdtype, std::vector<int32_t>& edges,
@@ -35,7 +37,7 @@ int64_t boundary_edges(const void* F, int64_t nf, int fdtype, std::vector<int32_
       keys.push_back(((lo << 32 | hi) << 1) | (s > t ? 1u : 0u));
     }
   }
-  std::sort(keys.begin(), keys.end());
+  radix_sort(keys, nullptr);
   size_t i = 0;
   while (i < keys.size()) {
     const uint64_t e = keys[i] >> 1;

@@ -17,7 +17,8 @@
 // tile that cannot hold another face is padded with flag-0 records.
 //
 // Stripification: greedy. Faces are adjacent across an edge shared by exactly
-// two faces (edges of more than two faces are treated as borders). From each
+// two faces (edges of more than two faces are treated as borders); the edge
+// matching is a radix sort of packed edge keys. From each
 // unvisited face (in index order) the walk tries the three rotations of its
 // corners and keeps the longest strip of unvisited faces, always crossing the
 // edge (w_{k-1}, w_k).
@@ -26,17 +27,13 @@
 #include <cstdint>
 #include <vector>
 
+#include "wn_radix.h"
+
 namespace wn {
 
 constexpr int64_t kStripTile = 255;
 
 namespace {
-
-struct EdgeRef {
-  int64_t lo, hi;  // vertex ids, lo < hi
-  int64_t face;
-  int32_t local;  // edge (c[local], c[(local + 1) % 3]) of the face
-};
 
 template <typename FT>
 int64_t corner(const FT* F, int64_t f, int c) {
@@ -50,24 +47,29 @@ struct Stripifier {
   std::vector<int64_t> nbr;  // 3 per face: face across local edge e, -1 if none
   std::vector<uint8_t> used;
 
+  // Vertex ids are below 2^31 (checked by every caller), so an edge is the
+  // key (lo << 32 | hi); the payload is (face << 2 | local edge). Entries are
+  // generated in face order and the radix sort is stable, so each edge's
+  // faces stay in ascending order, as a sort by (lo, hi, face) leaves them.
   Stripifier(const FT* F_, int64_t nf_) : F(F_), nf(nf_), nbr(3 * nf_, -1), used(nf_, 0) {
-    std::vector<EdgeRef> e;
-    e.reserve((size_t)(3 * nf));
+    std::vector<uint64_t> key, pay;
+    key.reserve((size_t)(3 * nf));
+    pay.reserve((size_t)(3 * nf));
     for (int64_t f = 0; f < nf; ++f)
       for (int k = 0; k < 3; ++k) {
-        int64_t a = corner(F, f, k), b = corner(F, f, (k + 1) % 3);
+        const int64_t a = corner(F, f, k), b = corner(F, f, (k + 1) % 3);
         if (a == b) continue;
-        e.push_back({std::min(a, b), std::max(a, b), f, k});
+        key.push_back((uint64_t)std::min(a, b) << 32 | (uint64_t)std::max(a, b));
+        pay.push_back((uint64_t)f << 2 | (uint64_t)k);
       }
-    std::sort(e.begin(), e.end(), [](const EdgeRef& x, const EdgeRef& y) {
-      return x.lo != y.lo ? x.lo < y.lo : (x.hi != y.hi ? x.hi < y.hi : x.face < y.face);
-    });
-    for (size_t i = 0; i < e.size();) {
+    radix_sort(key, &pay);
+    for (size_t i = 0; i < key.size();) {
       size_t j = i;
-      while (j < e.size() && e[j].lo == e[i].lo && e[j].hi == e[i].hi) ++j;
+      while (j < key.size() && key[j] == key[i]) ++j;
       if (j - i == 2) {
-        nbr[3 * e[i].face + e[i].local] = e[i + 1].face;
-        nbr[3 * e[i + 1].face + e[i + 1].local] = e[i].face;
+        const int64_t f0 = (int64_t)(pay[i] >> 2), f1 = (int64_t)(pay[i + 1] >> 2);
+        nbr[3 * f0 + (int64_t)(pay[i] & 3)] = f1;
+        nbr[3 * f1 + (int64_t)(pay[i + 1] & 3)] = f0;
       }
       i = j;
     }
@@ -96,32 +98,42 @@ struct Stripifier {
       if (corner(F, f, k) == p) return corner(F, f, (k + 1) % 3) == q ? 1 : -1;
     return 1;
   }
+};
+
+// Strip walks inside the face range [lo, hi) (faces outside it are
+// borders).
+template <typename FT>
+struct Walker {
+  Stripifier<FT>& s;
+  int64_t lo, hi;
+  std::vector<int64_t> touched;  // faces a trial walk marked, to unmark
 
   // Walk from face f with strip order (p, q, r); returns faces and vertices.
+  // mark = false leaves the used flags as they were.
   void walk(int64_t f, int64_t p, int64_t q, int64_t r, bool mark, std::vector<int64_t>& faces,
             std::vector<int64_t>& verts) {
     faces.assign(1, f);
     verts.assign({p, q, r});
-    std::vector<int64_t> touched;
+    touched.clear();
     if (!mark) touched.push_back(f);
-    used[f] = 1;
+    s.used[f] = 1;
     int64_t cur = f;
     for (;;) {
       const int64_t a = verts[verts.size() - 2], b = verts.back();
-      const int le = local_edge(cur, a, b);
+      const int le = s.local_edge(cur, a, b);
       if (le < 0) break;
-      const int64_t g = nbr[3 * cur + le];
-      if (g < 0 || used[g]) break;
-      const int64_t s = third(g, a, b);
-      if (s == a || s == b) break;
-      used[g] = 1;
+      const int64_t g = s.nbr[3 * cur + le];
+      if (g < lo || g >= hi || s.used[g]) break;
+      const int64_t t = s.third(g, a, b);
+      if (t == a || t == b) break;
+      s.used[g] = 1;
       if (!mark) touched.push_back(g);
       faces.push_back(g);
-      verts.push_back(s);
+      verts.push_back(t);
       cur = g;
     }
     if (!mark)
-      for (int64_t t : touched) used[t] = 0;
+      for (int64_t u : touched) s.used[u] = 0;
   }
 };
 
@@ -141,48 +153,57 @@ struct Emitter {
   }
 };
 
+// Strips of the faces in [lo, hi), appended to idx / flag from a tile start
+// and padded to a tile end.
 template <typename FT>
-int64_t build_topology(const FT* F, int64_t nf, std::vector<int32_t>& idx,
-                       std::vector<int8_t>& flag) {
-  Stripifier<FT> s(F, nf);
+void build_range(Stripifier<FT>& s, int64_t lo, int64_t hi, std::vector<int32_t>& idx,
+                 std::vector<int8_t>& flag) {
+  const FT* F = s.F;
+  Walker<FT> wk{s, lo, hi, {}};
   Emitter em{&idx, &flag};
-  std::vector<int64_t> faces, verts;
-  int64_t last_v = nf > 0 ? corner(F, 0, 0) : 0;
-  for (int64_t f = 0; f < nf; ++f) {
+  std::vector<int64_t> faces, verts, best_faces, best_verts;
+  int64_t last_v = corner(F, lo, 0);
+  for (int64_t f = lo; f < hi; ++f) {
     if (s.used[f]) continue;
-    // try the three rotations, keep the longest (no marking while trying)
-    size_t best = 0;
-    int best_rot = 0;
+    // try the three rotations (no marking), keep the longest, mark its faces
+    best_faces.clear();
     for (int rot = 0; rot < 3; ++rot) {
-      s.walk(f, corner(F, f, rot), corner(F, f, (rot + 1) % 3), corner(F, f, (rot + 2) % 3),
-             false, faces, verts);
-      if (faces.size() > best) {
-        best = faces.size();
-        best_rot = rot;
+      wk.walk(f, corner(F, f, rot), corner(F, f, (rot + 1) % 3), corner(F, f, (rot + 2) % 3),
+              false, faces, verts);
+      if (faces.size() > best_faces.size()) {
+        best_faces.swap(faces);
+        best_verts.swap(verts);
       }
     }
-    s.walk(f, corner(F, f, best_rot), corner(F, f, (best_rot + 1) % 3),
-           corner(F, f, (best_rot + 2) % 3), true, faces, verts);
+    for (int64_t g : best_faces) s.used[g] = 1;
     // emit in segments that fit the current tile
     size_t i = 0;  // index of the first face of the segment
-    while (i < faces.size()) {
+    while (i < best_faces.size()) {
       int64_t space = kStripTile - em.in_tile;
       if (space < 3) {
         em.pad_tile(last_v);
         space = kStripTile;
       }
-      const size_t m = std::min<size_t>(faces.size() - i, (size_t)(space - 2));
-      em.record(verts[i], 0);
-      em.record(verts[i + 1], 0);
+      const size_t m = std::min<size_t>(best_faces.size() - i, (size_t)(space - 2));
+      em.record(best_verts[i], 0);
+      em.record(best_verts[i + 1], 0);
       for (size_t k = 0; k < m; ++k) {
-        const int64_t face = faces[i + k];
-        em.record(verts[i + k + 2], s.orientation(face, verts[i + k], verts[i + k + 1]));
+        const int64_t face = best_faces[i + k];
+        em.record(best_verts[i + k + 2],
+                  s.orientation(face, best_verts[i + k], best_verts[i + k + 1]));
       }
-      last_v = verts[i + m + 1];
+      last_v = best_verts[i + m + 1];
       i += m;
     }
   }
   em.pad_tile(last_v);
+}
+
+template <typename FT>
+int64_t build_topology(const FT* F, int64_t nf, std::vector<int32_t>& idx,
+                       std::vector<int8_t>& flag) {
+  Stripifier<FT> s(F, nf);
+  build_range(s, 0, nf, idx, flag);
   return (int64_t)idx.size();
 }
 
