@@ -123,15 +123,21 @@ def boundary_edges(F, n_vertices: int, device="cuda"):
         raise ValueError("F must have shape (m, 3)")
     Fh = np.ascontiguousarray(Fh, dtype=np.int64)
     h = lib()
-    n = h.wn_boundary_edges(addr(Fh), int(n_vertices), len(Fh), 0, 0, 0, 0)
+    cap = len(Fh) // 4 + 64  # one call when the rim fits
+    e = np.zeros((cap, 2), dtype=np.int32)
+    m = np.zeros(cap, dtype=np.float64)
+    n = h.wn_boundary_edges(addr(Fh), int(n_vertices), len(Fh), 0, e.ctypes.data, m.ctypes.data,
+                            cap)
     if n < 0:
         raise (IndexError if "outside V" in last_error() else ValueError)(last_error())
-    e = np.zeros((n, 2), dtype=np.int32)
-    m = np.zeros(n, dtype=np.float64)
-    if n and h.wn_boundary_edges(addr(Fh), int(n_vertices), len(Fh), 0, e.ctypes.data,
-                                 m.ctypes.data, n) != n:
-        raise RuntimeError(last_error())
-    return torch.as_tensor(e, device=device), torch.as_tensor(m, device=device)
+    if n > cap:
+        e = np.zeros((n, 2), dtype=np.int32)
+        m = np.zeros(n, dtype=np.float64)
+        if h.wn_boundary_edges(addr(Fh), int(n_vertices), len(Fh), 0, e.ctypes.data,
+                               m.ctypes.data, n) != n:
+            raise RuntimeError(last_error())
+    return (torch.as_tensor(np.ascontiguousarray(e[:n]), device=device),
+            torch.as_tensor(np.ascontiguousarray(m[:n]), device=device))
 
 
 def backward_edges(V: torch.Tensor, edges: torch.Tensor, weight: torch.Tensor, Q: torch.Tensor,

@@ -22,6 +22,9 @@ evaluates one vertex per face instead of three:
 
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import PRECISIONS
@@ -135,6 +138,10 @@ class StripBatch(WindingBatch):
             raise ValueError("records must be a contiguous (R_total, 4) float64 tensor")
 
 
+def _workers(n: int) -> int:
+    return max(1, min(n, 16, os.cpu_count() or 1))
+
+
 def pack_batch_strips(vertices, faces, device="cuda"):
     """Strip records of every mesh (host stripifier), concatenated on the
     device: ((R_total, 4) float64 tensor, per-mesh record counts)."""
@@ -142,7 +149,9 @@ def pack_batch_strips(vertices, faces, device="cuda"):
 
     from .mesh import strip_records
 
-    recs = [strip_records(V, F) for V, F in zip(vertices, faces)]
+    # meshes stripify independently; the library calls release the GIL
+    with ThreadPoolExecutor(max_workers=_workers(len(vertices))) as pool:
+        recs = list(pool.map(strip_records, vertices, faces))
     counts = np.array([len(r) for r in recs], dtype=np.int64)
     rec = np.concatenate(recs) if recs else np.zeros((0, 4))
     return torch.as_tensor(np.ascontiguousarray(rec), device=device), counts
@@ -165,21 +174,32 @@ class BatchTopology:
         from . import _faces
 
         h = lib()
-        ids, flags, counts, off = [], [], [], 0
-        for F, nv in zip(faces, vertex_counts):
+
+        def topology(F, nv):
             Fa = _faces(F)
-            nv = int(nv)
-            n = h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, 0, 0, 0)
+            cap = len(Fa) + len(Fa) // 4 + 2 * 255  # one call when it fits
+            idx = np.empty(cap, dtype=np.int32)
+            flag = np.empty(cap, dtype=np.int8)
+            n = h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, idx.ctypes.data, flag.ctypes.data,
+                                    cap)
             if n < 0:
                 raise (IndexError if "outside V" in last_error() else ValueError)(last_error())
-            idx = np.empty(n, dtype=np.int32)
-            flag = np.empty(n, dtype=np.int8)
-            if n and h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, idx.ctypes.data,
-                                         flag.ctypes.data, n) != n:
-                raise RuntimeError(last_error())
+            if n > cap:
+                idx = np.empty(n, dtype=np.int32)
+                flag = np.empty(n, dtype=np.int8)
+                if h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, idx.ctypes.data,
+                                       flag.ctypes.data, n) != n:
+                    raise RuntimeError(last_error())
+            return idx[:n], flag[:n]
+
+        nvs = [int(nv) for nv in vertex_counts]
+        with ThreadPoolExecutor(max_workers=_workers(len(nvs))) as pool:
+            tops = list(pool.map(topology, faces, nvs))
+        ids, flags, counts, off = [], [], [], 0
+        for (idx, flag), nv in zip(tops, nvs):
             ids.append(idx.astype(np.int64) + off)
             flags.append(flag)
-            counts.append(n)
+            counts.append(len(idx))
             off += nv
         if off > np.iinfo(np.int32).max:
             raise ValueError("more than 2^31-1 vertices in the batch")

@@ -25,16 +25,19 @@ def strip_records(V, F) -> np.ndarray:
 
     vertices, faces = _mesh(V, F, three_dimensional=True)
     h = lib()
-    n = h.wn_mesh_records(addr(vertices), addr(faces), len(vertices), len(faces), 0, 0, 0)
+    # one stripification when the record count fits the first guess (about
+    # 1.06 records per face on meshes of long strips), a second otherwise
+    out = np.empty((len(faces) + len(faces) // 4 + 2 * 255, 4), dtype=np.float64)
+    n = h.wn_mesh_records(addr(vertices), addr(faces), len(vertices), len(faces), 0, addr(out),
+                          len(out))
     if n < 0:
         raise ValueError(last_error())
-    out = np.empty((max(n, 0), 4), dtype=np.float64)
-    if n:
-        got = h.wn_mesh_records(addr(vertices), addr(faces), len(vertices), len(faces), 0,
-                                addr(out), n)
-        if got != n:
+    if n > len(out):
+        out = np.empty((n, 4), dtype=np.float64)
+        if h.wn_mesh_records(addr(vertices), addr(faces), len(vertices), len(faces), 0, addr(out),
+                             n) != n:
             raise RuntimeError(last_error())
-    return out
+    return out[:n]
 
 
 class Mesh:
