@@ -382,6 +382,11 @@ def run_batch(args, h):
     w64 = plan64.run(tri, Q, use_graph=False).cpu().numpy()
     ms64, _ = timed(lambda: plan64.run(tri, Q, out=out, use_graph=True), max(1, args.steps // 2), 1)
     plan64.close()
+    splan64 = StripBatch(rc, qc, chunk_tiles=chunk_tiles, precision="fp64")
+    ms64s, _ = timed(lambda: splan64.run(rec, Q, out=out, use_graph=True), max(1, args.steps // 2), 1)
+    w64s = out.cpu().numpy()
+    splan64.close()
+    log(f"batched strips fp64 (graph) {ms64s:.3f} ms/step")
     info = plan.info()
     plan.close()
     d = np.abs(w_batch - w64)
@@ -395,7 +400,9 @@ def run_batch(args, h):
         face_records_batched={"value": pairs / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face},
         separate_launches={"value": pairs / (ms_sep * 1e-3) / 1e9, "ms_per_step": ms_sep},
         speedup_batched_vs_separate=ms_sep / ms_graph,
-        fp64_batched={"value": pairs / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64},
+        fp64_batched={"value": pairs / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64,
+                      "strips": {"value": pairs / (ms64s * 1e-3) / 1e9, "ms_per_step": ms64s,
+                                 "max_abs_dw_vs_fp64_faces": float(np.abs(w64s - w64).max())}},
         strip_records={"records": int(rc.sum()), "host_build_seconds": strip_build_s},
         tuning={"chunk_tiles": chunk_tiles, "seconds": tune_strip_s, "items": sinfo["items"],
                 "grid": sinfo["grid"],

@@ -212,6 +212,12 @@ int64_t wn_batch_create_strips(const int64_t *rec_off, const int64_t *query_off,
                                int64_t n_meshes, int64_t chunk_tiles, int64_t sort,
                                double kappa);
 
+/* The same strip plan in FP64 (the FP64 strip kernel of wn_mesh_forward64):
+ * allocates 64 bytes of sweep records per record, rebuilt from the records
+ * by every wn_batch_run. No kappa. */
+int64_t wn_batch_create_strips64(const int64_t *rec_off, const int64_t *query_off,
+                                 int64_t n_meshes, int64_t chunk_tiles, int64_t sort);
+
 /* Runs a plan: (device) tri (9 x F_total; the records of a strip plan),
  * q (Q_total x 3), dst (Q_total).
  * use_graph != 0 replays a hipGraph of the persistent launch, captured on

@@ -156,6 +156,7 @@ struct BatchPlan {
   double* d_qs = nullptr;  // queries in launch order (sorted plans)
   void* d_sort_ws = nullptr;
   size_t sort_bytes = 0;
+  StripRec64* d_rec64 = nullptr;  // FP64 strip plans: sweep records, rebuilt per run
   // graph cache
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -329,10 +330,11 @@ const char* batch_strip_check(const int64_t* rec_off, const int64_t* query_off, 
 }
 
 hipError_t batch_create_strips(const int64_t* rec_off, const int64_t* query_off, int64_t n_meshes,
-                               int64_t chunk_tiles, int sort, float kappa, BatchPlan** out) {
+                               int64_t chunk_tiles, int sort, float kappa, int precision,
+                               BatchPlan** out) {
   BatchPlan* p = new BatchPlan();
   p->strips = true;
-  p->precision = kFP32C;
+  p->precision = precision;
   p->qpl = 1;
   p->sort = sort;
   p->kappa = kappa;
@@ -348,11 +350,15 @@ hipError_t batch_create_strips(const int64_t* rec_off, const int64_t* query_off,
   std::vector<std::pair<int64_t, BatchItem>> items;
   const int64_t pbase = plan_items(tile_off.data(), query_off, n_meshes, chunk_tiles,
                                    kBlockThreads, 255, kBlockThreads * 255, meshes, items);
-  const int resident = batch_strip_resident_blocks();
+  const int resident =
+      precision == kFP64 ? batch_strip64_resident_blocks() : batch_strip_resident_blocks();
   hipError_t e = resident > 0 ? hipSuccess : hipErrorNoDevice;
   if (e == hipSuccess) e = plan_finish(p, meshes, items, pbase, query_off, resident);
+  if (e == hipSuccess && precision == kFP64 && p->ftot > 0)
+    e = hipMalloc(&p->d_rec64, (size_t)p->ftot * 255 * sizeof(StripRec64));
   if (e != hipSuccess) {
     if (p->ws) (void)hipFree(p->ws);
+    if (p->d_rec64) (void)hipFree(p->d_rec64);
     delete p;
     return e;
   }
@@ -383,7 +389,14 @@ static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q
   counter_reset_kernel<<<1, 64, 0, s>>>(p->d_counter);
   const int32_t* perm = p->sort ? p->d_perm : nullptr;
   if (p->sort) q = p->d_qs;
-  if (p->strips) {
+  if (p->strips && p->precision == kFP64) {
+    hipError_t e = strip64_prep_launch((const double4*)tri, p->ftot * 255, p->d_rec64, s);
+    if (e == hipSuccess)
+      e = batch_strip64_enqueue(p->d_rec64, (const double4*)tri, q, perm, dst, p->d_partial,
+                                p->d_meshes, p->d_items, p->n_items, p->d_counter,
+                                p->chunk_faces, p->grid, s);
+    if (e != hipSuccess) return e;
+  } else if (p->strips) {
     const hipError_t e = batch_strip_enqueue((const double4*)tri, q, perm, dst, p->d_partial,
                                              p->d_meshes, p->d_items, p->n_items, p->d_counter,
                                              p->chunk_faces, p->kappa, p->grid, s);
@@ -440,10 +453,9 @@ void batch_destroy(BatchPlan* p) {
     (void)hipStreamSynchronize(p->cap_stream);
     (void)hipStreamDestroy(p->cap_stream);
   }
-  if (p->ws) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(p->ws);
-  }
+  if (p->ws || p->d_rec64) (void)hipDeviceSynchronize();
+  if (p->ws) (void)hipFree(p->ws);
+  if (p->d_rec64) (void)hipFree(p->d_rec64);
   delete p;
 }
 

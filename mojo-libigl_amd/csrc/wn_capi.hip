@@ -539,8 +539,23 @@ int64_t wn_batch_create_strips(const int64_t* rec_off, const int64_t* query_off,
   wn::BatchPlan* plan = nullptr;
   hipError_t e = wn::batch_create_strips(rec_off, query_off, n_meshes, chunk,
                                          sort < 0 ? 1 : (sort ? 1 : 0),
-                                         kappa < 0.0 ? 1.0f : (float)kappa, &plan);
+                                         kappa < 0.0 ? 1.0f : (float)kappa, wn::kFP32C, &plan);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_batch_create_strips", e));
+  std::lock_guard<std::mutex> lock(batch_registry_mu());
+  batch_registry().insert(plan);
+  return (int64_t)(intptr_t)plan;
+}
+
+int64_t wn_batch_create_strips64(const int64_t* rec_off, const int64_t* query_off,
+                                 int64_t n_meshes, int64_t chunk_tiles, int64_t sort) {
+  const int64_t chunk = chunk_tiles <= 0 ? 8 : chunk_tiles;
+  if (const char* why = wn::batch_strip_check(rec_off, query_off, n_meshes, chunk))
+    return fail(kErrArg, why);
+  if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
+  wn::BatchPlan* plan = nullptr;
+  hipError_t e = wn::batch_create_strips(rec_off, query_off, n_meshes, chunk,
+                                         sort < 0 ? 1 : (sort ? 1 : 0), 1.0f, wn::kFP64, &plan);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_batch_create_strips64", e));
   std::lock_guard<std::mutex> lock(batch_registry_mu());
   batch_registry().insert(plan);
   return (int64_t)(intptr_t)plan;

@@ -148,6 +148,13 @@ struct BatchItem {
 // Persistent strip kernel over a work list (wn_strip.hip): one query per
 // lane, blocks of kBlockThreads queries, chunks of chunk_tiles tiles.
 int batch_strip_resident_blocks();
+// The FP64 strip counterpart (wn_strip64.hip) over the plan's sweep records.
+int batch_strip64_resident_blocks();
+hipError_t batch_strip64_enqueue(const StripRec64* rec64, const double4* rec, const double* q,
+                                 const int32_t* perm, double* dst, double* partial,
+                                 const BatchMesh* meshes, const BatchItem* items, int64_t n_items,
+                                 int32_t* counter, int64_t chunk_tiles, int grid,
+                                 hipStream_t stream);
 hipError_t batch_strip_enqueue(const double4* rec, const double* q, const int32_t* perm,
                                double* dst, double* partial, const BatchMesh* meshes,
                                const BatchItem* items, int64_t n_items, int32_t* counter,
@@ -162,11 +169,12 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
 hipError_t batch_run(BatchPlan* p, const double* tri, const double* q, double* dst,
                      bool use_graph, hipStream_t stream);
 // Strip plans: rec_off in records (each mesh a multiple of 255), chunks of
-// chunk_tiles tiles; FP32-compensated, one query per lane.
+// chunk_tiles tiles; FP32-compensated or FP64, one query per lane.
 const char* batch_strip_check(const int64_t* rec_off, const int64_t* query_off, int64_t n_meshes,
                               int64_t chunk_tiles);
 hipError_t batch_create_strips(const int64_t* rec_off, const int64_t* query_off, int64_t n_meshes,
-                               int64_t chunk_tiles, int sort, float kappa, BatchPlan** out);
+                               int64_t chunk_tiles, int sort, float kappa, int precision,
+                               BatchPlan** out);
 void batch_destroy(BatchPlan* p);
 // out6 = {items, partial slots, workspace bytes, persistent grid, faces, queries}
 void batch_info(const BatchPlan* p, int64_t* out6);

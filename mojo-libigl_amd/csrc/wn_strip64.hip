@@ -152,6 +152,24 @@ __device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag
   s_carry = s_qr;
 }
 
+// Unscaled FP64 angle sum of one query over strip tiles [t_begin, t_end).
+__device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ rec64,
+                                                const double4* __restrict__ rec, int64_t t_begin,
+                                                int64_t t_end, double ox, double oy, double oz) {
+  double acc = 0.0, s_carry = 0.0;
+  Slot64 A{0.0, 0.0, 0.0, 0.0, 0.0}, B = A, C = A;
+  for (int64_t t = t_begin; t < t_end; ++t) {
+    const StripRec64* r = rec64 + t * kStripTileRecords;
+    const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
+    for (int j = 0; j < kStripTileRecords; j += 3) {
+      strip64_step(r[j], flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz);
+      strip64_step(r[j + 1], flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz);
+      strip64_step(r[j + 2], flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz);
+    }
+  }
+  return acc;
+}
+
 // 6 waves per SIMD: 80 VGPRs, no spills
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6))) void fwd_strip64_kernel(
     const StripRec64* __restrict__ rec64, const double4* __restrict__ rec, int64_t ntiles,
@@ -165,22 +183,51 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6
   const double oz = valid ? q[3 * slot + 2] : 0.0;
   const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
   const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
-  double acc = 0.0, s_carry = 0.0;
-  Slot64 A{0.0, 0.0, 0.0, 0.0, 0.0}, B = A, C = A;
-  for (int64_t t = t_begin; t < t_end; ++t) {
-    const StripRec64* r = rec64 + t * kStripTileRecords;
-    const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
-    for (int j = 0; j < kStripTileRecords; j += 3) {
-      strip64_step(r[j], flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz);
-      strip64_step(r[j + 1], flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz);
-      strip64_step(r[j + 2], flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz);
-    }
-  }
+  const double acc = strip64_sweep(rec64, rec, t_begin, t_end, ox, oy, oz);
   if (valid) {
     if (splits == 1)
       out[perm ? (int64_t)perm[slot] : slot] = acc * kInv2Pi;
     else
       out[(int64_t)blockIdx.y * nq + slot] = acc;
+  }
+}
+
+// Persistent FP64 strip kernel over a batch plan's work list (mesh, query
+// tile, tile chunk), with the FP32 strip batch kernel's item protocol
+// (atomic counter, block-uniform exit) and output rule (single-chunk meshes
+// write w, others unscaled chunk-major partials summed in chunk order).
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6))) void batch_strip64_kernel(
+    const StripRec64* __restrict__ rec64, const double4* __restrict__ rec,
+    const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ dst,
+    double* __restrict__ partial, const BatchMesh* __restrict__ meshes,
+    const BatchItem* __restrict__ items, int64_t n_items, int32_t* __restrict__ counter,
+    int64_t chunk_tiles) {
+  __shared__ int32_t s_item;
+  const int tid = threadIdx.x;
+  for (;;) {
+    __syncthreads();  // every thread has read the previous s_item
+    if (tid == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= n_items) break;  // block-uniform exit
+    const BatchItem item = items[it];
+    const BatchMesh m = meshes[item.mesh];
+    const int32_t local = item.qtile * kBlockThreads + tid;
+    const bool valid = local < m.nq;
+    const int64_t slot = m.query_off + local;
+    const double ox = valid ? q[3 * slot] : 0.0;
+    const double oy = valid ? q[3 * slot + 1] : 0.0;
+    const double oz = valid ? q[3 * slot + 2] : 0.0;
+    const int64_t t0 = m.face_off + (int64_t)item.chunk * chunk_tiles;
+    int64_t t1 = t0 + chunk_tiles;
+    if (t1 > m.face_off + m.nf) t1 = m.face_off + m.nf;
+    const double acc = strip64_sweep(rec64, rec, t0, t1, ox, oy, oz);
+    if (valid) {
+      if (m.chunks == 1)
+        dst[perm ? (int64_t)perm[slot] : slot] = acc * kInv2Pi;
+      else
+        partial[m.pbase + (int64_t)item.chunk * m.nq + local] = acc;
+    }
   }
 }
 
@@ -191,6 +238,28 @@ hipError_t strip64_prep_launch(const double4* rec, int64_t nrec, StripRec64* out
   if (nrec == 0) return hipSuccess;
   strip64_prep_kernel<<<(unsigned)((nrec + kBlockThreads - 1) / kBlockThreads), kBlockThreads,
                         0, stream>>>(rec, nrec, out);
+  return hipGetLastError();
+}
+
+int batch_strip64_resident_blocks() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)batch_strip64_kernel,
+                                                   kBlockThreads, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return cus * per_cu;
+}
+
+hipError_t batch_strip64_enqueue(const StripRec64* rec64, const double4* rec, const double* q,
+                                 const int32_t* perm, double* dst, double* partial,
+                                 const BatchMesh* meshes, const BatchItem* items, int64_t n_items,
+                                 int32_t* counter, int64_t chunk_tiles, int grid,
+                                 hipStream_t stream) {
+  batch_strip64_kernel<<<dim3((unsigned)grid), kBlockThreads, 0, stream>>>(
+      rec64, rec, q, perm, dst, partial, meshes, items, n_items, counter, chunk_tiles);
   return hipGetLastError();
 }
 

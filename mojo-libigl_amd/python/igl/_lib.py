@@ -55,6 +55,7 @@ _SIGNATURES = {
     "wn_mesh_destroy": ([I], I),
     "wn_batch_create": ([I] * 7 + [D], I),
     "wn_batch_create_strips": ([I] * 5 + [D], I),
+    "wn_batch_create_strips64": ([I] * 5, I),
     "mli_aabb_squared_distance": ([I] * 14, None),
     "wn_aabb_build": ([I] * 11, I),
     "wn_aabb_create": ([I] * 6, I),

@@ -108,9 +108,12 @@ class WindingBatch:
 
 
 class StripBatch(WindingBatch):
-    """A planned batch over meshes in strip-record form (FP32-compensated)."""
+    """A planned batch over meshes in strip-record form: FP32-compensated, or
+    FP64 with precision="fp64" (the FP64 strip kernel; kappa does not
+    apply)."""
 
-    def __init__(self, record_counts, query_counts, chunk_tiles=0, sort=-1, kappa=-1.0):
+    def __init__(self, record_counts, query_counts, chunk_tiles=0, sort=-1, kappa=-1.0,
+                 precision="fp32c"):
         rc = np.asarray(record_counts, dtype=np.int64)
         qc = np.asarray(query_counts, dtype=np.int64)
         if rc.ndim != 1 or rc.shape != qc.shape or len(rc) == 0:
@@ -119,10 +122,20 @@ class StripBatch(WindingBatch):
             raise ValueError("record counts must be non-negative multiples of 255")
         self.face_off = np.concatenate([[0], np.cumsum(rc)]).astype(np.int64)
         self.query_off = np.concatenate([[0], np.cumsum(qc)]).astype(np.int64)
-        self.precision = PRECISIONS["fp32c"]
-        self.config = dict(chunk_tiles=int(chunk_tiles), sort=int(sort), kappa=float(kappa))
-        handle = lib().wn_batch_create_strips(self.face_off.ctypes.data, self.query_off.ctypes.data,
-                                              len(rc), int(chunk_tiles), int(sort), float(kappa))
+        precision = str(precision).lower()
+        if precision not in ("fp32c", "fp64"):
+            raise ValueError("precision must be 'fp32c' or 'fp64'")
+        self.precision = PRECISIONS[precision]
+        self.config = dict(chunk_tiles=int(chunk_tiles), sort=int(sort), kappa=float(kappa),
+                           precision=precision)
+        if precision == "fp64":
+            handle = lib().wn_batch_create_strips64(self.face_off.ctypes.data,
+                                                    self.query_off.ctypes.data, len(rc),
+                                                    int(chunk_tiles), int(sort))
+        else:
+            handle = lib().wn_batch_create_strips(self.face_off.ctypes.data,
+                                                  self.query_off.ctypes.data, len(rc),
+                                                  int(chunk_tiles), int(sort), float(kappa))
         if handle <= 0:
             raise RuntimeError(f"wn_batch_create_strips failed: {last_error()}")
         self.handle = handle
@@ -249,7 +262,7 @@ def pack_batch(vertices, faces, device="cuda"):
 def winding_number_batch(vertices, faces, queries, precision="fp32c", use_graph=False,
                          strips=False, **config):
     """Host convenience: lists of (V_i, F_i, O_i) -> list of w_i (numpy).
-    strips=True runs an FP32-compensated strip plan (precision ignored)."""
+    strips=True runs a strip plan (precision "fp32c" or "fp64")."""
     import torch
 
     Q = torch.as_tensor(np.concatenate([np.asarray(o, dtype=np.float64).reshape(-1, 3)
@@ -257,7 +270,7 @@ def winding_number_batch(vertices, faces, queries, precision="fp32c", use_graph=
     qc = [len(np.asarray(o).reshape(-1, 3)) for o in queries]
     if strips:
         tri, counts = pack_batch_strips(vertices, faces)
-        plan = StripBatch(counts, qc, **config)
+        plan = StripBatch(counts, qc, precision=precision, **config)
     else:
         tri = pack_batch(vertices, faces)
         plan = WindingBatch([len(f) for f in faces], qc, precision=precision, **config)

@@ -201,3 +201,54 @@ def test_batch_topology_device_records(require_gpu, small_batch):
         assert torch.equal(plan.run(rec2, Q, use_graph=False), plan.run(host2, Q, use_graph=False))
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("chunk_tiles", [1, 3, 0])
+def test_strip64_batch_matches_oracle(require_gpu, small_batch, chunk_tiles):
+    """FP64 strip plans against the oracle at the FP64 bar."""
+    w = _run(small_batch, "fp64", strips=True, chunk_tiles=chunk_tiles)
+    worst = 0.0
+    for (V, F, Q), a in zip(small_batch, w):
+        assert a.shape == (len(Q),)
+        if len(Q):
+            worst = max(worst, np.abs(a - oracle.winding(V.astype(np.float64), F, Q)).max())
+    print(f"fp64 strip batch chunk_tiles={chunk_tiles}: worst {worst:.2e}", flush=True)
+    assert worst <= 1e-10
+
+
+def test_strip64_batch_graph_and_single_mesh(require_gpu, small_batch):
+    """FP64 strip plan: graph replays equal the eager run, and a one-chunk
+    unsorted plan equals single-mesh FP64 strip launches (same records, same
+    query blocks, same face order) bit for bit."""
+    from igl.batch import StripBatch, pack_batch_strips
+    from igl.mesh import Mesh
+
+    B = small_batch
+    rec, counts = pack_batch_strips([b[0] for b in B], [b[1] for b in B])
+    Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
+    qc = [len(b[2]) for b in B]
+    plan = StripBatch(counts, qc, chunk_tiles=2, precision="fp64")
+    try:
+        eager = plan.run(rec, Q, use_graph=False).clone()
+        for i in range(3):
+            torch.randn(1000, device="cuda").sum().item()
+            out = plan.run(rec, Q, use_graph=True)
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager), f"replay {i}"
+    finally:
+        plan.close()
+    one = StripBatch(counts, qc, chunk_tiles=1 << 20, sort=0, precision="fp64")
+    try:
+        w = one.run(rec, Q, use_graph=False).cpu().numpy()
+    finally:
+        one.close()
+    off = np.concatenate([[0], np.cumsum(qc)])
+    for k, (V, F, Qk) in enumerate(B):
+        if len(Qk) == 0 or len(F) == 0:
+            continue
+        m = Mesh(V.astype(np.float64), F)
+        try:
+            single = m.winding_number(Qk, splits=1, sort=0, precision="fp64")
+        finally:
+            m.close()
+        np.testing.assert_array_equal(w[off[k]:off[k + 1]], single)
