@@ -198,3 +198,43 @@ def test_config_validation_through_the_abi():
     assert h.wn_resolve_config(10, 2**31, 0, 1, 1, 0, 1.0, cfg.ctypes.data) == -1
     assert h.wn_forward(0, 0, 0, 10, 10, 9, 1, 1, 0, 1.0, 0, 0, 0) == -1
     assert "precision" in _lib.last_error()
+
+
+def test_new_entries_validate_arguments_before_the_device():
+    """Argument errors of the strip, edge and batch entry points come back as
+    -1 with a message, before any device work (no GPU needed)."""
+    from igl._lib import addr
+
+    h = igl.lib()
+    V, F = W.tetrahedron()
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int64)
+    Q = np.zeros((2, 3))
+    out = np.zeros(2)
+    # host strip route: precision, then sizes
+    assert h.wn_winding_gpu_mesh(addr(V), addr(F), 4, 4, 0, addr(Q), addr(out), 2, 5) == -1
+    assert "precision" in _lib.last_error()
+    assert h.wn_winding_gpu_mesh(addr(V), addr(F), 4, 4, 0, addr(Q), addr(out), -1, 0) == -1
+    assert h.wn_winding_gpu_mesh(addr(V), addr(F), 4, 4, 0, addr(Q), addr(out), 0, 0) == 0
+    # edge adjoint: dtype, lanes, splits
+    assert h.wn_backward_edges(0, 4, 3, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0) == -1
+    assert "vdtype" in _lib.last_error()
+    assert h.wn_backward_edges(0, 4, 0, 0, 0, 0, 0, 0, 0, 0, 1, 3, 1, 0) == -1
+    assert "queries_per_lane" in _lib.last_error()
+    assert h.wn_backward_edges(0, 4, 0, 0, 0, 0, 0, 0, 0, 0, 1, 4, 70000, 0) == -1
+    # device record gather: dtype
+    assert h.wn_strip_gather(0, 4, 2, 0, 0, 10, 0, 0) == -1
+    assert "vdtype" in _lib.last_error()
+    # meshes: unknown handles
+    assert h.wn_mesh_forward64(12345, addr(Q), addr(out), 2, 0, 0, 0, 0, 0) == -1
+    assert "unknown mesh" in _lib.last_error()
+    # FP64 strip plans: offsets must be whole tiles
+    ro = np.array([0, 100], dtype=np.int64)
+    qo = np.array([0, 10], dtype=np.int64)
+    assert h.wn_batch_create_strips64(addr(ro), addr(qo), 1, 0, 0) == -1
+    # topology and boundary edges: index range
+    bad = F + 10
+    assert h.wn_strip_topology(addr(bad), 4, 4, 0, 0, 0, 0) == -1
+    assert h.wn_boundary_edges(addr(bad), 4, 4, 0, 0, 0, 0) == -1
+    assert "outside V" in _lib.last_error()
+    assert h.wn_boundary_edges(addr(F), 2**31, 4, 0, 0, 0, 0) == -1
