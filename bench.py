@@ -295,6 +295,12 @@ def run_torus(args, h):
             e2e.append((time.perf_counter() - t0) * 1e3)
         result_fp64["host_call_end_to_end_ms"] = min(e2e)
         acc["max_abs_dw_host_call_vs_fp64"] = float(np.abs(w_e2e - w64).max())
+        e2e32 = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            w_e2e32 = igl.winding_number(V, F, Q, device="gpu", precision="fp32c")
+            e2e32.append((time.perf_counter() - t0) * 1e3)
+        acc["max_abs_dw_host_call_fp32c_vs_fp64"] = float(np.abs(w_e2e32 - w64).max())
     mesh.close()
 
     line = base_line(
@@ -312,6 +318,10 @@ def run_torus(args, h):
     line["config"].update(faces=nf, queries=nq, strip_records=info["records"], splits=strip.splits,
                           morton_sort=bool(strip.sort), mesh_build_ms=create_ms,
                           wall_ms_per_step=wall_strip)
+    if result_fp64 is not None:
+        # igl.winding_number(V, F, Q, device="gpu", precision="fp32c") from
+        # host arrays: stripify, upload, kernel, download
+        line["host_call_end_to_end_ms"] = min(e2e32)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(tri, Q, args.cpu_seconds)
