@@ -81,29 +81,34 @@ __device__ __forceinline__ double div_f64(double n, double d) {
   return __builtin_fma(__builtin_fma(-d, q, n), r, q);
 }
 
-// c + a * b with c in scalar registers (a wave-uniform constant): the VOP3
-// v_fma_f64 form. Without it the compiler keeps the polynomial's constants
-// in VGPRs and copies one into the destination of a v_fmac_f64 per step.
-__device__ __forceinline__ double fma_sc(double a, double b, double c) {
-  double d;
-  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
-  return d;
-}
-
 // atan(x / 4) for |x| <= 1: x * P(x^2) with the n = 9 relative-error
 // minimax coefficients of atan on (0, 1/4], each divided by 4^(2k+1)
-// (powers of two: exact).
+// (powers of two: exact). The Horner steps are one asm block of VOP3
+// v_fma_f64 with the coefficients in scalar registers: compiled from C, the
+// coefficients stay in VGPRs and each step copies one into a v_fmac_f64
+// destination (7 extra FP64 moves per pair), and one asm block per step
+// costs a wait state after each.
 __device__ __forceinline__ double atan_small_x4_f64(double x) {
   const double t = x * x;
-  double p = 0.0459998373611164223 / 17179869184.0;                   // x^17: 4^17
-  p = fma_sc(p, t, -0.065237123716060834312 / 1073741824.0);   // x^15: 4^15
-  p = fma_sc(p, t, 0.076834985777981783012 / 67108864.0);      // x^13: 4^13
-  p = fma_sc(p, t, -0.090905895478190080677 / 4194304.0);      // x^11: 4^11
-  p = fma_sc(p, t, 0.11111104268448408897 / 262144.0);         // x^9: 4^9
-  p = fma_sc(p, t, -0.14285714203175044227 / 16384.0);         // x^7: 4^7
-  p = fma_sc(p, t, 0.19999999999500416733 / 1024.0);           // x^5: 4^5
-  p = fma_sc(p, t, -0.33333333333332167915 / 64.0);            // x^3: 4^3
-  p = fma_sc(p, t, 0.99999999999999999552 / 4.0);              // x^1: 4
+  double p;
+  asm("v_fma_f64 %0, %2, %1, %3\n\t"
+      "v_fma_f64 %0, %0, %1, %4\n\t"
+      "v_fma_f64 %0, %0, %1, %5\n\t"
+      "v_fma_f64 %0, %0, %1, %6\n\t"
+      "v_fma_f64 %0, %0, %1, %7\n\t"
+      "v_fma_f64 %0, %0, %1, %8\n\t"
+      "v_fma_f64 %0, %0, %1, %9\n\t"
+      "v_fma_f64 %0, %0, %1, %10"
+      : "=&v"(p)
+      : "v"(t), "v"(0.0459998373611164223 / 17179869184.0),  // x^17: 4^17
+        "s"(-0.065237123716060834312 / 1073741824.0),          // x^15: 4^15
+        "s"(0.076834985777981783012 / 67108864.0),             // x^13: 4^13
+        "s"(-0.090905895478190080677 / 4194304.0),             // x^11: 4^11
+        "s"(0.11111104268448408897 / 262144.0),                // x^9: 4^9
+        "s"(-0.14285714203175044227 / 16384.0),                // x^7: 4^7
+        "s"(0.19999999999500416733 / 1024.0),                  // x^5: 4^5
+        "s"(-0.33333333333332167915 / 64.0),                   // x^3: 4^3
+        "s"(0.99999999999999999552 / 4.0));                    // x^1: 4
   return x * p;
 }
 
@@ -123,12 +128,14 @@ __device__ __forceinline__ bool wave_all_far_x4_f64(double det8, double den2, do
 }
 
 // One record: R takes vertex j; when the record closes face (P, Q, R) its
-// angle joins acc. s_carry is the doubled dot product of the previous
-// record's two newest vertices (Q, R of the last step = P, Q here).
+// angle joins acc (far pairs) or acc_slow (pairs of the plain formula; two
+// sums, so that each is only ever updated in place). s_carry is the doubled
+// dot product of the previous record's two newest vertices (Q, R of the last
+// step = P, Q here).
 __device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag_bits,
                                              const Slot64& P, const Slot64& Q, Slot64& R,
-                                             double& s_carry, double& acc, double ox,
-                                             double oy, double oz) {
+                                             double& s_carry, double& acc, double& acc_slow,
+                                             double ox, double oy, double oz) {
   R.ax = r.x - ox;
   R.ay = r.y - oy;
   R.az = r.z - oz;
@@ -146,7 +153,8 @@ __device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       // flag -1.0 has the sign bit set: the face runs against the strip order
-      acc += strip64_pair(P.ax, P.ay, P.az, Q.ax, Q.ay, Q.az, R.ax, R.ay, R.az, flag_bits < 0);
+      acc_slow +=
+          strip64_pair(P.ax, P.ay, P.az, Q.ax, Q.ay, Q.az, R.ax, R.ay, R.az, flag_bits < 0);
     }
   }
   s_carry = s_qr;
@@ -156,18 +164,18 @@ __device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag
 __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ rec64,
                                                 const double4* __restrict__ rec, int64_t t_begin,
                                                 int64_t t_end, double ox, double oy, double oz) {
-  double acc = 0.0, s_carry = 0.0;
+  double acc = 0.0, acc_slow = 0.0, s_carry = 0.0;
   Slot64 A{0.0, 0.0, 0.0, 0.0, 0.0}, B = A, C = A;
   for (int64_t t = t_begin; t < t_end; ++t) {
     const StripRec64* r = rec64 + t * kStripTileRecords;
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     for (int j = 0; j < kStripTileRecords; j += 3) {
-      strip64_step(r[j], flags[4 * j], A, B, C, s_carry, acc, ox, oy, oz);
-      strip64_step(r[j + 1], flags[4 * (j + 1)], B, C, A, s_carry, acc, ox, oy, oz);
-      strip64_step(r[j + 2], flags[4 * (j + 2)], C, A, B, s_carry, acc, ox, oy, oz);
+      strip64_step(r[j], flags[4 * j], A, B, C, s_carry, acc, acc_slow, ox, oy, oz);
+      strip64_step(r[j + 1], flags[4 * (j + 1)], B, C, A, s_carry, acc, acc_slow, ox, oy, oz);
+      strip64_step(r[j + 2], flags[4 * (j + 2)], C, A, B, s_carry, acc, acc_slow, ox, oy, oz);
     }
   }
-  return acc;
+  return acc + acc_slow;
 }
 
 // 6 waves per SIMD: 80 VGPRs, no spills
