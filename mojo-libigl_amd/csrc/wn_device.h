@@ -233,12 +233,13 @@ __device__ __forceinline__ bool wave_all_far(float det2, float den2, float s2) {
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }
 
-// wave_all_far for det8 = 4 x det2 and dp = den2 - 2 la lb lc:
-// |det8| < den2 is |det2| < kFarTau den2 (kFarTau = 1/4) and dp > 0 is
-// den2 > kFarKappa s2 (kFarKappa = 1), neither needing a multiply.
-__device__ __forceinline__ bool wave_all_far_x4(float det8, float den2, float dp) {
+// wave_all_far for det4 = 4 det, den and dp = den - la lb lc (any common
+// power-of-two scale of det, den and dp): |det4| < den is
+// |det| < kFarTau den (kFarTau = 1/4) and dp > 0 is den > kFarKappa la lb lc
+// (kFarKappa = 1), neither needing a multiply.
+__device__ __forceinline__ bool wave_all_far_x4(float det4, float den, float dp) {
   static_assert(kFarTau == 0.25f && kFarKappa == 1.0f, "x4 form assumes tau 1/4, kappa 1");
-  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det8) < den2);
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det4) < den);
   const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp > 0.0f);
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }

@@ -15,8 +15,12 @@
 //
 // LDS tile (255 records, a multiple of 3 so the three register slots rotate
 // statically through a 3-way unrolled loop):
-//   R0[j] = (w.x, w.y, w.z, |w_j - w_{j-1}|^2)      (block-local FP32 coords)
-//   R1[j] = (|w_j - w_{j-2}|^2, N8.x, N8.y, N8.z)     (N8 = 8 x face normal)
+//   R0[j] = (w.x, w.y, w.z, -|w_j - w_{j-1}|^2 / 2)  (block-local FP32 coords)
+//   R1[j] = (-|w_j - w_{j-2}|^2 / 2, N4.x, N4.y, N4.z)  (N4 = 4 x face normal)
+// so that a dot product is one add and one fma with 0.5 (law of cosines),
+// den is formed undoubled, and det4 = 4 det needs no multiply for the far
+// test |det| < den / 4 (all scalings powers of two: the values are those of
+// the doubled forms, halved exactly).
 // The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
 // orientation vs strip order) is read from the records themselves, at a
 // wave-uniform address (scalar loads), not staged.
@@ -56,13 +60,12 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   const double d2x = (double)cx - qx, d2y = (double)cy - qy, d2z = (double)cz - qz;
   const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
   // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag; stored
-  // as 8 N so that the lane's dot product is 4 x the doubled det, and the
-  // far test |det2| < den2 / 4 needs no multiply (powers of two: exact)
-  const double f2 = 8.0 * c.w;
-  sm.r0[j] = make_float4(cx, cy, cz, (float)(d1x * d1x + d1y * d1y + d1z * d1z));
-  sm.r1[j] = make_float4((float)(d2x * d2x + d2y * d2y + d2z * d2z),
-                         (float)(f2 * (ey * d2z - ez * d2y)), (float)(f2 * (ez * d2x - ex * d2z)),
-                         (float)(f2 * (ex * d2y - ey * d2x)));
+  // as 4 N so that the lane's dot product is 4 det
+  const double f4 = 4.0 * c.w;
+  sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
+  sm.r1[j] = make_float4((float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
+                         (float)(f4 * (ey * d2z - ez * d2y)), (float)(f4 * (ez * d2x - ex * d2z)),
+                         (float)(f4 * (ex * d2y - ey * d2x)));
 }
 
 // FP64 re-evaluation of face (j-2, j-1, j) from the LDS corners, oriented.
@@ -93,27 +96,27 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm, int j, int boff,
   R.l2 = __builtin_fmaf(R.az, R.az, __builtin_fmaf(R.ay, R.ay, R.ax * R.ax));
   R.l = __builtin_amdgcn_sqrtf(R.l2);
   const float4 r1 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r1) + boff);
-  const float s_qr = (Q.l2 + R.l2) - r0.w;  // 2 b.c
-  const float s_pr = (P.l2 + R.l2) - r1.x;  // 2 c.a
+  const float s_qr = __builtin_fmaf(Q.l2 + R.l2, 0.5f, r0.w);  // b.c
+  const float s_pr = __builtin_fmaf(P.l2 + R.l2, 0.5f, r1.x);  // c.a
   if (flag_bits != 0) {
-    // face (a, b, c) = (P, Q, R): s_ab = s_carry, s_bc = s_qr, s_ca = s_pr
-    // det8 = 4 x the doubled det (exactly: the normal is stored scaled by 8)
-    const float det8 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
-    // den2 = 2 la lb lc + dp with dp = lc s_ab + la s_bc + lb s_ca; the far
-    // test den2 > 2 la lb lc (kFarKappa = 1) is dp > 0
+    // face (a, b, c) = (P, Q, R): a.b = s_carry, b.c = s_qr, c.a = s_pr
+    // det4 = 4 det (the normal is stored scaled by 4)
+    const float det4 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
+    // den = la lb lc + dp with dp = lc a.b + la b.c + lb c.a; the far test
+    // den > la lb lc (kFarKappa = 1) is dp > 0
     const float pq = P.l * Q.l;
     const float dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
-    const float den2 = __builtin_fmaf(pq + pq, R.l, dp);
-    if (__builtin_expect(wave_all_far_x4(det8, den2, dp), 1)) {
-      part += atan_small_x4_f32(det8 * __builtin_amdgcn_rcpf(den2));
+    const float den = __builtin_fmaf(pq, R.l, dp);
+    if (__builtin_expect(wave_all_far_x4(det4, den, dp), 1)) {
+      part += atan_small_x4_f32(det4 * __builtin_amdgcn_rcpf(den));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       float mx;
-      float ang = atan2_f32(0.25f * det8, den2, mx);
-      // ill-conditioned (mx < kappa x 2 la lb lc): the FP64 angle, rounded
+      float ang = atan2_f32(0.25f * det4, den, mx);
+      // ill-conditioned (mx < kappa x la lb lc): the FP64 angle, rounded
       // once, replaces the FP32 one. Flag -1.0 has the sign bit set: the
       // face runs against the strip order.
-      if (mx < kappa * ((pq + pq) * R.l))
+      if (mx < kappa * (pq * R.l))
         ang = (float)strip_pair_f64(sm, j, flag_bits < 0, ox, oy, oz);
       part += ang;
     }
