@@ -22,8 +22,10 @@
 // test |det| < den / 4 (all scalings powers of two: the values are those of
 // the doubled forms, halved exactly).
 // The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
-// orientation vs strip order) is read from the records themselves, at a
-// wave-uniform address (scalar loads), not staged.
+// orientation vs strip order) is folded into N4; a record with flag 0 is
+// staged with N4 = 0 and R1.x = +inf, which makes its angle 0 without a
+// branch. The flag itself is read (from the records, at a wave-uniform
+// address) only on the rare slow path.
 // Squared lengths and N2 are computed in FP64 from the rounded FP32 corners.
 
 #include "wn_device.h"
@@ -61,9 +63,13 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
   // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag; stored
   // as 4 N so that the lane's dot product is 4 det
+  // A record that closes no face (flag 0) gets N = 0 and +inf in place of
+  // -|w_k - w_{k-2}|^2 / 2: c.a and with it dp and den are then +inf, and
+  // its fast-path angle is 4 det / den = 0 / inf = 0 exactly (see strip_step).
   const double f4 = 4.0 * c.w;
   sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
-  sm.r1[j] = make_float4((float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
+  sm.r1[j] = make_float4(c.w == 0.0 ? __builtin_inff()
+                                    : (float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
                          (float)(f4 * (ey * d2z - ez * d2y)), (float)(f4 * (ez * d2x - ex * d2z)),
                          (float)(f4 * (ex * d2y - ey * d2x)));
 }
@@ -79,49 +85,74 @@ __device__ __noinline__ double strip_pair_f64(const StripSmem& sm, int j, bool f
   return flip ? -v : v;
 }
 
-// flag_bits: the record's flag as raw bits (0 = no face ends here), read
-// from global memory at a wave-uniform address, so it arrives in SGPRs and
-// the emit test below is a scalar branch.
-// boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
-// The face's angle is added to part, the FP32 sum of at most three angles
-// that the caller adds to the FP64 sum.
-__device__ __forceinline__ void strip_step(const StripSmem& sm, int j, int boff,
-                                           long long flag_bits, VSlot& P, VSlot& Q, VSlot& R,
-                                           float& s_carry, float& part, float ox, float oy,
-                                           float oz, float kappa) {
-  const float4 r0 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r0) + boff);
+// Offset, squared length and length of the lane's query to record vertex
+// r0 (block-local FP32 coordinates).
+__device__ __forceinline__ void strip_vertex(const float4& r0, float ox, float oy, float oz,
+                                             VSlot& R) {
   R.ax = r0.x - ox;
   R.ay = r0.y - oy;
   R.az = r0.z - oz;
   R.l2 = __builtin_fmaf(R.az, R.az, __builtin_fmaf(R.ay, R.ay, R.ax * R.ax));
   R.l = __builtin_amdgcn_sqrtf(R.l2);
+}
+
+// Face (a, b, c) = (P, Q, R) closed by a record with tile data r0, r1; a.b
+// = s_carry (carried from the previous record), b.c = s_qr (carried to the
+// next), c.a from r1.x (law of cosines). det4 = 4 det (the normal is stored
+// scaled by 4); den = la lb lc + dp with dp = lc a.b + la b.c + lb c.a.
+struct StripGeom {
+  float det4, den, dp, pq, s_qr;
+};
+
+__device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& r1, const VSlot& P,
+                                                const VSlot& Q, const VSlot& R, float s_carry) {
+  StripGeom g;
+  g.s_qr = __builtin_fmaf(Q.l2 + R.l2, 0.5f, r0.w);
+  const float s_pr = __builtin_fmaf(P.l2 + R.l2, 0.5f, r1.x);
+  g.det4 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
+  g.pq = P.l * Q.l;
+  g.dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, g.s_qr, Q.l * s_pr));
+  g.den = __builtin_fmaf(g.pq, R.l, g.dp);
+  return g;
+}
+
+// Record j of the tile closes face (P, Q, R) (P and Q from the two previous
+// records, R computed here); its angle is added to part, the FP32 sum of at
+// most three angles that the caller adds to the FP64 sum. Every record takes
+// the same instructions: one that closes no face (flag 0: the first two of a
+// strip, tile padding) was staged (stage_strip_tile) with N4 = 0 and c.a =
+// +inf, so dp = den = +inf and its fast-path angle is 0 / inf = 0 exactly.
+// Its flag (raw bits of the record's w; read from global memory at a
+// wave-uniform address) is looked at only on the slow path, the one place
+// where such a record could come out nonzero (its Q.l = 0 makes dp a NaN).
+// boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
+__device__ __forceinline__ void strip_step(const StripSmem& sm,
+                                           const long long* __restrict__ flags, int j, int boff,
+                                           VSlot& P, VSlot& Q, VSlot& R, float& s_carry,
+                                           float& part, float ox, float oy, float oz, float kappa) {
+#pragma clang fp contract(off)  // part += angle rounds the angle first
+  const float4 r0 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r0) + boff);
+  strip_vertex(r0, ox, oy, oz, R);
   const float4 r1 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r1) + boff);
-  const float s_qr = __builtin_fmaf(Q.l2 + R.l2, 0.5f, r0.w);  // b.c
-  const float s_pr = __builtin_fmaf(P.l2 + R.l2, 0.5f, r1.x);  // c.a
-  if (flag_bits != 0) {
-    // face (a, b, c) = (P, Q, R): a.b = s_carry, b.c = s_qr, c.a = s_pr
-    // det4 = 4 det (the normal is stored scaled by 4)
-    const float det4 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
-    // den = la lb lc + dp with dp = lc a.b + la b.c + lb c.a; the far test
-    // den > la lb lc (kFarKappa = 1) is dp > 0
-    const float pq = P.l * Q.l;
-    const float dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, s_qr, Q.l * s_pr));
-    const float den = __builtin_fmaf(pq, R.l, dp);
-    if (__builtin_expect(wave_all_far_x4(det4, den, dp), 1)) {
-      part += atan_small_x4_f32(det4 * __builtin_amdgcn_rcpf(den));
+  const StripGeom g = strip_geom(r0, r1, P, Q, R, s_carry);
+  float ang = atan_small_x4_f32(g.det4 * __builtin_amdgcn_rcpf(g.den));
+  if (__builtin_expect(!wave_all_far_x4(g.det4, g.den, g.dp), 0)) {
+    asm volatile("" ::);  // keep a real wave-uniform branch
+    const long long flag_bits = flags[4 * j];
+    if (flag_bits == 0) {
+      ang = 0.0f;
     } else {
-      asm volatile("" ::);  // keep a real wave-uniform branch
       float mx;
-      float ang = atan2_f32(0.25f * det4, den, mx);
+      ang = atan2_f32(0.25f * g.det4, g.den, mx);
       // ill-conditioned (mx < kappa x la lb lc): the FP64 angle, rounded
       // once, replaces the FP32 one. Flag -1.0 has the sign bit set: the
       // face runs against the strip order.
-      if (mx < kappa * (pq * R.l))
+      if (mx < kappa * (g.pq * R.l))
         ang = (float)strip_pair_f64(sm, j, flag_bits < 0, ox, oy, oz);
-      part += ang;
     }
   }
-  s_carry = s_qr;
+  part += ang;
+  s_carry = g.s_qr;
 }
 
 // First thread of the calling wave, in a scalar register (all lanes active).
@@ -184,21 +215,20 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     stage_strip_tile(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
     __syncthreads();
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
+    // The record offset is wave-uniform, but an LDS read takes its address
+    // from a VGPR: a VGPR byte offset advanced once per three records and
+    // read with immediate offsets.
+    int boff = 0;
     for (int j = 0; j < kStripTileRecords; j += 3) {
-      // The record offset is wave-uniform, but an LDS read takes its address
-      // from a VGPR: one VGPR copy per three records, read with immediate
-      // offsets, instead of a copy from the scalar loop counter per read.
-      int boff = j * (int)sizeof(float4);
       asm volatile("" : "+v"(boff));
       // the three records' FP32 angles are summed in FP32 (at most three
       // terms, each |angle| <= pi) and added to the FP64 sum once
       float part = -0.0f;  // -0 + x == x for every x: the first add folds away
-      strip_step(sm, j, boff, flags[4 * j], A, B, C, s_carry, part, ox, oy, oz, kappa);
-      strip_step(sm, j + 1, boff + 16, flags[4 * (j + 1)], B, C, A, s_carry, part, ox, oy, oz,
-                 kappa);
-      strip_step(sm, j + 2, boff + 32, flags[4 * (j + 2)], C, A, B, s_carry, part, ox, oy, oz,
-                 kappa);
+      strip_step(sm, flags, j, boff, A, B, C, s_carry, part, ox, oy, oz, kappa);
+      strip_step(sm, flags, j + 1, boff + 16, B, C, A, s_carry, part, ox, oy, oz, kappa);
+      strip_step(sm, flags, j + 2, boff + 32, C, A, B, s_carry, part, ox, oy, oz, kappa);
       acc += (double)part;
+      boff += 3 * (int)sizeof(float4);
     }
   }
   __syncthreads();
