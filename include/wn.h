@@ -84,6 +84,18 @@ int64_t wn_winding_gpu_ex(const double *tri, const double *q, double *dst,
                           int64_t nf, int64_t nq, int64_t precision, int64_t qpl,
                           int64_t splits, int64_t sort, double kappa);
 
+/* Replaces the host SoA pack (python/igl/__init__.py:248-250) followed by
+ * mli_winding_gpu (src/capi.mojo:945-947): (host) V (nv x 3 float64) and F
+ * (nf x 3; fdtype 0 = int64, 1 = int32) are uploaded as they are and gathered
+ * into the face SoA on the device, then the forward of wn_winding_gpu_ex runs
+ * with its default tuning. Same results as packing on the host, bit for bit.
+ * A corner outside [0, nv) fails with -1 before anything is uploaded (the
+ * reference raises IndexError in Python, python/igl/__init__.py:119-124).
+ * (host) q, dst; synchronous; 0 or a negative status. */
+int64_t wn_winding_gpu_vf(const double *V, const void *F, int64_t nv, int64_t nf,
+                          int64_t fdtype, const double *q, double *dst, int64_t nq,
+                          int64_t precision);
+
 /* Resolved launch configuration for the arguments: out4 = {qpl, splits, sort,
  * precision}. */
 int64_t wn_resolve_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl,

@@ -131,11 +131,45 @@ wn::FwdConfig make_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl
   return c;
 }
 
-int64_t host_path(const double* tri, const double* q, double* dst, int64_t nf, int64_t nq,
+// The mesh of a host-buffer call: either the face SoA (tri, 9 x nf float64,
+// packed by the caller as python/igl/__init__.py:248-250 does) or V (nv x 3
+// float64) and F (nf x 3; fdtype 0 = int64, 1 = int32), uploaded as they are
+// and gathered into the SoA on the device (half the bytes of the SoA for a
+// closed mesh, and no host gather).
+struct HostMesh {
+  bool gather = false;  // false: tri; true: V and F
+  const double* tri = nullptr;
+  const double* V = nullptr;
+  const void* F = nullptr;
+  int64_t nv = 0;
+  int fdtype = 0;
+};
+
+// First face with a corner outside [0, nv), or -1.
+template <typename FT>
+int64_t first_bad_face(const FT* F, int64_t nv, int64_t nf) {
+  for (int64_t f = 0; f < nf; ++f)
+    for (int c = 0; c < 3; ++c) {
+      const int64_t vi = (int64_t)F[3 * f + c];
+      if (vi < 0 || vi >= nv) return f;
+    }
+  return -1;
+}
+
+int64_t host_path(const HostMesh& mesh, const double* q, double* dst, int64_t nf, int64_t nq,
                   const wn::FwdConfig& cfg) {
-  if (nq < 0 || nf < 0) return fail(kErrArg, "negative size");
+  if (nq < 0 || nf < 0 || mesh.nv < 0) return fail(kErrArg, "negative size");
   if (nq == 0) return kOk;
-  if (!tri && nf > 0) return fail(kErrArg, "null tri");
+  const bool gather = mesh.gather;
+  if (nf > 0 && !gather && !mesh.tri) return fail(kErrArg, "null tri");
+  if (nf > 0 && gather) {
+    if (!mesh.V || !mesh.F) return fail(kErrArg, "null V or F");
+    if (mesh.fdtype != 0 && mesh.fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+    const int64_t bad = mesh.fdtype == 0
+                            ? first_bad_face((const int64_t*)mesh.F, mesh.nv, nf)
+                            : first_bad_face((const int32_t*)mesh.F, mesh.nv, nf);
+    if (bad >= 0) return fail(kErrArg, "face " + std::to_string(bad) + " has a corner outside V");
+  }
   if (!q || !dst) return fail(kErrArg, "null query or output buffer");
   if (const char* why = wn::fwd_check(nf, nq, cfg)) return fail(kErrArg, why);
   HostPathContext& ctx = host_ctx();
@@ -156,7 +190,9 @@ int64_t host_path(const double* tri, const double* q, double* dst, int64_t nf, i
   const size_t q_b = wn::align_up((size_t)nq * 3 * sizeof(double));
   const size_t dst_b = wn::align_up((size_t)nq * sizeof(double));
   const size_t ws_b = wn::fwd_workspace_bytes(nf, nq, cfg);
-  const size_t need = tri_b + q_b + dst_b + ws_b;
+  const size_t v_b = gather ? wn::align_up((size_t)mesh.nv * 3 * sizeof(double)) : 0;
+  const size_t f_b = gather ? wn::align_up((size_t)nf * 3 * (mesh.fdtype == 0 ? 8 : 4)) : 0;
+  const size_t need = tri_b + q_b + dst_b + ws_b + v_b + f_b;
   if (injected("alloc")) return fail(kErrInjected, "injected allocation failure (WN_FAULT=alloc)");
   if (need > ctx.cap) {
     if (ctx.buf) {
@@ -178,10 +214,25 @@ int64_t host_path(const double* tri, const double* q, double* dst, int64_t nf, i
   double* d_dst = (double*)(base + tri_b + q_b);
   void* d_ws = base + tri_b + q_b + dst_b;
 
-  if (nf > 0) {
-    e = hipMemcpyAsync(d_tri, tri, (size_t)nf * 9 * sizeof(double), hipMemcpyHostToDevice,
+  if (nf > 0 && !gather) {
+    e = hipMemcpyAsync(d_tri, mesh.tri, (size_t)nf * 9 * sizeof(double), hipMemcpyHostToDevice,
                        ctx.stream);
     if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy tri", e));
+  } else if (nf > 0) {
+    char* d_v = base + tri_b + q_b + dst_b + ws_b;
+    char* d_f = d_v + v_b;
+    e = hipMemcpyAsync(d_v, mesh.V, (size_t)mesh.nv * 3 * sizeof(double), hipMemcpyHostToDevice,
+                       ctx.stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_f, mesh.F, (size_t)nf * 3 * (mesh.fdtype == 0 ? 8 : 4),
+                         hipMemcpyHostToDevice, ctx.stream);
+    if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy V/F", e));
+    // indices were checked above: the gather reads only inside V
+    e = wn::pack_soa_launch(d_v, d_f, mesh.nv, nf, 0, mesh.fdtype, d_tri, nullptr, ctx.stream);
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(ctx.stream);
+      return fail(kErrHip, hip_msg("pack launch", e));
+    }
   }
   e = hipMemcpyAsync(d_q, q, (size_t)nq * 3 * sizeof(double), hipMemcpyHostToDevice, ctx.stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy q", e));
@@ -377,14 +428,31 @@ int64_t wn_device_free_bytes(void) {
 int64_t mli_winding_gpu(const double* tri, const double* q, double* dst, int64_t nf,
                         int64_t nq) {
   const wn::FwdConfig cfg = make_config(nf, nq, wn::kFP64, 0, 0, 0, 1.0);
-  return host_path(tri, q, dst, nf, nq, cfg) == kOk ? 1 : 0;
+  HostMesh mesh;
+  mesh.tri = tri;
+  return host_path(mesh, q, dst, nf, nq, cfg) == kOk ? 1 : 0;
 }
 
 int64_t wn_winding_gpu_ex(const double* tri, const double* q, double* dst, int64_t nf,
                           int64_t nq, int64_t precision, int64_t qpl, int64_t splits,
                           int64_t sort, double kappa) {
   const wn::FwdConfig cfg = make_config(nf, nq, precision, qpl, splits, sort, kappa);
-  return host_path(tri, q, dst, nf, nq, cfg);
+  HostMesh mesh;
+  mesh.tri = tri;
+  return host_path(mesh, q, dst, nf, nq, cfg);
+}
+
+int64_t wn_winding_gpu_vf(const double* V, const void* F, int64_t nv, int64_t nf,
+                          int64_t fdtype, const double* q, double* dst, int64_t nq,
+                          int64_t precision) {
+  const wn::FwdConfig cfg = make_config(nf, nq, precision, 0, 0, -1, -1.0);
+  HostMesh mesh;
+  mesh.gather = true;
+  mesh.V = V;
+  mesh.F = F;
+  mesh.nv = nv;
+  mesh.fdtype = (int)fdtype;
+  return host_path(mesh, q, dst, nf, nq, cfg);
 }
 
 int64_t wn_resolve_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl,

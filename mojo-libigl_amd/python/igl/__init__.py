@@ -184,19 +184,14 @@ def winding_number(V, F, O, device="cpu", *, precision="fp64"):
                     raise RuntimeError(
                         f"the HIP GPU winding-number kernel failed: {_last_error(handle)}")
                 return float(result[0]) if scalar else result
-            triangles = pack_faces(vertices, faces)
-            if mode == 0:
-                ok = handle.mli_winding_gpu(addr(triangles), addr(queries), addr(result),
-                                            len(faces), len(queries))
-                if not ok:
-                    raise RuntimeError(
-                        f"the HIP GPU winding-number kernel failed: {_last_error(handle)}")
-            else:
-                status = handle.wn_winding_gpu_ex(addr(triangles), addr(queries), addr(result),
-                                                  len(faces), len(queries), mode, 0, 0, -1, -1.0)
-                if status != 0:
-                    raise RuntimeError(
-                        f"the HIP GPU winding-number kernel failed: {_last_error(handle)}")
+            # V and F go up as they are and the face SoA is gathered on the
+            # device (the reference packs it on the host, :248-250)
+            status = handle.wn_winding_gpu_vf(addr(vertices), addr(faces), len(vertices),
+                                              len(faces), 0, addr(queries), addr(result),
+                                              len(queries), mode)
+            if status != 0:
+                raise RuntimeError(
+                    f"the HIP GPU winding-number kernel failed: {_last_error(handle)}")
         else:
             triangles = pack_faces(vertices, faces)
             # Every pair costs three square roots and an arctangent, so the

@@ -34,6 +34,7 @@ _SIGNATURES = {
     "wn_last_error": ([], S),
     "wn_version": ([], I),
     "wn_winding_gpu_ex": ([I] * 9 + [D], I),
+    "wn_winding_gpu_vf": ([I] * 9, I),
     "wn_resolve_config": ([I] * 6 + [D, I], I),
     "wn_forward_workspace_bytes": ([I] * 6, I),
     "wn_forward": ([I] * 9 + [D] + [I] * 3, I),

@@ -162,6 +162,10 @@ def test_gpu_kernel_failure_is_not_swallowed(monkeypatch):
         def wn_winding_gpu_ex(*args):
             return -3
 
+        @staticmethod
+        def wn_winding_gpu_vf(*args):
+            return -3
+
     monkeypatch.setattr(igl, "_gpu_memory_available", lambda: True)
     monkeypatch.setattr(igl, "lib", lambda: FailedGpu())
     with pytest.raises(RuntimeError, match="GPU"):
@@ -238,3 +242,14 @@ def test_new_entries_validate_arguments_before_the_device():
     assert h.wn_boundary_edges(addr(bad), 4, 4, 0, 0, 0, 0) == -1
     assert "outside V" in _lib.last_error()
     assert h.wn_boundary_edges(addr(F), 2**31, 4, 0, 0, 0, 0) == -1
+    # host V/F forward: index range (int64 and int32 faces), face dtype, null mesh
+    assert h.wn_winding_gpu_vf(addr(V), addr(bad), 4, 4, 0, addr(Q), addr(out), 2, 0) == -1
+    assert "face 0 has a corner outside V" in _lib.last_error()
+    bad32 = np.ascontiguousarray(F, dtype=np.int32)
+    bad32[3, 2] = -1
+    assert h.wn_winding_gpu_vf(addr(V), addr(bad32), 4, 4, 1, addr(Q), addr(out), 2, 0) == -1
+    assert "face 3 has a corner outside V" in _lib.last_error()
+    assert h.wn_winding_gpu_vf(addr(V), addr(F), 4, 4, 2, addr(Q), addr(out), 2, 0) == -1
+    assert "fdtype" in _lib.last_error()
+    assert h.wn_winding_gpu_vf(0, 0, 4, 4, 0, addr(Q), addr(out), 2, 0) == -1
+    assert h.wn_winding_gpu_vf(addr(V), addr(F), 4, 4, 0, addr(Q), addr(out), 0, 0) == 0

@@ -293,3 +293,42 @@ def test_large_calls_use_strip_kernels(require_gpu, monkeypatch):
         igl.winding_number(V, F + len(V), Q, device="gpu")
     F32 = F.astype(np.int32)
     assert np.array_equal(igl.winding_number(V, F32, Q, device="gpu"), w64)
+
+
+def test_vf_entry_matches_host_packing(require_gpu, c1, c2):
+    """wn_winding_gpu_vf (V and F uploaded, face SoA gathered on the device;
+    the path of igl.winding_number below the strip threshold) equals the
+    host-packed entries bit for bit: mli_winding_gpu for FP64,
+    wn_winding_gpu_ex with default tuning for FP32c; int32 faces give the same
+    bits; an empty F gives zeros; and igl.winding_number returns the same."""
+    from igl._lib import addr
+
+    h = igl.lib()
+    for V, F, Q, tri, ref in (c1, c2):
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        tri = np.ascontiguousarray(tri)
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        for precision in (0, 1):
+            out = np.full(len(Q), np.nan)
+            for fdtype, Fc in ((0, np.ascontiguousarray(F, dtype=np.int64)),
+                               (1, np.ascontiguousarray(F, dtype=np.int32))):
+                assert h.wn_winding_gpu_vf(addr(V), addr(Fc), len(V), len(Fc), fdtype, addr(Q),
+                                           addr(out), len(Q), precision) == 0, igl._lib.last_error()
+                if precision == 0:
+                    packed = np.empty(len(Q))
+                    assert h.mli_winding_gpu(addr(tri), addr(Q), addr(packed), tri.shape[1],
+                                             len(Q)) == 1
+                else:
+                    packed = host_call(tri, Q, 1)
+                assert np.array_equal(out, packed)
+            bar = FP64_TOL if precision == 0 else FP32C_TOL
+            assert np.abs(out - ref).max() <= bar
+            w = igl.winding_number(V, F, Q, device="gpu",
+                                   precision="fp64" if precision == 0 else "fp32c")
+            assert np.array_equal(w, out)
+    V, F, Q, tri, ref = c1
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F0 = np.zeros((0, 3), dtype=np.int64)
+    out = np.full(len(Q), np.nan)
+    assert h.wn_winding_gpu_vf(addr(V), addr(F0), len(V), 0, 0, addr(Q), addr(out), len(Q), 0) == 0
+    assert np.array_equal(out, np.zeros(len(Q)))
