@@ -191,11 +191,21 @@ int64_t wn_mesh_destroy(int64_t mesh);
  * is mli_winding_gpu above): host V (nv x 3 float64) and F (nf x 3, fdtype 0
  * = int64, 1 = int32) are stripified and uploaded, (host) q (nq x 3) runs
  * through the FP64 (precision 0) or FP32-compensated (1) strip kernel into
- * (host) dst (nq). Synchronous; allocates and frees its own device memory.
- * Returns 0 or a negative status. */
+ * (host) dst (nq). Synchronous; returns 0 or a negative status. The strip
+ * mesh of the last call stays on the device with host copies of its V and F:
+ * a call with byte-identical F reuses its topology (no stripifier, no
+ * upload), and with byte-identical V too its records; results are those of a
+ * fresh mesh, bit for bit. wn_release_host_buffers frees it. */
 int64_t wn_winding_gpu_mesh(const double *V, const void *F, int64_t nv, int64_t nf,
                             int64_t fdtype, const double *q, double *dst, int64_t nq,
                             int64_t precision);
+
+/* No reference counterpart (the reference frees everything per call,
+ * src/capi.mojo:366-398). Frees the device memory the host-buffer entries
+ * keep between calls (mli_winding_gpu, wn_winding_gpu_ex, wn_winding_gpu_vf:
+ * the grow-only buffer; wn_winding_gpu_mesh: also the kept strip mesh).
+ * The next call allocates again. Returns 0. */
+int64_t wn_release_host_buffers(void);
 
 /* ---- Batched forward ------------------------------------------------------ */
 

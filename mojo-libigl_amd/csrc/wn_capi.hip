@@ -21,6 +21,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <vector>
 
 #include "wn_internal.h"
 
@@ -58,8 +59,20 @@ int device_count() {
   return n;
 }
 
+// The strip mesh kept by the host-buffer strip path (wn_winding_gpu_mesh):
+// its handle and the host bytes of the V and F it was built from.
+struct HostStripCache {
+  int64_t mesh = 0;
+  int fdtype = 0;
+  int64_t nv = 0;
+  int64_t nf = 0;
+  std::vector<char> F;
+  std::vector<double> V;
+};
+
 struct HostPathContext {
   std::mutex mu;
+  HostStripCache strips;
   bool ready = false;
   int device = -1;
   hipStream_t stream = nullptr;
@@ -833,76 +846,127 @@ int64_t wn_mesh_destroy(int64_t mesh) {
   return kOk;
 }
 
-// Host-buffer strip path: a temporary mesh (host stripifier, upload), then
-// the host path's stream and grow-only buffer for queries, output and
-// workspace. Same contract as wn_winding_gpu_ex: synchronous, returns 0 or a
-// negative status.
+// Host-buffer strip path. The strip mesh of the last call is kept on the
+// device (HostStripCache, under the host path's mutex) with host copies of
+// the V and F it was built from: a call whose F is byte-for-byte the cached
+// one skips the host stripifier and the upload (the topology is reused), and
+// when V differs only the records are regathered on the device from the new
+// V (wn_mesh_set_vertices). The comparison is exact (memcmp), so a hit gives
+// the results a fresh mesh would, bit for bit. The queries, output and
+// workspace use the host path's stream and grow-only buffer. Same contract
+// as wn_winding_gpu_ex: synchronous, returns 0 or a negative status.
 int64_t wn_winding_gpu_mesh(const double* V, const void* F, int64_t nv, int64_t nf,
                             int64_t fdtype, const double* q, double* dst, int64_t nq,
                             int64_t precision) {
   if (precision != wn::kFP64 && precision != wn::kFP32C)
     return fail(kErrArg, "precision must be 0 (FP64) or 1 (FP32-compensated)");
-  if (nq < 0) return fail(kErrArg, "negative size");
+  if (nq < 0 || nv < 0 || nf < 0) return fail(kErrArg, "negative size");
   if (nq == 0) return kOk;
   if (!q || !dst) return fail(kErrArg, "null query or output buffer");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if (nf > 0 && (!V || !F)) return fail(kErrArg, "null host buffer");
+  HostPathContext& ctx = host_ctx();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  HostStripCache& cache = ctx.strips;
+  const size_t f_bytes = (size_t)nf * 3 * (fdtype == 0 ? 8 : 4);
+  const size_t v_bytes = (size_t)nv * 3 * sizeof(double);
+  if (cache.mesh != 0 && !lookup_mesh(cache.mesh)) cache = HostStripCache();  // destroyed elsewhere
+  const bool same_f = cache.mesh != 0 && cache.fdtype == fdtype && cache.nv == nv &&
+                      cache.nf == nf && (f_bytes == 0 || std::memcmp(cache.F.data(), F, f_bytes) == 0);
+  const bool same_v = same_f && (v_bytes == 0 || std::memcmp(cache.V.data(), V, v_bytes) == 0);
+  if (!same_f) {
+    if (cache.mesh != 0) (void)wn_mesh_destroy(cache.mesh);
+    cache = HostStripCache();
+    const int64_t mesh = wn_mesh_create(V, F, nv, nf, fdtype);  // validates F against nv
+    if (mesh < 0) return mesh;
+    cache.mesh = mesh;
+    cache.fdtype = (int)fdtype;
+    cache.nv = nv;
+    cache.nf = nf;
+    cache.F.assign((const char*)F, (const char*)F + f_bytes);
+    cache.V.assign(V, V + 3 * nv);
+  }
+  const int64_t mesh = cache.mesh;
   int64_t cfg[2];
-  const int64_t mesh = wn_mesh_create(V, F, nv, nf, fdtype);
-  if (mesh < 0) return mesh;
   int64_t st = wn_mesh_resolve(mesh, nq, 0, -1, cfg);
-  if (st == kOk) {
-    MeshHandle* m = lookup_mesh(mesh);
-    HostPathContext& ctx = host_ctx();
-    std::lock_guard<std::mutex> lock(ctx.mu);
-    hipError_t e = hipSuccess;
-    if (!ctx.ready) {
-      e = hipGetDevice(&ctx.device);
-      if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking);
-      if (e == hipSuccess) ctx.ready = true;
+  if (st != kOk) return st;
+  MeshHandle* m = lookup_mesh(mesh);
+  hipError_t e = hipSuccess;
+  if (!ctx.ready) {
+    e = hipGetDevice(&ctx.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) ctx.ready = true;
+  }
+  const size_t q_b = wn::align_up((size_t)nq * 3 * sizeof(double));
+  const size_t dst_b = wn::align_up((size_t)nq * sizeof(double));
+  const size_t ws_b = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+  const size_t v_b = same_v ? 0 : wn::align_up(v_bytes);
+  const size_t need = q_b + dst_b + ws_b + v_b;
+  if (e == hipSuccess && need > ctx.cap) {
+    if (ctx.buf) {
+      (void)hipStreamSynchronize(ctx.stream);
+      (void)hipFree(ctx.buf);
+      ctx.buf = nullptr;
+      ctx.cap = 0;
     }
-    const size_t q_b = wn::align_up((size_t)nq * 3 * sizeof(double));
-    const size_t dst_b = wn::align_up((size_t)nq * sizeof(double));
-    const size_t ws_b = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
-    const size_t need = q_b + dst_b + ws_b;
-    if (e == hipSuccess && need > ctx.cap) {
-      if (ctx.buf) {
-        (void)hipStreamSynchronize(ctx.stream);
-        (void)hipFree(ctx.buf);
-        ctx.buf = nullptr;
-        ctx.cap = 0;
-      }
-      e = hipMalloc(&ctx.buf, need);
-      if (e == hipSuccess)
-        ctx.cap = need;
-      else
-        ctx.buf = nullptr;
+    e = hipMalloc(&ctx.buf, need);
+    if (e == hipSuccess)
+      ctx.cap = need;
+    else
+      ctx.buf = nullptr;
+  }
+  if (e == hipSuccess) {
+    char* base = (char*)ctx.buf;
+    double* d_q = (double*)base;
+    double* d_dst = (double*)(base + q_b);
+    void* d_ws = base + q_b + dst_b;
+    if (same_f && !same_v && m->nrec > 0) {
+      // new vertices for the cached topology: upload V, regather the records
+      double* d_v = (double*)(base + q_b + dst_b + ws_b);
+      e = hipMemcpyAsync(d_v, V, v_bytes, hipMemcpyHostToDevice, ctx.stream);
+      if (e == hipSuccess && wn_mesh_set_vertices(mesh, d_v, nv, 0, ctx.stream) != kOk)
+        e = hipErrorUnknown;
+      if (e == hipSuccess) cache.V.assign(V, V + 3 * nv);
     }
-    if (e == hipSuccess) {
-      char* base = (char*)ctx.buf;
-      double* d_q = (double*)base;
-      double* d_dst = (double*)(base + q_b);
-      void* d_ws = base + q_b + dst_b;
+    if (e == hipSuccess)
       e = hipMemcpyAsync(d_q, q, (size_t)nq * 3 * sizeof(double), hipMemcpyHostToDevice,
                          ctx.stream);
-      if (e == hipSuccess) {
-        if (precision == wn::kFP64)
-          e = wn::strip64_launch(m->d_rec64, (const double4*)m->d_rec, m->nrec, d_q, d_dst, nq,
-                                 (int)cfg[0], (int)cfg[1], d_ws, ws_b, ctx.stream);
-        else
-          e = wn::strip_launch((const double4*)m->d_rec, m->nrec, d_q, d_dst, nq, (int)cfg[0],
-                               (int)cfg[1], 1.0f, d_ws, ws_b, ctx.stream);
-      }
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(dst, d_dst, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost,
-                           ctx.stream);
-      const hipError_t es = hipStreamSynchronize(ctx.stream);
-      if (e == hipSuccess) e = es;
+    if (e == hipSuccess) {
+      if (precision == wn::kFP64)
+        e = wn::strip64_launch(m->d_rec64, (const double4*)m->d_rec, m->nrec, d_q, d_dst, nq,
+                               (int)cfg[0], (int)cfg[1], d_ws, ws_b, ctx.stream);
+      else
+        e = wn::strip_launch((const double4*)m->d_rec, m->nrec, d_q, d_dst, nq, (int)cfg[0],
+                             (int)cfg[1], 1.0f, d_ws, ws_b, ctx.stream);
     }
-    if (e != hipSuccess) st = fail(kErrHip, hip_msg("wn_winding_gpu_mesh", e));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dst, d_dst, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx.stream);
+    const hipError_t es = hipStreamSynchronize(ctx.stream);
+    if (e == hipSuccess) e = es;
   }
-  const std::string why = st == kOk ? std::string() : g_error;
-  (void)wn_mesh_destroy(mesh);
-  if (st != kOk) set_error(why);
-  return st;
+  if (e != hipSuccess) {
+    // drop the cached mesh: its records may be half updated
+    const std::string why = hip_msg("wn_winding_gpu_mesh", e);
+    (void)wn_mesh_destroy(cache.mesh);
+    cache = HostStripCache();
+    return fail(kErrHip, why);
+  }
+  return kOk;
+}
+
+int64_t wn_release_host_buffers(void) {
+  HostPathContext& ctx = host_ctx();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  if (ctx.strips.mesh != 0) (void)wn_mesh_destroy(ctx.strips.mesh);
+  ctx.strips = HostStripCache();
+  if (ctx.buf) {
+    (void)hipStreamSynchronize(ctx.stream);
+    (void)hipFree(ctx.buf);
+    ctx.buf = nullptr;
+    ctx.cap = 0;
+  }
+  return kOk;
 }
 
 int64_t wn_scatter_corners(const double* grad_tri, const void* F, int64_t fdtype, int64_t nf,

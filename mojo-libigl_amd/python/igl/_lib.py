@@ -53,6 +53,7 @@ _SIGNATURES = {
     "wn_mesh_forward": ([I] * 6 + [D] + [I] * 3, I),
     "wn_mesh_forward64": ([I] * 9, I),
     "wn_winding_gpu_mesh": ([I] * 9, I),
+    "wn_release_host_buffers": ([], I),
     "wn_mesh_destroy": ([I], I),
     "wn_batch_create": ([I] * 7 + [D], I),
     "wn_batch_create_strips": ([I] * 5 + [D], I),

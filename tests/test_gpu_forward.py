@@ -332,3 +332,39 @@ def test_vf_entry_matches_host_packing(require_gpu, c1, c2):
     out = np.full(len(Q), np.nan)
     assert h.wn_winding_gpu_vf(addr(V), addr(F0), len(V), 0, 0, addr(Q), addr(out), len(Q), 0) == 0
     assert np.array_equal(out, np.zeros(len(Q)))
+
+
+def test_strip_route_keeps_the_last_mesh(require_gpu, monkeypatch):
+    """wn_winding_gpu_mesh keeps the last strip mesh on the device: repeated
+    calls (hit), the same F with new V (records regathered on the device), a
+    different F, int32 faces and meshes interleaved all give the bits of a
+    freshly built mesh (wn_release_host_buffers before the call), in FP64
+    and FP32c."""
+    from igl._lib import addr
+
+    h = igl.lib()
+    V, F, Q = W.config2(4096)
+    V1, F1, _ = W.config1()
+    monkeypatch.setattr(igl, "STRIP_MIN_QUERIES", {0: 1000, 1: 1000})
+
+    def fresh(Vm, Fm, precision):
+        assert h.wn_release_host_buffers() == 0
+        return igl.winding_number(Vm, Fm, Q, device="gpu", precision=precision)
+
+    cases = [(V, F), (V, F), (V * 1.25 + [0.1, 0.0, -0.2], F), (V1, F1), (V, F),
+             (V, F.astype(np.int32)), (V1 + [0.05, 0.05, 0.05], F1), (V, F)]
+    for precision in ("fp64", "fp32c"):
+        expect = [fresh(Vm, Fm, precision) for Vm, Fm in cases]
+        assert h.wn_release_host_buffers() == 0
+        for (Vm, Fm), e in zip(cases, expect):
+            w = igl.winding_number(Vm, Fm, Q, device="gpu", precision=precision)
+            assert np.array_equal(w, e)
+    # the moved mesh agrees with the plain FP64 kernel (the regathered records are right)
+    Vm = V * 1.25 + [0.1, 0.0, -0.2]
+    igl.winding_number(V, F, Q, device="gpu")
+    w = igl.winding_number(Vm, F, Q, device="gpu")
+    tri = igl.pack_faces(np.asarray(Vm, dtype=np.float64), F)
+    plain = np.empty(len(Q))
+    assert h.mli_winding_gpu(addr(tri), addr(Q), addr(plain), tri.shape[1], len(Q)) == 1
+    assert np.abs(w - plain).max() <= 1e-12
+    assert h.wn_release_host_buffers() == 0
