@@ -130,9 +130,17 @@ def _soup(seed, nf=500):
     return V, F
 
 
-@pytest.mark.parametrize("case", ["grid8", "grid60", "soup", "soup_int32", "one_face"])
+@pytest.mark.parametrize("case", ["grid8", "grid60", "soup", "soup_int32", "one_face",
+                                  "large_ties"])
 def test_tree_builder_equals_reference_builder(case):
-    if case == "grid8":
+    """Same boxes, nodes, order and stack depth as the reference's builder;
+    "large_ties" (140k faces on integer coordinates: many equal centres) runs
+    the multi-threaded sorts of the top levels."""
+    if case == "large_ties":
+        rng = np.random.default_rng(9)
+        V = np.round(rng.uniform(-20, 20, (4000, 3)))
+        F = rng.integers(0, 4000, (140000, 3)).astype(np.int64)
+    elif case == "grid8":
         V, F = grid(8)
     elif case == "grid60":
         V, F = grid(60)
