@@ -121,22 +121,31 @@ void fwn_build(const double* V, const void* F, int fdtype, int64_t nf, int64_t l
   out.order = t.order;
   out.stack_size = t.stack_size;
   out.corners.resize(9 * (size_t)nf);
-  for (int64_t k = 0; k < nf; ++k) {
-    const int64_t f = t.order[(size_t)k];
-    for (int c = 0; c < 3; ++c) {
-      const int64_t v = fdtype == 0 ? ((const int64_t*)F)[3 * f + c]
-                                    : (int64_t)((const int32_t*)F)[3 * f + c];
-      for (int d = 0; d < 3; ++d) out.corners[(size_t)(9 * k + 3 * c + d)] = V[3 * v + d];
+  // faces and nodes are independent: up to 16 threads
+  const int64_t want = std::min<int64_t>(16, std::max<int64_t>(1, nf / 32768));
+  const int workers = (int)std::min<int64_t>(
+      want, std::max<unsigned>(1u, std::thread::hardware_concurrency()));
+  auto gather = [&](int w) {
+    for (int64_t k = nf * w / workers; k < nf * (w + 1) / workers; ++k) {
+      const int64_t f = t.order[(size_t)k];
+      for (int c = 0; c < 3; ++c) {
+        const int64_t v = fdtype == 0 ? ((const int64_t*)F)[3 * f + c]
+                                      : (int64_t)((const int32_t*)F)[3 * f + c];
+        for (int d = 0; d < 3; ++d) out.corners[(size_t)(9 * k + 3 * c + d)] = V[3 * v + d];
+      }
     }
+  };
+  {
+    std::vector<std::thread> pool;
+    for (int w = 1; w < workers; ++w) pool.emplace_back(gather, w);
+    gather(0);
+    for (auto& th : pool) th.join();
   }
   std::vector<int64_t> lo((size_t)nn), hi((size_t)nn);
   node_ranges(t, 0, lo, hi);
   out.nodes.resize((size_t)nn);
-  // nodes are independent: threads take them round-robin (every level holds
-  // all faces once, so the work interleaves evenly)
-  const int64_t want = std::min<int64_t>(16, std::max<int64_t>(1, nf / 32768));
-  const int workers = (int)std::min<int64_t>(
-      want, std::max<unsigned>(1u, std::thread::hardware_concurrency()));
+  // nodes: threads take them round-robin (every level holds all faces once,
+  // so the work interleaves evenly)
   auto run = [&](int w) {
     for (int64_t n = w; n < nn; n += workers) {
       FwnNode& e = out.nodes[(size_t)n];
