@@ -131,15 +131,14 @@ __device__ __forceinline__ float atan_small_f32(float r) {
   return r * p;
 }
 
-// atan(x / 4) for |x| <= 1: atan_small_f32 with its coefficients scaled by
-// powers of two, so atan_small_x4_f32(4 r) equals atan_small_f32(r) bit for
-// bit (every product and sum is the same value times an exact power of two).
-__device__ __forceinline__ float atan_small_x4_f32(float x) {
+// atan(x / 8) for |x| <= 1: the odd minimax polynomial of 3 terms for
+// atan(r) on |r| <= 1/8 (max relative error 1.67e-8, gen_atan_coeffs.py),
+// with its coefficients scaled by powers of two so that it takes x = 8 r.
+__device__ __forceinline__ float atan_small_x8_f32(float x) {
   const float t = x * x;
-  float p = -0.12990966888801511f * (1.0f / 16384.0f);
-  p = __builtin_fmaf(p, t, 0.19950626010644533f * (1.0f / 1024.0f));
-  p = __builtin_fmaf(p, t, -0.33332723769728184f * (1.0f / 64.0f));
-  p = __builtin_fmaf(p, t, 0.99999998819005977f * 0.25f);
+  float p = 0.19669884933950198f * (1.0f / 32768.0f);
+  p = __builtin_fmaf(p, t, -0.33331408377741376f * (1.0f / 512.0f));
+  p = __builtin_fmaf(p, t, 0.99999998333354555f * 0.125f);
   return x * p;
 }
 
@@ -233,13 +232,12 @@ __device__ __forceinline__ bool wave_all_far(float det2, float den2, float s2) {
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }
 
-// wave_all_far for det4 = 4 det, den and dp = den - la lb lc (any common
-// power-of-two scale of det, den and dp): |det4| < den is
-// |det| < kFarTau den (kFarTau = 1/4) and dp > 0 is den > kFarKappa la lb lc
-// (kFarKappa = 1), neither needing a multiply.
-__device__ __forceinline__ bool wave_all_far_x4(float det4, float den, float dp) {
-  static_assert(kFarTau == 0.25f && kFarKappa == 1.0f, "x4 form assumes tau 1/4, kappa 1");
-  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det4) < den);
+// Far test of the strip kernel, for det8 = 8 det, den and dp = den - la lb
+// lc: every active lane has |det8| < den (|det / den| < 1/8, the range of
+// atan_small_x8_f32) and dp > 0 (den > la lb lc, away from every edge line).
+// Each compare is balloted on its own so both masks stay in SGPRs.
+__device__ __forceinline__ bool wave_all_far_x8(float det8, float den, float dp) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det8) < den);
   const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp > 0.0f);
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }

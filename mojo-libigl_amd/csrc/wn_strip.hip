@@ -3,7 +3,9 @@
 //
 // Same arithmetic as fwd_f32c_kernel (block-local origin, FP32 pairs, FP64
 // accumulation, far-pair fast path, FP64 re-evaluation of ill-conditioned
-// pairs), except that the angles of three consecutive records are summed in
+// pairs), except that the fast path takes |det / den| < 1/8 (a 3-term
+// arctangent, one fma fewer than the face kernel's 4-term one on 1/4, at the
+// same relative accuracy), the angles of three consecutive records are summed in
 // FP32 before each FP64 add (error at most 2 eps times the three angles'
 // magnitudes, the bound of a compensated sum, with a third of the FP64
 // instructions), and the re-evaluated angle is rounded to FP32 once before
@@ -16,14 +18,14 @@
 // LDS tile (255 records, a multiple of 3 so the three register slots rotate
 // statically through a 3-way unrolled loop):
 //   R0[j] = (w.x, w.y, w.z, -|w_j - w_{j-1}|^2 / 2)  (block-local FP32 coords)
-//   R1[j] = (-|w_j - w_{j-2}|^2 / 2, N4.x, N4.y, N4.z)  (N4 = 4 x face normal)
+//   R1[j] = (-|w_j - w_{j-2}|^2 / 2, N8.x, N8.y, N8.z)  (N8 = 8 x face normal)
 // so that a dot product is one add and one fma with 0.5 (law of cosines),
-// den is formed undoubled, and det4 = 4 det needs no multiply for the far
-// test |det| < den / 4 (all scalings powers of two: the values are those of
-// the doubled forms, halved exactly).
+// den is formed undoubled, and det8 = 8 det needs no multiply for the far
+// test |det| < den / 8 (all scalings powers of two: the values are those of
+// the doubled forms, scaled exactly).
 // The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
-// orientation vs strip order) is folded into N4; a record with flag 0 is
-// staged with N4 = 0 and R1.x = +inf, which makes its angle 0 without a
+// orientation vs strip order) is folded into N8; a record with flag 0 is
+// staged with N8 = 0 and R1.x = +inf, which makes its angle 0 without a
 // branch. The flag itself is read (from the records, at a wave-uniform
 // address) only on the rare slow path.
 // Squared lengths and N2 are computed in FP64 from the rounded FP32 corners.
@@ -62,16 +64,16 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   const double d2x = (double)cx - qx, d2y = (double)cy - qy, d2z = (double)cz - qz;
   const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
   // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag; stored
-  // as 4 N so that the lane's dot product is 4 det
+  // as 8 N so that the lane's dot product is 8 det
   // A record that closes no face (flag 0) gets N = 0 and +inf in place of
   // -|w_k - w_{k-2}|^2 / 2: c.a and with it dp and den are then +inf, and
-  // its fast-path angle is 4 det / den = 0 / inf = 0 exactly (see strip_step).
-  const double f4 = 4.0 * c.w;
+  // its fast-path angle is 8 det / den = 0 / inf = 0 exactly (see strip_step).
+  const double f8 = 8.0 * c.w;
   sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
   sm.r1[j] = make_float4(c.w == 0.0 ? __builtin_inff()
                                     : (float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
-                         (float)(f4 * (ey * d2z - ez * d2y)), (float)(f4 * (ez * d2x - ex * d2z)),
-                         (float)(f4 * (ex * d2y - ey * d2x)));
+                         (float)(f8 * (ey * d2z - ez * d2y)), (float)(f8 * (ez * d2x - ex * d2z)),
+                         (float)(f8 * (ex * d2y - ey * d2x)));
 }
 
 // FP64 re-evaluation of face (j-2, j-1, j) from the LDS corners, oriented.
@@ -98,10 +100,10 @@ __device__ __forceinline__ void strip_vertex(const float4& r0, float ox, float o
 
 // Face (a, b, c) = (P, Q, R) closed by a record with tile data r0, r1; a.b
 // = s_carry (carried from the previous record), b.c = s_qr (carried to the
-// next), c.a from r1.x (law of cosines). det4 = 4 det (the normal is stored
-// scaled by 4); den = la lb lc + dp with dp = lc a.b + la b.c + lb c.a.
+// next), c.a from r1.x (law of cosines). det8 = 8 det (the normal is stored
+// scaled by 8); den = la lb lc + dp with dp = lc a.b + la b.c + lb c.a.
 struct StripGeom {
-  float det4, den, dp, pq, s_qr;
+  float det8, den, dp, pq, s_qr;
 };
 
 __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& r1, const VSlot& P,
@@ -109,7 +111,7 @@ __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& 
   StripGeom g;
   g.s_qr = __builtin_fmaf(Q.l2 + R.l2, 0.5f, r0.w);
   const float s_pr = __builtin_fmaf(P.l2 + R.l2, 0.5f, r1.x);
-  g.det4 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
+  g.det8 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
   g.pq = P.l * Q.l;
   g.dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, g.s_qr, Q.l * s_pr));
   g.den = __builtin_fmaf(g.pq, R.l, g.dp);
@@ -120,7 +122,7 @@ __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& 
 // records, R computed here); its angle is added to part, the FP32 sum of at
 // most three angles that the caller adds to the FP64 sum. Every record takes
 // the same instructions: one that closes no face (flag 0: the first two of a
-// strip, tile padding) was staged (stage_strip_tile) with N4 = 0 and c.a =
+// strip, tile padding) was staged (stage_strip_tile) with N8 = 0 and c.a =
 // +inf, so dp = den = +inf and its fast-path angle is 0 / inf = 0 exactly.
 // Its flag (raw bits of the record's w; read from global memory at a
 // wave-uniform address) is looked at only on the slow path, the one place
@@ -135,15 +137,17 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm,
   strip_vertex(r0, ox, oy, oz, R);
   const float4 r1 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r1) + boff);
   const StripGeom g = strip_geom(r0, r1, P, Q, R, s_carry);
-  float ang = atan_small_x4_f32(g.det4 * __builtin_amdgcn_rcpf(g.den));
-  if (__builtin_expect(!wave_all_far_x4(g.det4, g.den, g.dp), 0)) {
+  // far pairs (|det / den| < 1/8 and den > la lb lc on every lane): the
+  // small-angle arctangent
+  float ang = atan_small_x8_f32(g.det8 * __builtin_amdgcn_rcpf(g.den));
+  if (__builtin_expect(!wave_all_far_x8(g.det8, g.den, g.dp), 0)) {
     asm volatile("" ::);  // keep a real wave-uniform branch
     const long long flag_bits = flags[4 * j];
     if (flag_bits == 0) {
       ang = 0.0f;
     } else {
       float mx;
-      ang = atan2_f32(0.25f * g.det4, g.den, mx);
+      ang = atan2_f32(0.125f * g.det8, g.den, mx);
       // ill-conditioned (mx < kappa x la lb lc): the FP64 angle, rounded
       // once, replaces the FP32 one. Flag -1.0 has the sign bit set: the
       // face runs against the strip order.
