@@ -20,7 +20,7 @@ constexpr int kStripTileRecords = 255;  // strip records per tile (strips cut at
 
 // FP64 sweep record of strip record k (wn_strip64.hip), built on the device
 // from the records: its vertex, the squared lengths to the previous two
-// vertices of its tile, and 8 x the normal of the face it closes, oriented
+// vertices of its tile, and 16 x the normal of the face it closes, oriented
 // by its flag (zero when it closes none).
 struct alignas(64) StripRec64 {
   double x, y, z, e1, e2, nx, ny, nz;

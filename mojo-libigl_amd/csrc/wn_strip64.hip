@@ -6,9 +6,9 @@
 // the doubled dot products from the law of cosines with the squared edge
 // lengths of the sweep record, and det = N . c with the face normal of the
 // sweep record (exact: N . (a - c) = 0). A wave whose 64 pairs are all far
-// (|det| < den / 4 and den > 2 la lb lc, the FP32 strip kernel's test) takes
-// the angle from a 9-term Remez polynomial of det / den (relative error
-// 4.5e-18 on |x| <= 1/4, below FP64 rounding; gen_atan_coeffs.py, n = 9);
+// (|det| < den / 8 and den > la lb lc, the FP32 strip kernel's test) takes
+// the angle from a 7-term Remez polynomial of det / den (relative error
+// 1.8e-18 on |x| <= 1/8, below FP64 rounding; gen_atan_coeffs.py, n = 7);
 // any other wave evaluates every lane's pair with the plain FP64 formula
 // (pair_f64, the plain kernel's arithmetic). Square roots and the quotient
 // are refined from the hardware estimates to FP64 rounding (Goldschmidt for
@@ -42,16 +42,16 @@ __global__ __launch_bounds__(kBlockThreads) void strip64_prep_kernel(
   const double d1x = c.x - p.x, d1y = c.y - p.y, d1z = c.z - p.z;
   const double d2x = c.x - pp.x, d2y = c.y - pp.y, d2z = c.z - pp.z;
   const double ex = p.x - pp.x, ey = p.y - pp.y, ez = p.z - pp.z;
-  const double f8 = 8.0 * c.w;  // orientation flag (0 = no face), times 8
+  const double f16 = 16.0 * c.w;  // orientation flag (0 = no face), times 16
   StripRec64 r;
   r.x = c.x;
   r.y = c.y;
   r.z = c.z;
   r.e1 = d1x * d1x + d1y * d1y + d1z * d1z;
   r.e2 = d2x * d2x + d2y * d2y + d2z * d2z;
-  r.nx = f8 * (ey * d2z - ez * d2y);
-  r.ny = f8 * (ez * d2x - ex * d2z);
-  r.nz = f8 * (ex * d2y - ey * d2x);
+  r.nx = f16 * (ey * d2z - ez * d2y);
+  r.ny = f16 * (ez * d2x - ex * d2z);
+  r.nz = f16 * (ex * d2y - ey * d2x);
   out[k] = r;
 }
 
@@ -81,14 +81,14 @@ __device__ __forceinline__ double div_f64(double n, double d) {
   return __builtin_fma(__builtin_fma(-d, q, n), r, q);
 }
 
-// atan(x / 4) for |x| <= 1: x * P(x^2) with the n = 9 relative-error
-// minimax coefficients of atan on (0, 1/4], each divided by 4^(2k+1)
+// atan(x / 8) for |x| <= 1: x * P(x^2) with the n = 7 relative-error
+// minimax coefficients of atan on (0, 1/8], each divided by 8^(2k+1)
 // (powers of two: exact). The Horner steps are one asm block of VOP3
 // v_fma_f64 with the coefficients in scalar registers: compiled from C, the
 // coefficients stay in VGPRs and each step copies one into a v_fmac_f64
-// destination (7 extra FP64 moves per pair), and one asm block per step
-// costs a wait state after each.
-__device__ __forceinline__ double atan_small_x4_f64(double x) {
+// destination (extra FP64 moves per pair), and one asm block per step costs
+// a wait state after each.
+__device__ __forceinline__ double atan_small_x8_f64(double x) {
   const double t = x * x;
   double p;
   asm("v_fma_f64 %0, %2, %1, %3\n\t"
@@ -96,19 +96,15 @@ __device__ __forceinline__ double atan_small_x4_f64(double x) {
       "v_fma_f64 %0, %0, %1, %5\n\t"
       "v_fma_f64 %0, %0, %1, %6\n\t"
       "v_fma_f64 %0, %0, %1, %7\n\t"
-      "v_fma_f64 %0, %0, %1, %8\n\t"
-      "v_fma_f64 %0, %0, %1, %9\n\t"
-      "v_fma_f64 %0, %0, %1, %10"
+      "v_fma_f64 %0, %0, %1, %8"
       : "=&v"(p)
-      : "v"(t), "v"(0.0459998373611164223 / 17179869184.0),  // x^17: 4^17
-        "s"(-0.065237123716060834312 / 1073741824.0),          // x^15: 4^15
-        "s"(0.076834985777981783012 / 67108864.0),             // x^13: 4^13
-        "s"(-0.090905895478190080677 / 4194304.0),             // x^11: 4^11
-        "s"(0.11111104268448408897 / 262144.0),                // x^9: 4^9
-        "s"(-0.14285714203175044227 / 16384.0),                // x^7: 4^7
-        "s"(0.19999999999500416733 / 1024.0),                  // x^5: 4^5
-        "s"(-0.33333333333332167915 / 64.0),                   // x^3: 4^3
-        "s"(0.99999999999999999552 / 4.0));                    // x^1: 4
+      : "v"(t), "v"(0.073382651424641993073 / 549755813888.0),  // x^13: 8^13
+        "s"(-0.090833776841241786793 / 8589934592.0),             // x^11: 8^11
+        "s"(0.11111031276417376124 / 134217728.0),                // x^9: 8^9
+        "s"(-0.14285713850448952581 / 2097152.0),                 // x^7: 8^7
+        "s"(0.19999999998868866968 / 32768.0),                    // x^5: 8^5
+        "s"(-0.33333333333332230442 / 512.0),                     // x^3: 8^3
+        "s"(0.99999999999999999824 / 8.0));                       // x^1: 8
   return x * p;
 }
 
@@ -121,8 +117,10 @@ __device__ __noinline__ double strip64_pair(double ax, double ay, double az, dou
   return flip ? -v : v;
 }
 
-__device__ __forceinline__ bool wave_all_far_x4_f64(double det8, double den2, double dp) {
-  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabs(det8) < den2);
+// |det16| < den2 is |det / den| < 1/8 (det16 = 16 det, den2 = 2 den); dp > 0
+// is den > la lb lc.
+__device__ __forceinline__ bool wave_all_far_x8_f64(double det16, double den2, double dp) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabs(det16) < den2);
   const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp > 0.0);
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }
@@ -144,12 +142,12 @@ __device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag
   const double s_qr = (Q.l2 + R.l2) - r.e1;  // 2 b.c
   if (flag_bits != 0) {
     const double s_pr = (P.l2 + R.l2) - r.e2;  // 2 c.a
-    const double det8 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
+    const double det16 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
     const double pq = P.l * Q.l;
     const double dp = __builtin_fma(R.l, s_carry, __builtin_fma(P.l, s_qr, Q.l * s_pr));
     const double den2 = __builtin_fma(pq + pq, R.l, dp);
-    if (__builtin_expect(wave_all_far_x4_f64(det8, den2, dp), 1)) {
-      acc += atan_small_x4_f64(div_f64(det8, den2));
+    if (__builtin_expect(wave_all_far_x8_f64(det16, den2, dp), 1)) {
+      acc += atan_small_x8_f64(div_f64(det16, den2));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       // flag -1.0 has the sign bit set: the face runs against the strip order
