@@ -20,6 +20,8 @@ w is float64 and gradients come back in the dtype of their input.
 
 from __future__ import annotations
 
+import threading
+
 import numpy as np
 import torch
 
@@ -202,8 +204,68 @@ class WindingNumber(torch.autograd.Function):
 def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision="fp64",
                    adjoint="edges"):
     """Differentiable generalized winding number of (V, F) at the rows of O.
-    ``adjoint`` picks the backward: "edges" (boundary edges only) or "faces"."""
+    ``adjoint`` picks the backward: "edges" (boundary edges only) or "faces".
+
+    Calls with at least igl.STRIP_MIN_QUERIES[precision] queries run on the
+    strip kernels through a WindingField that is kept for the next call with
+    an equal F (vertex count, precision and adjoint), as an optimisation loop
+    makes them: the strip topology and boundary edges are built once, and
+    each call regathers the records from V on the device. The route depends
+    on the sizes only, never on earlier calls; release() frees the kept
+    field."""
+    if adjoint not in ADJOINTS:
+        raise ValueError(f"adjoint must be one of {ADJOINTS}")
+    key = str(precision).lower()
+    if key not in PRECISIONS:
+        raise ValueError("precision must be 'fp64' or 'fp32c'")
+    _require_cuda(V, F, O)
+    field = _kept_field(V, F, key, adjoint, O.shape[0] if O.dim() == 2 else 0)
+    if field is not None:
+        return field(V, O)
     return WindingNumber.apply(V, F, O, precision, adjoint)
+
+
+_kept_lock = threading.Lock()
+_kept = {"field": None, "key": None}
+
+
+def _kept_field(V, F, precision: str, adjoint: str, nq: int):
+    """The WindingField for a large call (None: face-record route), reusing
+    the one kept from the last large call when F is equal."""
+    from . import STRIP_MIN_QUERIES
+
+    if nq < STRIP_MIN_QUERIES[PRECISIONS[precision]] or F.dim() != 2 or F.shape[0] == 0:
+        return None
+    if V.dim() != 2 or V.shape[1] != 3:
+        raise ValueError("V must have shape (n, 3)")
+    if F.shape[1] != 3:
+        raise ValueError("F must have shape (m, 3)")
+    if V.dtype not in (torch.float32, torch.float64):
+        raise TypeError("V must be float32 or float64")
+    if F.dtype not in (torch.int32, torch.int64):
+        raise TypeError("F must be int32 or int64")
+    Fd = F.to(device=V.device, dtype=torch.int64).contiguous()
+    key = (int(V.shape[0]), precision, adjoint, int(Fd.shape[0]))
+    with _kept_lock:
+        kept = _kept["field"]
+        if kept is not None and _kept["key"] == key and torch.equal(kept.F, Fd):
+            return kept
+        if int(Fd.min()) < 0 or int(Fd.max()) >= V.shape[0]:
+            raise IndexError("face index is outside V")
+        if kept is not None:
+            kept.close()
+            _kept["field"], _kept["key"] = None, None
+        field = WindingField(Fd, V.shape[0], adjoint=adjoint, precision=precision)
+        _kept["field"], _kept["key"] = field, key
+        return field
+
+
+def release() -> None:
+    """Frees the WindingField that winding_number keeps between large calls."""
+    with _kept_lock:
+        if _kept["field"] is not None:
+            _kept["field"].close()
+        _kept["field"], _kept["key"] = None, None
 
 
 class _StripWinding(torch.autograd.Function):

@@ -304,3 +304,46 @@ def test_edge_adjoint_empty_inputs(require_gpu):
     O = torch.zeros((0, 3), dtype=torch.float64, device=dev)
     gq, gV = A.backward_edges(V, e, m, O, torch.zeros(0, dtype=torch.float64, device=dev))
     assert gq.shape == (0, 3) and gV.shape == (4, 3) and not gV.any()
+
+
+def test_large_functional_calls_use_a_kept_strip_field(require_gpu, c5):
+    """igl.autograd.winding_number at >= STRIP_MIN_QUERIES queries runs the
+    strip kernels through a WindingField kept for the next call with an equal
+    F; w and both gradients match the face-record route (WindingNumber) on
+    an open cap x 262,144 queries: FP64 w to 1e-12, FP32c w to 1e-6;
+    gradients (the same FP64 edge adjoint) to 1e-12 relative in FP64 and
+    1e-6 in FP32c, whose upstream dL/dw = 2 (w - t) carries the routes'
+    difference in w."""
+    import igl.autograd as A
+
+    V, F, Q, t = c5
+    V = V.astype(np.float64)
+    Fs = _open_subset(V, F)
+    td = torch.tensor(t, device="cuda")
+
+    def run(fn, vdtype, precision, faces):
+        Vd = torch.tensor(V, device="cuda", dtype=vdtype, requires_grad=True)
+        Od = torch.tensor(Q, device="cuda", requires_grad=True)
+        w = fn(Vd, torch.tensor(faces, device="cuda"), Od, precision, "edges")
+        ((w - td) ** 2).sum().backward()
+        return w.detach().cpu().numpy(), Vd.grad.cpu().numpy(), Od.grad.cpu().numpy()
+
+    A.release()
+    for precision, wbar, gbar in (("fp64", 1e-12, 1e-12), ("fp32c", 1e-6, 1e-6)):
+        for vdtype in (torch.float64, torch.float32):
+            for faces in (Fs, Fs[::-1].copy(), Fs):
+                got = run(A.winding_number, vdtype, precision, faces)
+                field = A._kept["field"]
+                assert field is not None and field.precision == precision
+                ref = run(A.WindingNumber.apply, vdtype, precision, faces)
+                assert np.abs(got[0] - ref[0]).max() <= wbar
+                for i in (1, 2):
+                    scale = np.abs(ref[i]).max()
+                    assert np.abs(got[i] - ref[i]).max() <= gbar * scale + 1e-300
+            # a call repeating the last F reuses the kept field, with the same bits
+            kept = A._kept["field"]
+            again = run(A.winding_number, torch.float64, precision, Fs)
+            assert A._kept["field"] is kept
+            assert np.array_equal(again[0], run(A.winding_number, torch.float64, precision, Fs)[0])
+    A.release()
+    assert A._kept["field"] is None
