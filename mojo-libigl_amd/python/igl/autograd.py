@@ -220,18 +220,28 @@ def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision=
         raise ValueError("precision must be 'fp64' or 'fp32c'")
     _require_cuda(V, F, O)
     field = _kept_field(V, F, key, adjoint, O.shape[0] if O.dim() == 2 else 0)
-    if field is not None:
-        return field(V, O)
-    return WindingNumber.apply(V, F, O, precision, adjoint)
+    if field is None:
+        return WindingNumber.apply(V, F, O, precision, adjoint)
+    # the field's strip records are regathered from V by each call: this
+    # call's stream first waits for the previous call's forward
+    stream = torch.cuda.current_stream(V.device)
+    done = getattr(_kept, "done", None)
+    if done is not None:
+        stream.wait_event(done)
+    w = field(V, O)
+    _kept.done = torch.cuda.Event()
+    _kept.done.record(stream)
+    return w
 
 
-_kept_lock = threading.Lock()
-_kept = {"field": None, "key": None}
+# per thread: the WindingField of the thread's last large call (its strip
+# records are rewritten by every call, so threads do not share one)
+_kept = threading.local()
 
 
 def _kept_field(V, F, precision: str, adjoint: str, nq: int):
     """The WindingField for a large call (None: face-record route), reusing
-    the one kept from the last large call when F is equal."""
+    the one this thread kept from its last large call when F is equal."""
     from . import STRIP_MIN_QUERIES
 
     if nq < STRIP_MIN_QUERIES[PRECISIONS[precision]] or F.dim() != 2 or F.shape[0] == 0:
@@ -246,26 +256,29 @@ def _kept_field(V, F, precision: str, adjoint: str, nq: int):
         raise TypeError("F must be int32 or int64")
     Fd = F.to(device=V.device, dtype=torch.int64).contiguous()
     key = (int(V.shape[0]), precision, adjoint, int(Fd.shape[0]))
-    with _kept_lock:
-        kept = _kept["field"]
-        if kept is not None and _kept["key"] == key and torch.equal(kept.F, Fd):
-            return kept
-        if int(Fd.min()) < 0 or int(Fd.max()) >= V.shape[0]:
-            raise IndexError("face index is outside V")
-        if kept is not None:
-            kept.close()
-            _kept["field"], _kept["key"] = None, None
-        field = WindingField(Fd, V.shape[0], adjoint=adjoint, precision=precision)
-        _kept["field"], _kept["key"] = field, key
-        return field
+    kept = getattr(_kept, "field", None)
+    if kept is not None and _kept.key == key and torch.equal(kept.F, Fd):
+        return kept
+    if int(Fd.min()) < 0 or int(Fd.max()) >= V.shape[0]:
+        raise IndexError("face index is outside V")
+    # the previous field is dropped, not closed: a pending backward may still
+    # hold it, and its mesh is freed with the last reference
+    _kept.field, _kept.key = None, None
+    field = WindingField(Fd, V.shape[0], adjoint=adjoint, precision=precision)
+    _kept.field, _kept.key = field, key
+    return field
+
+
+def kept_field():
+    """The WindingField this thread's winding_number keeps (None if none)."""
+    return getattr(_kept, "field", None)
 
 
 def release() -> None:
-    """Frees the WindingField that winding_number keeps between large calls."""
-    with _kept_lock:
-        if _kept["field"] is not None:
-            _kept["field"].close()
-        _kept["field"], _kept["key"] = None, None
+    """Drops the WindingField that this thread's winding_number keeps between
+    large calls; its device mesh is freed with the last reference (at once
+    unless a pending backward still holds it)."""
+    _kept.field, _kept.key, _kept.done = None, None, None
 
 
 class _StripWinding(torch.autograd.Function):

@@ -333,7 +333,7 @@ def test_large_functional_calls_use_a_kept_strip_field(require_gpu, c5):
         for vdtype in (torch.float64, torch.float32):
             for faces in (Fs, Fs[::-1].copy(), Fs):
                 got = run(A.winding_number, vdtype, precision, faces)
-                field = A._kept["field"]
+                field = A.kept_field()
                 assert field is not None and field.precision == precision
                 ref = run(A.WindingNumber.apply, vdtype, precision, faces)
                 assert np.abs(got[0] - ref[0]).max() <= wbar
@@ -341,9 +341,46 @@ def test_large_functional_calls_use_a_kept_strip_field(require_gpu, c5):
                     scale = np.abs(ref[i]).max()
                     assert np.abs(got[i] - ref[i]).max() <= gbar * scale + 1e-300
             # a call repeating the last F reuses the kept field, with the same bits
-            kept = A._kept["field"]
+            kept = A.kept_field()
             again = run(A.winding_number, torch.float64, precision, Fs)
-            assert A._kept["field"] is kept
+            assert A.kept_field() is kept
             assert np.array_equal(again[0], run(A.winding_number, torch.float64, precision, Fs)[0])
     A.release()
-    assert A._kept["field"] is None
+    assert A.kept_field() is None
+
+
+def test_kept_strip_fields_are_per_thread(require_gpu, c5):
+    """Two threads calling winding_number concurrently on the same F with
+    different V (each thread keeps its own field, whose records every call
+    rewrites) get the values of single-threaded calls, bit for bit."""
+    import threading
+
+    import igl.autograd as A
+
+    V, F, Q, _ = c5
+    V = V.astype(np.float64)
+    Fd = torch.tensor(_open_subset(V, F), device="cuda")
+    Od = torch.tensor(Q, device="cuda")
+    Vs = [torch.tensor(V * s, device="cuda") for s in (1.0, 1.1)]
+    A.release()
+    expect = [A.winding_number(v, Fd, Od, "fp32c").cpu().numpy() for v in Vs]
+    A.release()
+    out, errors = {}, []
+
+    def worker(k):
+        try:
+            res = [A.winding_number(Vs[k], Fd, Od, "fp32c").cpu().numpy() for _ in range(4)]
+            out[k] = res
+            A.release()
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in (0, 1)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for k in (0, 1):
+        for r in out[k]:
+            assert np.array_equal(r, expect[k])
