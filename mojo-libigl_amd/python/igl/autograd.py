@@ -210,9 +210,10 @@ def winding_number(V: torch.Tensor, F: torch.Tensor, O: torch.Tensor, precision=
     strip kernels through a WindingField that is kept for the next call with
     an equal F (vertex count, precision and adjoint), as an optimisation loop
     makes them: the strip topology and boundary edges are built once, and
-    each call regathers the records from V on the device. The route depends
-    on the sizes only, never on earlier calls; release() frees the kept
-    field."""
+    each call regathers the records from V on the device. The field is kept
+    per thread, and each call's stream waits for the previous call's
+    forward. The route depends on the sizes only, never on earlier calls;
+    release() drops the kept field."""
     if adjoint not in ADJOINTS:
         raise ValueError(f"adjoint must be one of {ADJOINTS}")
     key = str(precision).lower()
