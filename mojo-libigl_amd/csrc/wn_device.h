@@ -119,21 +119,19 @@ __device__ __noinline__ double pair_rec32_f64(float4 r0, float4 r1, float4 r2,
                   (double)r1.z - qx, (double)r1.w - qy, (double)r2.x - qz);
 }
 
-// atan(r) for |r| <= 0.25: r * Q(r^2), Q of degree 3 in r^2, relative-error
-// minimax on [0, 0.25] (max 1.18e-8), fitted like atan2_f32's polynomial by
+// atan(r) for |r| <= 1/8: r * Q(r^2), Q of degree 2 in r^2, relative-error
+// minimax on [0, 1/8] (max 1.67e-8), fitted like atan2_f32's polynomial by
 // gen_atan_coeffs.py.
-__device__ __forceinline__ float atan_small_f32(float r) {
+__device__ __forceinline__ float atan_small3_f32(float r) {
   const float t = r * r;
-  float p = -0.12990966888801511f;
-  p = __builtin_fmaf(p, t, 0.19950626010644533f);
-  p = __builtin_fmaf(p, t, -0.33332723769728184f);
-  p = __builtin_fmaf(p, t, 0.99999998819005977f);
+  float p = 0.19669884933950198f;
+  p = __builtin_fmaf(p, t, -0.33331408377741376f);
+  p = __builtin_fmaf(p, t, 0.99999998333354555f);
   return r * p;
 }
 
-// atan(x / 8) for |x| <= 1: the odd minimax polynomial of 3 terms for
-// atan(r) on |r| <= 1/8 (max relative error 1.67e-8, gen_atan_coeffs.py),
-// with its coefficients scaled by powers of two so that it takes x = 8 r.
+// atan(x / 8) for |x| <= 1: atan_small3_f32 with its coefficients scaled by
+// powers of two so that it takes x = 8 r (the same values, bit for bit).
 __device__ __forceinline__ float atan_small_x8_f32(float x) {
   const float t = x * x;
   float p = 0.19669884933950198f * (1.0f / 32768.0f);
@@ -152,9 +150,9 @@ struct PairGeom32 {
 // |det| < kFarTau * den (so den > 0 and |atan argument| < kFarTau) and
 // den > kFarKappa * (2 la lb lc) (away from every edge line; a distant face
 // has den -> 4 la lb lc). A wave whose 64 pairs are all far evaluates
-// atan_small_f32(det / den) instead of the octant-reduced atan2 and skips the
-// conditioning test.
-constexpr float kFarTau = 0.25f;
+// atan_small3_f32(det / den) instead of the octant-reduced atan2 and skips
+// the conditioning test.
+constexpr float kFarTau = 0.125f;
 constexpr float kFarKappa = 1.0f;
 
 __device__ __forceinline__ PairGeom32 pair_geom32(const float4 r0, const float4 r1,

@@ -3,9 +3,8 @@
 //
 // Same arithmetic as fwd_f32c_kernel (block-local origin, FP32 pairs, FP64
 // accumulation, far-pair fast path, FP64 re-evaluation of ill-conditioned
-// pairs), except that the fast path takes |det / den| < 1/8 (a 3-term
-// arctangent, one fma fewer than the face kernel's 4-term one on 1/4, at the
-// same relative accuracy), the angles of three consecutive records are summed in
+// pairs, the fast path on |det / den| < 1/8 with a 3-term arctangent),
+// except that the angles of three consecutive records are summed in
 // FP32 before each FP64 add (error at most 2 eps times the three angles'
 // magnitudes, the bound of a compensated sum, with a third of the FP64
 // instructions), and the re-evaluated angle is rounded to FP32 once before

@@ -119,7 +119,7 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
         bool risky = false;
         if (wave_all_far(g.det2, g.den2, g.s2)) {
           // wave-uniform: every lane's angle is small and well conditioned
-          ang = atan_small_f32(g.det2 * __builtin_amdgcn_rcpf(g.den2));
+          ang = atan_small3_f32(g.det2 * __builtin_amdgcn_rcpf(g.den2));
         } else {
           // An empty volatile asm gives this path a side effect, so the
           // compiler keeps a real (wave-uniform) branch instead of computing
