@@ -9,6 +9,7 @@
 #include "wn_fwn.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <thread>
 
@@ -144,17 +145,38 @@ void fwn_build(const double* V, const void* F, int fdtype, int64_t nf, int64_t l
   std::vector<int64_t> lo((size_t)nn), hi((size_t)nn);
   node_ranges(t, 0, lo, hi);
   out.nodes.resize((size_t)nn);
-  // nodes: threads take them round-robin (every level holds all faces once,
-  // so the work interleaves evenly)
-  auto run = [&](int w) {
-    for (int64_t n = w; n < nn; n += workers) {
-      FwnNode& e = out.nodes[(size_t)n];
-      node_expansion(out.corners.data(), lo[n], hi[n], e);
-      const int64_t* d = t.nodes.data() + 4 * n;
-      e.left = d[0];
-      e.right = d[1];
-      e.start = d[2];
-      e.count = d[3];
+  // nodes, largest first (breadth-first order), handed out by an atomic
+  // counter: one at a time for the top levels, where one node can hold a
+  // large share of the work, in chunks of 256 below (each node's expansion
+  // is computed alike whichever thread takes it)
+  std::vector<int64_t> bfs;
+  bfs.reserve((size_t)nn);
+  bfs.push_back(0);
+  for (size_t i = 0; i < bfs.size(); ++i) {
+    const int64_t* d = t.nodes.data() + 4 * bfs[i];
+    if (d[3] == 0) {
+      bfs.push_back(d[0]);
+      bfs.push_back(d[1]);
+    }
+  }
+  constexpr int64_t kSingles = 4096, kChunk = 256;
+  std::atomic<int64_t> next{0};
+  auto run = [&](int) {
+    for (;;) {
+      const int64_t at = next.load(std::memory_order_relaxed);
+      const int64_t take = at < kSingles ? 1 : kChunk;
+      const int64_t b = next.fetch_add(take);
+      if (b >= nn) break;
+      for (int64_t i = b; i < std::min(nn, b + take); ++i) {
+        const int64_t n = bfs[(size_t)i];
+        FwnNode& e = out.nodes[(size_t)n];
+        node_expansion(out.corners.data(), lo[n], hi[n], e);
+        const int64_t* d = t.nodes.data() + 4 * n;
+        e.left = d[0];
+        e.right = d[1];
+        e.start = d[2];
+        e.count = d[3];
+      }
     }
   };
   std::vector<std::thread> pool;
