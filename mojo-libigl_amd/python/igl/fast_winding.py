@@ -19,6 +19,8 @@ leaf radius every face is summed exactly. Inputs are validated as
 
 from __future__ import annotations
 
+import threading
+
 import numpy as np
 
 from ._lib import addr, fan_out, last_error, lib, worker_count
@@ -130,13 +132,35 @@ class FastWindingNumber:
             pass
 
 
+# per thread: the tree of the last fast_winding_number call and the bytes of
+# the V and F it was built from
+_kept = threading.local()
+
+
 def fast_winding_number(V, F, Q, beta: float = BETA, device: str = "cpu"):
-    """Hierarchical winding number of (V, F) at the rows of Q."""
-    fw = FastWindingNumber(V, F)
-    try:
-        return fw.query(Q, beta=beta, device=device)
-    finally:
-        fw.close()
+    """Hierarchical winding number of (V, F) at the rows of Q.
+
+    The tree of the last call (per thread) is kept and reused when V and F
+    are byte-for-byte equal (the build is deterministic, so the results are
+    those of a new tree); release() drops it."""
+    from . import _mesh
+
+    vertices, faces = _mesh(V, F, three_dimensional=True)
+    kept = getattr(_kept, "tree", None)
+    if (kept is None or kept._vertices.shape != vertices.shape
+            or kept._faces.shape != faces.shape
+            or not np.array_equal(kept._vertices.view(np.uint64), vertices.view(np.uint64))
+            or not np.array_equal(kept._faces, faces)):
+        _kept.tree = None
+        # private copies: the caller may change its arrays in place later
+        kept = FastWindingNumber(vertices.copy(), faces.copy())
+        _kept.tree = kept
+    return kept.query(Q, beta=beta, device=device)
+
+
+def release() -> None:
+    """Drops the tree this thread's fast_winding_number keeps."""
+    _kept.tree = None
 
 
 def fast_winding_number_for_meshes(V, F, Q):

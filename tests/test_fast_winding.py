@@ -154,3 +154,36 @@ def test_serial_and_parallel_spans_identical(monkeypatch):
     parallel = fw.query(Q)
     assert np.array_equal(serial, parallel)
     fw.close()
+
+
+def test_convenience_call_keeps_its_tree():
+    """fast_winding_number keeps the last tree (per thread) for byte-identical
+    V and F: repeated calls reuse it, an in-place change of V or another F
+    builds a new one, and every result equals a freshly built tree's."""
+    import igl.fast_winding as FW
+
+    V, F = W.icosphere(3)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    Q = np.random.default_rng(3).uniform(-1.5, 1.5, size=(500, 3))
+
+    def fresh(Vm, Fm):
+        t = igl.FastWindingNumber(Vm, Fm)
+        try:
+            return t.query(Q)
+        finally:
+            t.close()
+
+    FW.release()
+    a = igl.fast_winding_number(V, F, Q)
+    tree = FW._kept.tree
+    b = igl.fast_winding_number(V, F, Q)
+    assert FW._kept.tree is tree
+    assert np.array_equal(a, b) and np.array_equal(a, fresh(V, F))
+    V *= 1.1  # in place: the kept tree holds its own copy and must not be reused
+    c = igl.fast_winding_number(V, F, Q)
+    assert FW._kept.tree is not tree
+    assert np.array_equal(c, fresh(V, F))
+    d = igl.fast_winding_number(V, F[::-1].copy(), Q)
+    assert np.array_equal(d, fresh(V, F[::-1].copy()))
+    FW.release()
+    assert FW._kept.tree is None
