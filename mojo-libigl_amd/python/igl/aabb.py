@@ -12,6 +12,8 @@ HIP device is usable; it never falls back to the CPU.
 
 from __future__ import annotations
 
+import threading
+
 import numpy as np
 
 from ._lib import addr, fan_out, last_error, lib, worker_count
@@ -171,12 +173,40 @@ class AABB:
             pass
 
 
+# per thread: the tree of the last point_mesh_squared_distance call, built
+# from private copies of its V and Ele
+_kept = threading.local()
+
+
+def _same_bytes(a: np.ndarray, b: np.ndarray) -> bool:
+    return a.shape == b.shape and a.dtype == b.dtype and np.array_equal(
+        a.view(np.uint64) if a.dtype == np.float64 else a,
+        b.view(np.uint64) if b.dtype == np.float64 else b)
+
+
 def point_mesh_squared_distance(P, V, Ele, device="cpu"):
     """(sqrd, face index, closest point) of each row of P to the mesh
-    (python/igl/__init__.py:398-401)."""
-    tree = AABB()
-    tree.init(V, Ele)
-    return tree.squared_distance(V, Ele, P, device=device)
+    (python/igl/__init__.py:398-401).
+
+    The tree of the last call (per thread) is kept and reused when V and Ele
+    are byte-for-byte equal (the build is deterministic, so the results are
+    those of a new tree); release() drops it."""
+    from . import _mesh
+
+    vertices, faces = _mesh(V, Ele, three_dimensional=True)
+    tree = getattr(_kept, "tree", None)
+    if (tree is None or not _same_bytes(tree._vertices, vertices)
+            or not _same_bytes(tree._faces, faces)):
+        _kept.tree = None
+        tree = AABB()
+        tree.init(vertices.copy(), faces.copy())  # the caller may change its arrays later
+        _kept.tree = tree
+    return tree.squared_distance(tree._vertices, tree._faces, P, device=device)
+
+
+def release() -> None:
+    """Drops the tree this thread's point_mesh_squared_distance keeps."""
+    _kept.tree = None
 
 
 def signed_distance(P, V, F, device="cpu", winding="exact", beta=2.0):
@@ -189,7 +219,7 @@ def signed_distance(P, V, F, device="cpu", winding="exact", beta=2.0):
     exact one (``winding="exact"``; FP64) or the hierarchical one
     (``winding="fast"``, accuracy ``beta``), on the same device."""
     from . import _float_array, winding_number
-    from .fast_winding import FastWindingNumber
+    from .fast_winding import fast_winding_number
 
     points = _float_array(P, "P")
     if points.ndim != 2 or points.shape[1] != 3:
@@ -201,10 +231,6 @@ def signed_distance(P, V, F, device="cpu", winding="exact", beta=2.0):
     if mode == "exact":
         w = winding_number(V, F, points, device=device) if len(points) else np.zeros(0)
     else:
-        fw = FastWindingNumber(V, F)
-        try:
-            w = fw.query(points, beta=beta, device=device)
-        finally:
-            fw.close()
+        w = fast_winding_number(V, F, points, beta=beta, device=device)
     sign = np.where(np.asarray(w) >= 0.5, -1.0, 1.0)
     return sign * np.sqrt(sqrd), index, closest

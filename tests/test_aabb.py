@@ -287,3 +287,36 @@ def test_signed_distance_closed_mesh(winding):
     assert (S[r < 0.8] < 0).all() and (S[r > 1.2] > 0).all()
     with pytest.raises(ValueError):
         igl.signed_distance(P, V, F, winding="other")
+
+
+def test_convenience_call_keeps_its_tree():
+    """point_mesh_squared_distance keeps the last tree (per thread) for
+    byte-identical V and Ele: repeated calls reuse it, an in-place change of
+    V or another Ele builds a new one, and every result equals a freshly
+    built tree's bit for bit."""
+    import igl.aabb as AB
+
+    V, F = grid(20)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    P = np.random.default_rng(8).uniform(V.min(0) - 1, V.max(0) + 1, size=(700, 3))
+
+    def fresh(Vm, Fm):
+        t = igl.AABB()
+        t.init(Vm, Fm)
+        return t.squared_distance(Vm, Fm, P)
+
+    def same(a, b):
+        return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+    AB.release()
+    a = igl.point_mesh_squared_distance(P, V, F)
+    tree = AB._kept.tree
+    b = igl.point_mesh_squared_distance(P, V, F)
+    assert AB._kept.tree is tree and same(a, b) and same(a, fresh(V, F))
+    V[:, 2] += 0.25  # in place: the kept tree holds its own copy
+    c = igl.point_mesh_squared_distance(P, V, F)
+    assert AB._kept.tree is not tree and same(c, fresh(V, F))
+    F2 = F[::-1].copy()
+    assert same(igl.point_mesh_squared_distance(P, V, F2), fresh(V, F2))
+    AB.release()
+    assert AB._kept.tree is None
