@@ -19,9 +19,9 @@ constexpr int kFaceTile = 256;      // faces staged in LDS per tile (one per thr
 constexpr int kStripTileRecords = 255;  // strip records per tile (strips cut at tile ends)
 
 // FP64 sweep record of strip record k (wn_strip64.hip), built on the device
-// from the records: its vertex, the squared lengths to the previous two
-// vertices of its tile, and 16 x the normal of the face it closes, oriented
-// by its flag (zero when it closes none).
+// from the records: its vertex, minus half the squared lengths to the
+// previous two vertices of its tile, and 8 x the normal of the face it
+// closes, oriented by its flag (zero when it closes none).
 struct alignas(64) StripRec64 {
   double x, y, z, e1, e2, nx, ny, nz;
 };

@@ -3,7 +3,7 @@
 //
 // Per face the lane computes one vertex offset, one squared length and one
 // square root (the other two vertices are kept from the previous records),
-// the doubled dot products from the law of cosines with the squared edge
+// the dot products from the law of cosines with the halved squared edge
 // lengths of the sweep record, and det = N . c with the face normal of the
 // sweep record (exact: N . (a - c) = 0). A wave whose 64 pairs are all far
 // (|det| < den / 8 and den > la lb lc, the FP32 strip kernel's test) takes
@@ -42,16 +42,16 @@ __global__ __launch_bounds__(kBlockThreads) void strip64_prep_kernel(
   const double d1x = c.x - p.x, d1y = c.y - p.y, d1z = c.z - p.z;
   const double d2x = c.x - pp.x, d2y = c.y - pp.y, d2z = c.z - pp.z;
   const double ex = p.x - pp.x, ey = p.y - pp.y, ez = p.z - pp.z;
-  const double f16 = 16.0 * c.w;  // orientation flag (0 = no face), times 16
+  const double f8 = 8.0 * c.w;  // orientation flag (0 = no face), times 8
   StripRec64 r;
   r.x = c.x;
   r.y = c.y;
   r.z = c.z;
-  r.e1 = d1x * d1x + d1y * d1y + d1z * d1z;
-  r.e2 = d2x * d2x + d2y * d2y + d2z * d2z;
-  r.nx = f16 * (ey * d2z - ez * d2y);
-  r.ny = f16 * (ez * d2x - ex * d2z);
-  r.nz = f16 * (ex * d2y - ey * d2x);
+  r.e1 = -0.5 * (d1x * d1x + d1y * d1y + d1z * d1z);
+  r.e2 = -0.5 * (d2x * d2x + d2y * d2y + d2z * d2z);
+  r.nx = f8 * (ey * d2z - ez * d2y);
+  r.ny = f8 * (ez * d2x - ex * d2z);
+  r.nz = f8 * (ex * d2y - ey * d2x);
   out[k] = r;
 }
 
@@ -59,17 +59,17 @@ struct Slot64 {
   double ax, ay, az, l2, l;
 };
 
-// sqrt(x) for x > 0 finite: Goldschmidt iteration from v_rsq_f64 (two
-// steps: the estimate's ~2^-23 relative error squares twice). x == 0 gives
-// NaN (0 * inf), which fails the far test and sends the wave to pair_f64.
+// sqrt(x) for x > 0 finite: one Goldschmidt step from v_rsq_f64 (relative
+// error d of the estimate -> 1.5 d^2), then the root corrected by its exact
+// remainder x - g^2 times the unrefined half reciprocal (-> 1.5 d^3 before
+// the final rounding). x == 0 gives NaN (0 * inf), which fails the far test
+// and sends the wave to pair_f64.
 __device__ __forceinline__ double sqrt_gs_f64(double x) {
   const double r = __builtin_amdgcn_rsq(x);
-  double g = x * r, h = 0.5 * r;
-  double e = __builtin_fma(-g, h, 0.5);
-  g = __builtin_fma(g, e, g);
-  h = __builtin_fma(h, e, h);
-  e = __builtin_fma(-g, h, 0.5);
-  return __builtin_fma(g, e, g);
+  const double g = x * r, h = 0.5 * r;
+  const double e = __builtin_fma(-g, h, 0.5);
+  const double g1 = __builtin_fma(g, e, g);
+  return __builtin_fma(__builtin_fma(-g1, g1, x), h, g1);
 }
 
 // n / d for d > 0 finite: v_rcp_f64, one Newton step, then the quotient
@@ -108,6 +108,14 @@ __device__ __forceinline__ double atan_small_x8_f64(double x) {
   return x * p;
 }
 
+// h + e for h = (u + v) / 2, e in a scalar register: one VOP3 v_fma_f64
+// (compiled from C, the addend is copied to VGPRs for a v_fmac_f64).
+__device__ __forceinline__ double half_sum_plus(double u, double v, double e) {
+  double r;
+  asm("v_fma_f64 %0, %1, 0.5, %2" : "=v"(r) : "v"(u + v), "s"(e));
+  return r;
+}
+
 // The plain FP64 pair, oriented; out of line so that its registers are not
 // reserved for the whole sweep.
 __device__ __noinline__ double strip64_pair(double ax, double ay, double az, double bx,
@@ -117,19 +125,21 @@ __device__ __noinline__ double strip64_pair(double ax, double ay, double az, dou
   return flip ? -v : v;
 }
 
-// |det16| < den2 is |det / den| < 1/8 (det16 = 16 det, den2 = 2 den); dp > 0
-// is den > la lb lc.
-__device__ __forceinline__ bool wave_all_far_x8_f64(double det16, double den2, double dp) {
-  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabs(det16) < den2);
+// |det8| < den is |det / den| < 1/8 (det8 = 8 det); dp > 0 is
+// den > la lb lc.
+__device__ __forceinline__ bool wave_all_far_x8_f64(double det8, double den, double dp) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabs(det8) < den);
   const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp > 0.0);
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }
 
 // One record: R takes vertex j; when the record closes face (P, Q, R) its
 // angle joins acc (far pairs) or acc_slow (pairs of the plain formula; two
-// sums, so that each is only ever updated in place). s_carry is the doubled
-// dot product of the previous record's two newest vertices (Q, R of the last
-// step = P, Q here).
+// sums, so that each is only ever updated in place). s_carry is the dot
+// product of the previous record's two newest vertices (Q, R of the last
+// step = P, Q here). Dot products come from the law of cosines,
+// b.c = (|b|^2 + |c|^2) / 2 - |b - c|^2 / 2, with -|b - c|^2 / 2 stored in the
+// record: one fma, and den = pq lc + dp needs no doubling.
 __device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag_bits,
                                              const Slot64& P, const Slot64& Q, Slot64& R,
                                              double& s_carry, double& acc, double& acc_slow,
@@ -139,15 +149,15 @@ __device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag
   R.az = r.z - oz;
   R.l2 = __builtin_fma(R.az, R.az, __builtin_fma(R.ay, R.ay, R.ax * R.ax));
   R.l = sqrt_gs_f64(R.l2);
-  const double s_qr = (Q.l2 + R.l2) - r.e1;  // 2 b.c
+  const double s_qr = half_sum_plus(Q.l2, R.l2, r.e1);  // b.c
   if (flag_bits != 0) {
-    const double s_pr = (P.l2 + R.l2) - r.e2;  // 2 c.a
-    const double det16 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
+    const double s_pr = half_sum_plus(P.l2, R.l2, r.e2);  // c.a
+    const double det8 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
     const double pq = P.l * Q.l;
     const double dp = __builtin_fma(R.l, s_carry, __builtin_fma(P.l, s_qr, Q.l * s_pr));
-    const double den2 = __builtin_fma(pq + pq, R.l, dp);
-    if (__builtin_expect(wave_all_far_x8_f64(det16, den2, dp), 1)) {
-      acc += atan_small_x8_f64(div_f64(det16, den2));
+    const double den = __builtin_fma(pq, R.l, dp);
+    if (__builtin_expect(wave_all_far_x8_f64(det8, den, dp), 1)) {
+      acc += atan_small_x8_f64(div_f64(det8, den));
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       // flag -1.0 has the sign bit set: the face runs against the strip order
