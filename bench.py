@@ -56,17 +56,27 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def host_cores() -> int:
+# Worker threads of the CPU baseline at most: the CPU share of one GPU on the
+# measured boxes (their affinity mask shows more cores than the share).
+CPU_THREAD_CAP = 16
+
+
+def visible_cores() -> int:
+    """Cores in this process's affinity mask, as the OS reports them."""
     try:
-        cores = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    return max(1, min(cores, 16))
+        return os.cpu_count() or 1
+
+
+def host_cores() -> int:
+    return max(1, min(visible_cores(), CPU_THREAD_CAP))
 
 
 def cpu_baseline(tri, Q, seconds: float):
     """FP64 oracle over a bounded sample of the same queries, all usable host
-    cores (at most 16, the box's CPU share), contiguous query spans."""
+    cores up to CPU_THREAD_CAP (the affinity count itself is reported as
+    host_cores_visible), contiguous query spans."""
     import oracle
 
     cores = host_cores()
@@ -94,6 +104,8 @@ def cpu_baseline(tri, Q, seconds: float):
         "value": nf * nq / dt / 1e9,
         "unit": "G query-face pairs/s",
         "cores": cores,
+        "host_cores_visible": visible_cores(),
+        "thread_cap": CPU_THREAD_CAP,
         "kind": "port",
         "sample": f"first {nq} of the {len(Q)} queries x all {nf} faces, FP64, "
                   f"{cores} threads, {dt:.1f} s in ~1 s chunks after a warm-up",

@@ -1,11 +1,16 @@
-"""Launch-shape autotuner for the forward and batched kernels.
+"""Launch-shape autotuner for the batched kernels.
 
 Times candidate configurations on the live device (PyTorch events on the
-current stream) and caches the winner in a JSON file keyed by precision and
-log2 size buckets, so a shape class is tuned once per machine:
+current stream) and caches the winner in a JSON file, so a shape class is
+tuned once per machine and build:
 
-  forward: queries per lane x face splits
-  batch:   queries per lane x faces per work item (chunk)
+  face-record batch plans: queries per lane x faces per work item (chunk)
+  strip batch plans:       255-record tiles per work item (chunk_tiles)
+
+A cache key names the device (its name and gfx architecture), the library
+build (ABI version, size and modification time of the shared library), the
+precision and the batch's log2 size buckets, so a winner found on another
+device or with another build of the kernels is never reused.
 
 Cache file: $WN_TUNE_CACHE, else ~/.cache/mojo-libigl-amd/tune.json.
 """
@@ -66,106 +71,99 @@ def _time(fn, repeats: int) -> float:
     return best
 
 
-def forward_candidates(nf: int, nq: int, precision: int):
-    from .autograd import resolve_config
+def device_name() -> str:
+    """Name and gfx architecture of the current device."""
+    try:
+        import torch
 
-    qpls = (1, 2) if precision == 0 else (1, 2, 4)
-    out = []
-    for qpl in qpls:
-        _, s0, _ = resolve_config(nf, nq, precision, qpl, 0)
-        for splits in sorted({max(1, s0 // 2), s0, min(65535, s0 * 2)}):
-            out.append((qpl, splits))
-    return out
+        index = torch.cuda.current_device()
+        props = torch.cuda.get_device_properties(index)
+        arch = getattr(props, "gcnArchName", "")
+        return f"{props.name}/{arch}" if arch else str(props.name)
+    except Exception:
+        return "unknown"
 
 
-def tune_forward(tri, Q, precision: int, repeats: int = 3, use_cache: bool = True):
-    """Best (qpl, splits) for this shape class; times every candidate once
-    per cache miss."""
-    from .autograd import forward
+def library_build() -> str:
+    """ABI version of the loaded library plus the size and modification time
+    of its file: a rebuilt library is a new build."""
+    from ._lib import LIB, lib
 
-    nf, nq = int(tri.shape[1]), int(Q.shape[0])
-    key = f"fwd:p{precision}:f{_bucket(nf)}:q{_bucket(nq)}"
-    data = _load() if use_cache else {}
-    if key in data:
-        return tuple(data[key]["config"])
-    results = []
-    for qpl, splits in forward_candidates(nf, nq, precision):
-        ms = _time(lambda: forward(tri, Q, precision, qpl=qpl, splits=splits), repeats)
-        results.append((ms, qpl, splits))
-    results.sort()
-    best = (results[0][1], results[0][2])
+    try:
+        st = os.stat(LIB)
+        stamp = f"{st.st_size}.{int(st.st_mtime)}"
+    except OSError:
+        stamp = "nofile"
+    return f"abi{int(lib().wn_version())}.{stamp}"
+
+
+def cache_key(kind: str, precision: int, signature: str) -> str:
+    return f"{device_name()}|{library_build()}|{kind}:p{int(precision)}:{signature}"
+
+
+def _signature(counts, query_counts) -> str:
+    c = np.asarray(counts, dtype=np.int64)
+    q = np.asarray(query_counts, dtype=np.int64)
+    return f"{len(c)}:{_bucket(int(c.sum()))}:{_bucket(int(q.sum()))}:{_bucket(int((c * q).sum()))}"
+
+
+def _tuned(key: str, candidates, measure, use_cache: bool):
+    """The cached winner for key, else time every candidate (measure(c) ->
+    ms), cache and return the fastest."""
+    if use_cache:
+        hit = _load().get(key)
+        if hit is not None:
+            value = hit["config"]
+            return tuple(value) if isinstance(value, list) else value
+    results = sorted(((measure(c), i, c) for i, c in enumerate(candidates)),
+                     key=lambda r: (r[0], r[1]))
+    best = results[0][2]
     if use_cache:
         data = _load()
-        data[key] = {"config": list(best), "ms": results[0][0],
-                     "tried": [[r[1], r[2], r[0]] for r in results]}
+        data[key] = {"config": list(best) if isinstance(best, tuple) else best,
+                     "ms": results[0][0],
+                     "tried": [[list(c) if isinstance(c, tuple) else c, ms] for ms, _, c in results]}
         _store(data)
     return best
 
 
 def tune_batch(face_counts, query_counts, tri, Q, precision: int, repeats: int = 2,
                use_cache: bool = True, chunks=(1024, 2048, 4096, 8192), qpls=(1, 2)):
-    """Best (qpl, chunk_faces) for a batch shape."""
+    """Best (qpl, chunk_faces) of a face-record batch plan for this shape."""
     from .batch import WindingBatch
 
     fc = np.asarray(face_counts, dtype=np.int64)
     qc = np.asarray(query_counts, dtype=np.int64)
-    sig = f"{len(fc)}:{_bucket(int(fc.sum()))}:{_bucket(int(qc.sum()))}:{_bucket(int((fc * qc).sum()))}"
-    key = f"batch:p{precision}:{sig}"
-    data = _load() if use_cache else {}
-    if key in data:
-        return tuple(data[key]["config"])
-    results = []
-    for qpl in qpls:
-        for chunk in chunks:
-            plan = WindingBatch(fc, qc, precision=precision, qpl=qpl, chunk_faces=chunk)
-            try:
-                ms = _time(lambda: plan.run(tri, Q, use_graph=False), repeats)
-            finally:
-                plan.close()
-            results.append((ms, qpl, chunk))
-    results.sort()
-    best = (results[0][1], results[0][2])
-    if use_cache:
-        data = _load()
-        data[key] = {"config": list(best), "ms": results[0][0],
-                     "tried": [[r[1], r[2], r[0]] for r in results]}
-        _store(data)
-    return best
+
+    def measure(config):
+        qpl, chunk = config
+        plan = WindingBatch(fc, qc, precision=precision, qpl=qpl, chunk_faces=chunk)
+        try:
+            return _time(lambda: plan.run(tri, Q, use_graph=False), repeats)
+        finally:
+            plan.close()
+
+    key = cache_key("batch", precision, _signature(fc, qc))
+    return _tuned(key, [(q, c) for q in qpls for c in chunks], measure, use_cache)
 
 
 def tune_batch_strips(record_counts, query_counts, rec, Q, repeats: int = 2,
-                      use_cache: bool = True, chunks=(2, 4, 8, 16, 32)) -> int:
-    """Best chunk_tiles (255-record tiles per work item) for a strip batch."""
-    from .batch import StripBatch
+                      use_cache: bool = True, chunks=(2, 4, 8, 16, 32),
+                      precision="fp32c") -> int:
+    """Best chunk_tiles (255-record tiles per work item) of a strip batch
+    plan in the given precision."""
+    from .batch import StripBatch, _mode
 
     rc = np.asarray(record_counts, dtype=np.int64)
     qc = np.asarray(query_counts, dtype=np.int64)
-    sig = f"{len(rc)}:{_bucket(int(rc.sum()))}:{_bucket(int(qc.sum()))}:{_bucket(int((rc * qc).sum()))}"
-    key = f"batch_strips:{sig}"
-    data = _load() if use_cache else {}
-    if key in data:
-        return int(data[key]["config"])
-    results = []
-    for chunk in chunks:
-        plan = StripBatch(rc, qc, chunk_tiles=chunk)
+    mode = _mode(precision)
+
+    def measure(chunk):
+        plan = StripBatch(rc, qc, chunk_tiles=chunk, precision=precision)
         try:
-            ms = _time(lambda: plan.run(rec, Q, use_graph=False), repeats)
+            return _time(lambda: plan.run(rec, Q, use_graph=False), repeats)
         finally:
             plan.close()
-        results.append((ms, chunk))
-    results.sort()
-    best = int(results[0][1])
-    if use_cache:
-        data = _load()
-        data[key] = {"config": best, "ms": results[0][0], "tried": [[r[1], r[0]] for r in results]}
-        _store(data)
-    return best
 
-
-def device_name() -> str:
-    try:
-        import torch
-
-        return torch.cuda.get_device_name(0)
-    except Exception:
-        return "unknown"
+    key = cache_key("batch_strips", mode, _signature(rc, qc))
+    return int(_tuned(key, list(chunks), measure, use_cache))

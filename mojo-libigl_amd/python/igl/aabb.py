@@ -6,8 +6,9 @@ results and errors): ``AABB.init(V, Ele)`` builds the median-split tree
 ``AABB.squared_distance(V, Ele, P)`` returns ``(sqrd (p,), face index (p,)
 int64, closest point (p, 3))``. ``device="gpu"`` runs the same descent on the
 GPU (``wn_aabb_query``) over a device copy of the tree made on first use;
-its results equal the CPU path's bit for bit. The GPU path raises when no
-HIP device is usable; it never falls back to the CPU.
+its results equal the CPU path's bit for bit. With no usable HIP device a
+GPU request runs the CPU path (the rule of ``igl.winding_number``, reference
+python/igl/__init__.py:251,264; ``WN_GPU_STRICT=1`` raises instead).
 """
 
 from __future__ import annotations
@@ -75,7 +76,10 @@ class AABB:
         if selected not in ("cpu", "gpu"):
             raise ValueError("device must be 'cpu' or 'gpu'")
         if selected == "gpu":
-            return self._squared_distance_gpu(points)
+            from . import _use_gpu
+
+            if _use_gpu(True):
+                return self._squared_distance_gpu(points)
         vertices, faces = self._vertices, self._faces
         count = len(points)
         if not len(faces):
@@ -103,11 +107,6 @@ class AABB:
     def _squared_distance_gpu(self, points: np.ndarray):
         import torch
 
-        from . import _gpu_memory_available
-
-        if not _gpu_memory_available():
-            raise RuntimeError("device='gpu' requested but no HIP GPU with >= 4000 MiB free is "
-                               "available (this build does not fall back to the CPU)")
         P = torch.as_tensor(points, device="cuda")
         return tuple(t.cpu().numpy() for t in self.query_device(P))
 

@@ -15,10 +15,14 @@ leaf radius every face is summed exactly. Inputs are validated as
     w = fw.query(Q)                     # CPU
     w = fw.query(Q, device="gpu")       # GPU (tree copied on first use)
     w = fast_winding_number(V, F, Q, beta=4.0, device="gpu")
+
+A GPU request for an array of queries with no usable HIP device runs the CPU
+walk (the rule of ``igl.winding_number``; ``WN_GPU_STRICT=1`` raises).
 """
 
 from __future__ import annotations
 
+import sys
 import threading
 
 import numpy as np
@@ -26,6 +30,11 @@ import numpy as np
 from ._lib import addr, fan_out, last_error, lib, worker_count
 
 BETA = 2.0
+
+
+def _is_cuda_tensor(value) -> bool:
+    torch = sys.modules.get("torch")  # a CUDA tensor means torch is imported
+    return torch is not None and isinstance(value, torch.Tensor) and value.is_cuda
 
 
 class FastWindingNumber:
@@ -68,7 +77,7 @@ class FastWindingNumber:
         (q, 3) CUDA tensor (-> CUDA tensor, current stream) with device="gpu".
         sort (GPU): -1 = walk queries in Morton order when q >= 4096, 1 =
         always, 0 = in the given order; the result order never changes."""
-        from . import _float_array, _gpu_memory_available
+        from . import _float_array, _use_gpu
 
         beta = float(beta)
         if not beta > 0.0:
@@ -76,13 +85,12 @@ class FastWindingNumber:
         selected = str(device).lower()
         if selected not in ("cpu", "gpu"):
             raise ValueError("device must be 'cpu' or 'gpu'")
-        if selected == "gpu":
+        on_device = selected == "gpu" and _is_cuda_tensor(Q)
+        # a CUDA tensor stays on its device; an array follows the winding
+        # number's rule (no usable device: the CPU walk, same tree)
+        if on_device or _use_gpu(selected == "gpu"):
             import torch
 
-            if not _gpu_memory_available():
-                raise RuntimeError("device='gpu' requested but no HIP GPU with >= 4000 MiB free "
-                                   "is available (this build does not fall back to the CPU)")
-            on_device = isinstance(Q, torch.Tensor) and Q.is_cuda
             if on_device:
                 P = Q.to(torch.float64).contiguous()
             else:

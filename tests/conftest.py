@@ -16,6 +16,16 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X); run with -m gpu")
 
 
+@pytest.fixture(autouse=True)
+def _gpu_tests_never_use_the_cpu_fallback(request, monkeypatch):
+    """A GPU test that asks igl for device="gpu" must run the HIP path: with
+    no usable device the package's CPU fallback raises instead (the
+    WN_GPU_STRICT switch), so such a test fails rather than passing on the
+    CPU."""
+    if request.node.get_closest_marker("gpu") is not None:
+        monkeypatch.setenv("WN_GPU_STRICT", "1")
+
+
 def gpu_available() -> bool:
     try:
         import igl

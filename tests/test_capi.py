@@ -140,13 +140,56 @@ def test_input_contracts():
     assert np.array_equal(igl.winding_number(V, np.zeros((0, 3), dtype=np.int64), [[0.0, 0, 0]]), [0.0])
 
 
-def test_gpu_request_without_device_raises(monkeypatch):
-    """The reference silently used the CPU here (python/igl/__init__.py:251,
-    264, tests/test_parity.py:108-114); this build refuses instead."""
+def test_winding_gpu_request_and_cpu_fallback(monkeypatch):
+    """Reference tests/test_parity.py:108-114 (no-device half; the GPU half
+    is in tests/test_gpu_forward.py): with no usable device a GPU request
+    returns the CPU path's bits (python/igl/__init__.py:251,264)."""
     V, F = W.tetrahedron()
+    Q = np.array([[0.1, 0.1, 0.1], [2.0, 2.0, 2.0]])
+    monkeypatch.delenv(igl.STRICT_ENV, raising=False)
+    expected = igl.winding_number(V, F, Q)
     monkeypatch.setattr(igl, "_gpu_memory_available", lambda: False)
+    assert np.array_equal(igl.winding_number(V, F, Q, device="gpu"), expected)
+    assert np.array_equal(igl.winding_number(V, F, Q, device="gpu", precision="fp32c"), expected)
+    assert igl.winding_number(V, F, Q[0], device="gpu") == float(expected[0])
+
+
+def test_gpu_request_without_device_strict(monkeypatch):
+    """WN_GPU_STRICT=1 turns the no-device fallback into RuntimeError (every
+    GPU entry of the package); unset or 0 keeps the reference's fallback."""
+    V, F = W.tetrahedron()
+    P = np.array([[0.1, 0.1, 0.1], [2.0, 2.0, 2.0]])
+    monkeypatch.setattr(igl, "_gpu_memory_available", lambda: False)
+    monkeypatch.setenv(igl.STRICT_ENV, "1")
     with pytest.raises(RuntimeError, match="GPU"):
-        igl.winding_number(V, F, [[0.1, 0.1, 0.1]], device="gpu")
+        igl.winding_number(V, F, P, device="gpu")
+    tree = igl.AABB()
+    tree.init(V, F)
+    with pytest.raises(RuntimeError, match="GPU"):
+        tree.squared_distance(V, F, P, device="gpu")
+    with pytest.raises(RuntimeError, match="GPU"):
+        igl.fast_winding_number(V, F, P, device="gpu")
+    monkeypatch.setenv(igl.STRICT_ENV, "0")
+    assert np.array_equal(igl.winding_number(V, F, P, device="gpu"), igl.winding_number(V, F, P))
+
+
+def test_distance_and_fast_winding_cpu_fallback(monkeypatch):
+    """AABB.squared_distance and fast_winding_number follow the same rule:
+    no usable device -> the CPU path's results, bit for bit."""
+    V, F = W.grid(8)
+    rng = np.random.default_rng(4)
+    P = rng.uniform(-1.0, 8.0, size=(50, 3))
+    monkeypatch.delenv(igl.STRICT_ENV, raising=False)
+    tree = igl.AABB()
+    tree.init(V, F)
+    cpu = tree.squared_distance(V, F, P)
+    fast = igl.fast_winding_number(V, F, P)
+    monkeypatch.setattr(igl, "_gpu_memory_available", lambda: False)
+    gpu = tree.squared_distance(V, F, P, device="gpu")
+    for a, b in zip(cpu, gpu):
+        assert np.array_equal(a, b)
+    assert np.array_equal(igl.point_mesh_squared_distance(P, V, F, device="gpu")[0], cpu[0])
+    assert np.array_equal(igl.fast_winding_number(V, F, P, device="gpu"), fast)
 
 
 def test_gpu_kernel_failure_is_not_swallowed(monkeypatch):
@@ -174,7 +217,7 @@ def test_gpu_kernel_failure_is_not_swallowed(monkeypatch):
         igl.winding_number(V, F, [[0.1, 0.1, 0.1]], device="gpu", precision="fp32c")
 
 
-def test_no_device_here_reports_through_the_abi():
+def test_no_device_here_reports_through_the_abi(monkeypatch):
     h = igl.lib()
     if h.wn_device_count() > 0:
         pytest.skip("a HIP device is visible; this checks the no-device path")
@@ -185,6 +228,10 @@ def test_no_device_here_reports_through_the_abi():
     assert h.mli_winding_gpu(tri.ctypes.data, Q.ctypes.data, out.ctypes.data, 4, 1) == 0
     assert "no HIP device" in _lib.last_error()
     assert h.wn_device_free_bytes() < 0
+    # the real probe (not a monkeypatched one) sends the call to the CPU path
+    monkeypatch.delenv(igl.STRICT_ENV, raising=False)
+    assert igl.winding_number(V, F, Q, device="gpu") == igl.winding_number(V, F, Q)
+    monkeypatch.setenv(igl.STRICT_ENV, "1")
     with pytest.raises(RuntimeError, match="GPU"):
         igl.winding_number(V, F, Q, device="gpu")
 

@@ -100,28 +100,119 @@ def test_invalid_plans_are_rejected(require_gpu):
         WindingBatch([10, 20], [5])
 
 
-def test_config4_full_size(require_gpu):
-    """512 meshes (6.5M faces, 5.0M queries, 7.3e10 pairs): FP32-compensated
-    against FP64 on the GPU, FP64 against the oracle on a subsample."""
+@pytest.fixture(scope="module")
+def c4_full():
+    """512 meshes (6.5M faces, 5.0M queries, 7.3e10 pairs) and their FP64
+    face-plan result."""
     B = W.config4()
-    w64 = _run(B, "fp64")
-    w32 = _run(B, "fp32c")
-    closed_err, open_err = 0.0, 0.0
+    print("C4 full size: FP64 face plans", flush=True)
+    return B, _run(B, "fp64")
+
+
+def _oracle_subsample(B, results, bar_closed, bar_open, every=64, k=32):
+    """Every `every`-th mesh, k queries: max error against the oracle."""
     rng = np.random.default_rng(0)
-    for i, ((V, F, Q), a, b) in enumerate(zip(B, w64, w32)):
-        d = np.abs(a - b).max() if len(Q) else 0.0
+    worst = 0.0
+    for i in range(0, len(B), every):
+        V, F, Q = B[i]
+        if len(Q) == 0:
+            continue
+        sub = rng.choice(len(Q), min(k, len(Q)), replace=False)
+        ref = oracle.winding(V.astype(np.float64), F, Q[sub])
+        for w in results:
+            err = np.abs(w[i][sub] - ref).max()
+            worst = max(worst, err)
+            assert err <= (bar_closed if _closed(F, len(V)) else bar_open), i
+    return worst
+
+
+def _split_errors(B, a, b):
+    closed_err = open_err = 0.0
+    for (V, F, Q), x, y in zip(B, a, b):
+        d = np.abs(x - y).max() if len(Q) else 0.0
         if _closed(F, len(V)):
             closed_err = max(closed_err, d)
         else:
             open_err = max(open_err, d)
-        if i % 64 == 0:
-            sub = rng.choice(len(Q), min(32, len(Q)), replace=False)
-            ref = oracle.winding(V.astype(np.float64), F, Q[sub])
-            assert np.abs(a[sub] - ref).max() <= 1e-10
-    print(f"C4 fp32c vs fp64: closed meshes max {closed_err:.2e}, open meshes max {open_err:.2e}",
-          flush=True)
+    return closed_err, open_err
+
+
+def test_config4_full_size(require_gpu, c4_full):
+    """Face-record plans: FP32-compensated against FP64 on the GPU, FP64
+    against the oracle on a subsample."""
+    B, w64 = c4_full
+    w32 = _run(B, "fp32c")
+    closed_err, open_err = _split_errors(B, w64, w32)
+    worst = _oracle_subsample(B, [w64], 1e-10, 1e-10)
+    print(f"C4 face plans fp32c vs fp64: closed meshes max {closed_err:.2e}, open meshes max "
+          f"{open_err:.2e}; fp64 vs oracle {worst:.2e}", flush=True)
     assert closed_err <= 1e-6
     assert open_err <= 1e-5
+
+
+def test_config4_full_size_strip_plans(require_gpu, c4_full):
+    """The batch headline path at full size: strip plans in FP32c and FP64
+    against the FP64 face plans (FP64 strips <= 1e-12; FP32c at the face
+    plans' bars) and against the oracle on a per-mesh subsample; the FP32c
+    strip plan's graph replays equal its eager run."""
+    from igl.batch import StripBatch, pack_batch_strips
+
+    B, w64 = c4_full
+    s64 = _run(B, "fp64", strips=True)
+    worst64 = max(np.abs(a - b).max() for a, b, (_, _, Q) in zip(s64, w64, B) if len(Q))
+    rec, counts = pack_batch_strips([b[0] for b in B], [b[1] for b in B])
+    Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
+    qc = [len(b[2]) for b in B]
+    plan = StripBatch(counts, qc)
+    try:
+        eager = plan.run(rec, Q, use_graph=False).clone()
+        for i in range(3):
+            torch.randn(1000, device="cuda").sum().item()
+            out = plan.run(rec, Q, use_graph=True)
+            torch.cuda.synchronize()
+            assert torch.equal(out, eager), f"replay {i}"
+    finally:
+        plan.close()
+    w = eager.cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(qc)])
+    s32 = [w[off[k]:off[k + 1]] for k in range(len(B))]
+    closed_err, open_err = _split_errors(B, w64, s32)
+    worst = _oracle_subsample(B, [s64], 1e-10, 1e-10)
+    worst32 = _oracle_subsample(B, [s32], 1e-6, 1e-5)
+    print(f"C4 strip plans: fp64 strips vs fp64 faces {worst64:.2e}; fp32c strips vs fp64 faces: "
+          f"closed max {closed_err:.2e}, open max {open_err:.2e}; vs oracle fp64 {worst:.2e}, "
+          f"fp32c {worst32:.2e}", flush=True)
+    assert worst64 <= 1e-12
+    assert closed_err <= 1e-6
+    assert open_err <= 1e-5
+
+
+def test_tuned_strip_plan_reproduces_the_default(require_gpu, small_batch, tmp_path,
+                                                 monkeypatch):
+    """The autotuner's winner (timed on this device, cached, then read back)
+    gives a plan whose result equals the default plan's to the FP32c bar,
+    and a cache hit returns the same config without timing again."""
+    from igl import _autotune
+    from igl.batch import StripBatch, pack_batch_strips
+
+    monkeypatch.setenv("WN_TUNE_CACHE", str(tmp_path / "tune.json"))
+    B = small_batch
+    rec, counts = pack_batch_strips([b[0] for b in B], [b[1] for b in B])
+    Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
+    qc = [len(b[2]) for b in B]
+    best = _autotune.tune_batch_strips(counts, qc, rec, Q, chunks=(1, 2, 8))
+    assert best in (1, 2, 8)
+    calls = []
+    monkeypatch.setattr(_autotune, "_time", lambda fn, r: calls.append(1) or 0.0)
+    assert _autotune.tune_batch_strips(counts, qc, rec, Q, chunks=(1, 2, 8)) == best
+    assert not calls
+    plans = [StripBatch(counts, qc, chunk_tiles=best), StripBatch(counts, qc)]
+    try:
+        tuned, default = (p.run(rec, Q, use_graph=False).cpu().numpy() for p in plans)
+    finally:
+        for p in plans:
+            p.close()
+    assert np.abs(tuned - default).max() <= 1e-6
 
 
 @pytest.mark.parametrize("chunk_tiles", [1, 8, 0])

@@ -1,26 +1,109 @@
-"""Precision stress at full mesh size: the displaced level-8 icosphere
-(1,310,720 faces) with 262,144 of its workload's queries (half uniform in the
-box, half at |d| in [1e-7, 1e-2] along a face normal, so the exact answer is
-0 outside and 1 inside). Bars: FP32-compensated strip kernel vs FP64 plain
-<= 1e-6 on the uniform half and <= 1e-4 near the surface; every near-surface
-query on its correct side in both precisions; reversing every face's corner
-order negates w (size-independent property) to 2e-6."""
+"""Precision stress at full size: the displaced level-8 icosphere (1,310,720
+faces) with the workload's own 1,048,576 queries (seed 302: half uniform in
+the box, half at |d| in [1e-7, 1e-2] along a face normal, so the exact answer
+is 0 outside and 1 inside). 1.37e12 query-face pairs per precision.
+
+Bars (by distance bucket, FP32-compensated strip path against the FP64 strip
+path): <= 1e-6 on the uniform half, <= 1e-4 on every near-surface decade;
+every near-surface query on its correct side in both precisions; the uniform
+half integer-valued in FP64 (a closed mesh). A stratified subsample of 128
+queries (48 uniform, 16 per decade) is checked against the oracle (the FP64
+restatement of the reference CPU path, src/capi.mojo:205-294): every FP64
+path (strip route, plain face kernel) within 1e-10, every FP32-compensated
+path (strip route, face kernel) within its bucket's bar. Reversing every
+face's corner order negates w (size-independent property) to 2e-6."""
+
+import os
 
 import numpy as np
 import pytest
 
 import igl
+import oracle
 import wn_workloads as W
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
+DECADES = [(-7, -6), (-6, -5), (-5, -4), (-4, -3), (-3, -2)]
+
 
 @pytest.fixture(scope="module")
 def c3():
     V, F = W.config3_mesh()
-    Q, d = W.config3_queries(V, F, 262_144)
+    Q, d = W.config3_queries(V, F)
+    assert len(F) == 1_310_720 and len(Q) == 1_048_576
     return V, F, Q, d
+
+
+@pytest.fixture(scope="module")
+def c3_gpu(c3):
+    V, F, Q, d = c3
+    print("C3 full size: FP64 strip route", flush=True)
+    w64 = igl.winding_number(V, F, Q, device="gpu")
+    print("C3 full size: FP32c strip route", flush=True)
+    w32 = igl.winding_number(V, F, Q, device="gpu", precision="fp32c")
+    return w64, w32
+
+
+def buckets(d):
+    out = [("uniform", np.isnan(d), 1e-6)]
+    a = np.abs(d)
+    for lo, hi in DECADES:
+        out.append((f"|d| in [1e{lo}, 1e{hi})", (a >= 10.0 ** lo) & (a < 10.0 ** hi), 1e-4))
+    return out
+
+
+def test_full_size_by_bucket(require_gpu, c3, c3_gpu):
+    V, F, Q, d = c3
+    w64, w32 = c3_gpu
+    err = np.abs(w32 - w64)
+    near = ~np.isnan(d)
+    for name, m, bar in buckets(d):
+        assert m.sum() > 0, name
+        print(f"C3 {name}: {m.sum()} queries, fp32c vs fp64 max {err[m].max():.2e} "
+              f"mean {err[m].mean():.2e} (bar {bar:g})", flush=True)
+        assert err[m].max() <= bar, name
+    expect = (d[near] < 0).astype(np.float64)  # inside along the outward normal
+    assert (np.abs(w64[near] - expect) < 0.5).all()
+    assert (np.abs(w32[near] - expect) < 0.5).all()
+    # a closed mesh: integer w off the surface
+    assert np.abs(w64[~near] - np.round(w64[~near])).max() <= 1e-8
+
+
+def test_oracle_subsample(require_gpu, c3, c3_gpu):
+    V, F, Q, d = c3
+    w64, w32 = c3_gpu
+    rng = np.random.default_rng(3)
+    picks, bars = [], []
+    for name, m, bar in buckets(d):
+        idx = np.flatnonzero(m)
+        k = 48 if name == "uniform" else 16
+        sel = rng.choice(idx, k, replace=False)
+        picks.append(sel)
+        bars.append(np.full(k, bar))
+    pick, bar = np.concatenate(picks), np.concatenate(bars)
+    assert len(pick) == 128
+    tri = oracle.pack_soa(V, F)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    ref = oracle.winding_threads(tri, Q[pick], threads)
+    Qs = np.ascontiguousarray(Q[pick])
+    h = igl.lib()
+    face = {}
+    for precision in (0, 1):
+        out = np.full(len(Qs), np.nan)
+        st = h.wn_winding_gpu_ex(tri.ctypes.data, Qs.ctypes.data, out.ctypes.data, tri.shape[1],
+                                 len(Qs), precision, 0, 0, -1, -1.0)
+        assert st == 0, igl._lib.last_error()
+        face[precision] = out
+    e = {"fp64 strips": np.abs(w64[pick] - ref), "fp64 face": np.abs(face[0] - ref),
+         "fp32c strips": np.abs(w32[pick] - ref), "fp32c face": np.abs(face[1] - ref)}
+    print("C3 subsample vs oracle: " + ", ".join(f"{k} {v.max():.2e}" for k, v in e.items()),
+          flush=True)
+    assert e["fp64 strips"].max() <= 1e-10
+    assert e["fp64 face"].max() <= 1e-10
+    assert (e["fp32c strips"] <= bar).all()
+    assert (e["fp32c face"] <= bar).all()
 
 
 def _strip(V, F, Q):
@@ -31,20 +114,6 @@ def _strip(V, F, Q):
         return m.winding_number(torch.tensor(Q, device="cuda")).cpu().numpy()
     finally:
         m.close()
-
-
-def test_stress_precision_and_sides(require_gpu, c3):
-    V, F, Q, d = c3
-    w64 = igl.winding_number(V, F, Q, device="gpu")
-    w32 = _strip(V, F, Q)
-    near = ~np.isnan(d)
-    err = np.abs(w32 - w64)
-    print(f"uniform max {err[~near].max():.2e}, near-surface max {err[near].max():.2e}", flush=True)
-    assert err[~near].max() <= 1e-6
-    assert err[near].max() <= 1e-4
-    expect = (d[near] < 0).astype(np.float64)  # inside along the outward normal
-    assert (np.abs(w64[near] - expect) < 0.5).all()
-    assert (np.abs(w32[near] - expect) < 0.5).all()
 
 
 def test_stress_orientation_reversal(require_gpu, c3):
