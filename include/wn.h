@@ -73,7 +73,7 @@ int64_t wn_device_free_bytes(void);
  * swallow-all except (src/capi.mojo:397-398), which keeps no reason. */
 const char *wn_last_error(void);
 
-/* ABI version (1). */
+/* ABI version (2: adds wn_mesh_forward_noalloc). */
 int64_t wn_version(void);
 
 /* ---- Extended forward --------------------------------------------------- */
@@ -175,7 +175,22 @@ int64_t wn_mesh_forward(int64_t mesh, const double *q, double *dst, int64_t nq,
                         int64_t splits, int64_t sort, double kappa,
                         void *workspace, int64_t workspace_bytes, void *stream);
 
-/* Frees a mesh (synchronises the device first). */
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). wn_mesh_forward without the workspace argument: its scratch
+ * (split partials, Morton permutation, queries in launch order, sort
+ * temporaries) lives in a static 64 MiB device array that each GPU holds
+ * until the process ends. Calls nothing that allocates. splits 0 picks the
+ * default split count, lowered until the scratch fits; a configuration that
+ * needs more than 64 MiB (explicit splits, or more than ~1.9M queries with
+ * Morton order) fails with -4 and a message. Same results as wn_mesh_forward
+ * with the same splits and sort, bit for bit. Every call on one GPU shares
+ * that GPU's scratch: calls that overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller. Asynchronous on
+ * `stream`. */
+int64_t wn_mesh_forward_noalloc(int64_t mesh, const double *q, double *dst, int64_t nq,
+                                int64_t splits, int64_t sort, double kappa, void *stream);
+
 /* The same query in FP64 (no reference counterpart beyond
  * src/capi.mojo:218-235,286 itself): FP64 over the mesh's strip records, one
  * square root per face, far pairs from a polynomial accurate to FP64
@@ -185,6 +200,7 @@ int64_t wn_mesh_forward64(int64_t mesh, const double *q, double *dst, int64_t nq
                           int64_t splits, int64_t sort, void *workspace,
                           int64_t workspace_bytes, void *stream);
 
+/* Frees a mesh (synchronises the device first). */
 int64_t wn_mesh_destroy(int64_t mesh);
 
 /* Host-buffer strip path (no reference counterpart; the reference's GPU entry

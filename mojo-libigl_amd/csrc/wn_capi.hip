@@ -422,9 +422,47 @@ const char* aabb_args(const double* V, const void* F, int64_t nv, int64_t nf, in
 
 }  // namespace
 
+// Scratch of wn_mesh_forward_noalloc: the device memory the strip forward
+// needs beyond its inputs and outputs (split-K partials, Morton permutation,
+// queries in launch order, radix-sort temporaries), as one file-scope device
+// array. Every GPU the library runs on holds its own copy, at its own address,
+// until the process ends. In host code `scratch` names a host-side stand-in,
+// not device memory: host code uses only the per-device address that
+// hipGetSymbolAddress returns (noalloc_scratch below).
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;  // 64 MiB per GPU
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on `device` (the current device), looked up
+// once per device; nullptr with the reason in `why` on failure.
+void* noalloc_scratch(int device, std::string& why) {
+  static std::mutex mu;
+  static void* ptr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) {
+    why = "device ordinal outside the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      why = hip_msg("hipGetSymbolAddress(scratch)", e);
+      return nullptr;
+    }
+    ptr[device] = p;
+  }
+  return ptr[device];
+}
+
+}  // namespace
+
 extern "C" {
 
-int64_t wn_version(void) { return 1; }
+int64_t wn_version(void) { return 2; }
 
 const char* wn_last_error(void) { return g_error.c_str(); }
 
@@ -807,6 +845,45 @@ int64_t wn_mesh_forward(int64_t mesh, const double* q, double* dst, int64_t nq, 
                                   (int)cfg[1], kappa < 0.0 ? 1.0f : (float)kappa, workspace,
                                   (size_t)workspace_bytes, (hipStream_t)stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_forward", e));
+  return kOk;
+}
+
+int64_t wn_mesh_forward_noalloc(int64_t mesh, const double* q, double* dst, int64_t nq,
+                                int64_t splits, int64_t sort, double kappa, void* stream) {
+  int64_t cfg[2];
+  const int64_t st = wn_mesh_resolve(mesh, nq, splits, sort, cfg);
+  if (st != kOk) return st;
+  if (nq == 0) return kOk;
+  if (!q || !dst) return fail(kErrArg, "null device buffer");
+  MeshHandle* m = lookup_mesh(mesh);
+  int s = (int)cfg[0];
+  const int srt = (int)cfg[1];
+  if (splits <= 0 && s > 1 &&
+      wn::strip_workspace_bytes(nq, s, srt) > kNoallocScratchBytes) {
+    // default split count: the largest that fits the scratch (partials take
+    // 8 bytes per query and split on top of the split-free workspace)
+    const size_t base = wn::strip_workspace_bytes(nq, 1, srt);
+    const size_t room = base + 256 < kNoallocScratchBytes ? kNoallocScratchBytes - base - 256 : 0;
+    s = (int)std::max<size_t>(1, std::min<size_t>((size_t)s, room / ((size_t)nq * sizeof(double))));
+  }
+  const size_t need = wn::strip_workspace_bytes(nq, s, srt);
+  if (need > kNoallocScratchBytes)
+    return fail(kErrWorkspace, "wn_mesh_forward_noalloc: " + std::to_string(nq) + " queries with " +
+                                   std::to_string(s) + " splits need " + std::to_string(need) +
+                                   " bytes of scratch, more than the " +
+                                   std::to_string(kNoallocScratchBytes) +
+                                   " bytes of the static scratch; use fewer queries or splits, or "
+                                   "wn_mesh_forward with a workspace");
+  int device = -1;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipGetDevice", e));
+  std::string why;
+  void* ws = noalloc_scratch(device, why);
+  if (!ws) return fail(kErrHip, why);
+  e = wn::strip_launch((const double4*)m->d_rec, m->nrec, q, dst, nq, s, srt,
+                       kappa < 0.0 ? 1.0f : (float)kappa, ws, kNoallocScratchBytes,
+                       (hipStream_t)stream);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_forward_noalloc", e));
   return kOk;
 }
 

@@ -52,6 +52,7 @@ _SIGNATURES = {
     "wn_mesh_workspace_bytes": ([I] * 4, I),
     "wn_mesh_forward": ([I] * 6 + [D] + [I] * 3, I),
     "wn_mesh_forward64": ([I] * 9, I),
+    "wn_mesh_forward_noalloc": ([I] * 6 + [D, I], I),
     "wn_winding_gpu_mesh": ([I] * 9, I),
     "wn_release_host_buffers": ([], I),
     "wn_mesh_destroy": ([I], I),
