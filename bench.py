@@ -370,15 +370,15 @@ def run_batch(args, h):
     tri = pack_batch([b[0] for b in B], [b[1] for b in B])
     Q = torch.tensor(np.concatenate([b[2] for b in B]), device="cuda")
     t0 = time.perf_counter()
-    rec, rc = pack_batch_strips([b[0] for b in B], [b[1] for b in B])
+    rec, rc, bnd = pack_batch_strips([b[0] for b in B], [b[1] for b in B], boundary=True)
     torch.cuda.synchronize()
     strip_build_s = time.perf_counter() - t0
     log(f"strip records {int(rc.sum())} for {int(fc.sum())} faces in {strip_build_s:.2f} s")
     log("autotuning strip chunk")
     t0 = time.perf_counter()
-    chunk_tiles = _autotune.tune_batch_strips(rc, qc, rec, Q, use_cache=False)
+    chunk_tiles = _autotune.tune_batch_strips(rc, qc, rec, Q, use_cache=False, boundary=bnd)
     tune_strip_s = time.perf_counter() - t0
-    splan = StripBatch(rc, qc, chunk_tiles=chunk_tiles)
+    splan = StripBatch(rc, qc, chunk_tiles=chunk_tiles, boundary=bnd)
     out = torch.empty(len(Q), dtype=torch.float64, device="cuda")
     ms_graph, wall_graph = timed(lambda: splan.run(rec, Q, out=out, use_graph=True), args.steps,
                                  args.warmup)
@@ -424,6 +424,7 @@ def run_batch(args, h):
     info = plan.info()
     plan.close()
     d = np.abs(w_batch - w64)
+    open_mesh = np.repeat(bnd.edge_counts > 0, qc)  # per query: its mesh has a boundary
     return base_line(
         args, "G query-face pairs/s, batched FP32-comp, 512 uneven meshes, 1 MI355X",
         pairs / (ms_graph * 1e-3) / 1e9, "G query-face pairs/s", ms_graph, "f32",
@@ -437,12 +438,16 @@ def run_batch(args, h):
         fp64_batched={"value": pairs / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64,
                       "strips": {"value": pairs / (ms64s * 1e-3) / 1e9, "ms_per_step": ms64s,
                                  "max_abs_dw_vs_fp64_faces": float(np.abs(w64s - w64).max())}},
-        strip_records={"records": int(rc.sum()), "host_build_seconds": strip_build_s},
+        strip_records={"records": int(rc.sum()), "host_build_seconds": strip_build_s,
+                       "boundary_edges": bnd.n_edges,
+                       "open_meshes": int((bnd.edge_counts > 0).sum())},
         tuning={"chunk_tiles": chunk_tiles, "seconds": tune_strip_s, "items": sinfo["items"],
                 "grid": sinfo["grid"],
                 "face_batch": {"qpl": qpl, "chunk_faces": chunk, "seconds": tune_s,
                                "items": info["items"], "grid": info["grid"]}},
         accuracy={"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw": float(d.mean()),
+                  "max_abs_dw_open_meshes": float(d[open_mesh].max(initial=0.0)),
+                  "max_abs_dw_closed_meshes": float(d[~open_mesh].max(initial=0.0)),
                   "max_abs_dw_face_batch_vs_fp64": float(np.abs(w_face - w64).max()),
                   "max_abs_dw_batched_vs_separate": float(np.abs(w_face - w_sep).max())})
 

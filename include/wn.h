@@ -264,6 +264,20 @@ int64_t wn_batch_create_strips64(const int64_t *rec_off, const int64_t *query_of
 int64_t wn_batch_run(int64_t handle, const double *tri, const double *q,
                      double *dst, int64_t use_graph, void *stream);
 
+/* No reference counterpart. Boundary edges of a strip plan's meshes for the
+ * FP32-compensated boundary correction (an open mesh's FP32-rounded boundary
+ * differs from the true one by thin quads; the correction adds their angle,
+ * see wn_mesh_forward): (device) pairs (2 x ne int32, interleaved record
+ * indices into the plan's records, as wn_strip_boundary gives them per mesh,
+ * offset by the mesh's first record) and weight (ne float64), both kept by
+ * the caller while the plan runs; (host) edge_off (n_meshes + 1, from 0 to
+ * ne): mesh m owns edges [edge_off[m], edge_off[m+1]). Without it a plan runs
+ * no correction (open meshes then carry the FP32 boundary rounding, ~1e-6).
+ * FP64 strip plans accept and ignore it; face plans find their boundary
+ * edges themselves. Drops a captured graph. */
+int64_t wn_batch_set_boundary(int64_t handle, const int32_t *pairs, const double *weight,
+                              int64_t ne, const int64_t *edge_off);
+
 /* out6 = {work items, partial slots, workspace bytes, persistent grid,
  * F_total (tiles for a strip plan), Q_total}. */
 int64_t wn_batch_info(int64_t handle, int64_t *out6);
@@ -277,6 +291,16 @@ int64_t wn_batch_destroy(int64_t handle);
  * returns the record count or a negative status. */
 int64_t wn_strip_topology(const void *F, int64_t nv, int64_t nf, int64_t fdtype,
                           int32_t *idx, int8_t *flag, int64_t capacity);
+
+/* No reference counterpart. Net directed boundary edges of (host) F (as
+ * wn_boundary_edges) expressed as strip record indices: per edge the first
+ * record of (host) idx (the strip topology of F from wn_strip_topology, nrec
+ * records) holding each endpoint, as interleaved pairs (2 x count int32), and
+ * the weights (count float64); fills them when capacity suffices. Returns the
+ * count or a negative status. */
+int64_t wn_strip_boundary(const void *F, int64_t nv, int64_t nf, int64_t fdtype,
+                          const int32_t *idx, int64_t nrec, int32_t *pairs, double *weight,
+                          int64_t capacity);
 
 /* No reference counterpart. Device gather of strip records: rec[k] =
  * (V[idx[k]], flag[k]) for (device) V (nv x 3, vdtype 0 = float64,

@@ -23,7 +23,7 @@ LIB = os.path.join(LIBDIR, "libmojo_libigl_amd.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIP_SOURCES = ["wn_forward.hip", "wn_strip.hip", "wn_strip64.hip", "wn_sort.hip", "wn_batch.hip", "wn_backward.hip",
-               "wn_aabb.hip", "wn_fwn.hip", "wn_capi.hip"]
+               "wn_aabb.hip", "wn_fwn.hip", "wn_ribbon.hip", "wn_capi.hip"]
 CPU_SOURCES = ["wn_cpu.cpp", "wn_strips.cpp", "wn_edges.cpp", "wn_aabb_tree.cpp", "wn_fwn_tree.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -157,6 +157,23 @@ struct BatchPlan {
   void* d_sort_ws = nullptr;
   size_t sort_bytes = 0;
   StripRec64* d_rec64 = nullptr;  // FP64 strip plans: sweep records, rebuilt per run
+  // Boundary correction of FP32-compensated plans (wn_ribbon.hip): one block
+  // per (mesh, query tile), each mesh's edge range. Face plans find their
+  // boundary edges from the SoA on every run (the pass before the graph);
+  // strip plans take them from wn_batch_set_boundary.
+  bool boundary = false;
+  int64_t n_rblocks = 0;
+  void* ws2 = nullptr;  // rblocks, edge ranges, face offsets, the SoA pass
+  int2* d_rblocks = nullptr;
+  int32_t* d_e_begin = nullptr;
+  int32_t* d_e_end = nullptr;
+  int64_t* d_face_off = nullptr;
+  void* d_sb_ws = nullptr;
+  size_t sb_bytes = 0;
+  SoaBoundary sb{};
+  const int32_t* b_ids = nullptr;  // strip plans: caller's edges (record pairs)
+  const double* b_w = nullptr;
+  int64_t b_n = 0;
   // graph cache
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -273,6 +290,41 @@ int64_t plan_items(const int64_t* unit_off, const int64_t* query_off, int64_t n_
   return pbase;
 }
 
+// The boundary correction's block table ((mesh, query tile) for query
+// tiles of QB queries) plus per-mesh edge ranges, and, for a face plan, its
+// face offsets and the SoA boundary pass's workspace: one allocation.
+hipError_t plan_boundary(BatchPlan* p, const int64_t* face_off, int64_t QB) {
+  std::vector<int2> blocks;
+  for (int m = 0; m < p->n_meshes; ++m) {
+    const int64_t nq = p->query_off[(size_t)m + 1] - p->query_off[(size_t)m];
+    for (int64_t j = 0; j * QB < nq; ++j) blocks.push_back(make_int2(m, (int)j));
+  }
+  p->n_rblocks = (int64_t)blocks.size();
+  const size_t b_blocks = align_up(sizeof(int2) * std::max<size_t>(1, blocks.size()));
+  const size_t b_range = align_up(sizeof(int32_t) * (size_t)p->n_meshes);
+  const size_t b_foff = face_off ? align_up(sizeof(int64_t) * (size_t)(p->n_meshes + 1)) : 0;
+  p->sb_bytes = face_off ? soa_boundary_workspace_bytes(p->ftot, p->n_meshes) : 0;
+  hipError_t e = hipMalloc(&p->ws2, b_blocks + 2 * b_range + b_foff + p->sb_bytes);
+  if (e != hipSuccess) {
+    p->ws2 = nullptr;
+    return e;
+  }
+  char* c = (char*)p->ws2;
+  p->d_rblocks = (int2*)c; c += b_blocks;
+  p->d_e_begin = (int32_t*)c; c += b_range;
+  p->d_e_end = (int32_t*)c; c += b_range;
+  if (face_off) {
+    p->d_face_off = (int64_t*)c; c += b_foff;
+    p->d_sb_ws = c;
+  }
+  if (!blocks.empty())
+    e = hipMemcpy(p->d_rblocks, blocks.data(), sizeof(int2) * blocks.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && face_off)
+    e = hipMemcpy(p->d_face_off, face_off, sizeof(int64_t) * (size_t)(p->n_meshes + 1),
+                  hipMemcpyHostToDevice);
+  return e;
+}
+
 }  // namespace
 
 hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
@@ -304,8 +356,13 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
                                                      kBlockThreads, 0);
   if (e == hipSuccess)
     e = plan_finish(p, meshes, items, pbase, query_off, std::max(1, cus) * std::max(1, per_cu));
+  if (e == hipSuccess && precision == kFP32C) {
+    e = plan_boundary(p, face_off, QB);
+    p->boundary = e == hipSuccess;
+  }
   if (e != hipSuccess) {
     if (p->ws) (void)hipFree(p->ws);
+    if (p->ws2) (void)hipFree(p->ws2);
     delete p;
     return e;
   }
@@ -408,13 +465,36 @@ static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q
     batch_reduce_kernel<<<dim3((unsigned)((p->qtot + kBlockThreads - 1) / kBlockThreads)),
                           kBlockThreads, 0, s>>>(p->d_partial, perm, dst, p->d_meshes,
                                                  p->d_query_off, p->n_meshes, p->qtot);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !p->boundary || p->precision != kFP32C) return e;
+  // boundary correction of the open meshes, for the same query blocks
+  RibbonSrc src;
+  src.pts = tri;
+  if (p->strips) {
+    src.ids = p->b_ids;
+    src.w = p->b_w;
+    src.n = p->b_n;
+    return ribbon_batch_launch(src, 0, 1, p->d_e_begin, p->d_e_end, p->d_rblocks, p->n_rblocks,
+                               p->d_meshes, q, perm, dst, s);
+  }
+  src.stride = p->ftot;
+  src.ids = p->sb.ids;
+  src.w = p->sb.w;
+  src.count = p->sb.count;
+  return ribbon_batch_launch(src, 1, p->qpl, p->sb.e_begin, p->sb.e_end, p->d_rblocks,
+                             p->n_rblocks, p->d_meshes, q, perm, dst, s);
 }
 
 hipError_t batch_run(BatchPlan* p, const double* tri, const double* q, double* dst,
                      bool use_graph, hipStream_t stream) {
   if (p->qtot == 0) return hipSuccess;
   hipError_t e;
+  if (p->boundary && !p->strips) {
+    // boundary edges of this run's faces (the buffers keep their addresses)
+    e = soa_boundary_launch(tri, p->ftot, p->d_face_off, p->n_meshes, p->d_sb_ws, p->sb_bytes,
+                            &p->sb, stream);
+    if (e != hipSuccess) return e;
+  }
   if (p->sort) {
     e = morton_order_segmented(q, p->qtot, p->d_query_off, p->n_meshes, p->d_perm,
                                p->d_sort_ws, p->sort_bytes, stream);
@@ -453,10 +533,48 @@ void batch_destroy(BatchPlan* p) {
     (void)hipStreamSynchronize(p->cap_stream);
     (void)hipStreamDestroy(p->cap_stream);
   }
-  if (p->ws || p->d_rec64) (void)hipDeviceSynchronize();
+  if (p->ws || p->d_rec64 || p->ws2) (void)hipDeviceSynchronize();
   if (p->ws) (void)hipFree(p->ws);
   if (p->d_rec64) (void)hipFree(p->d_rec64);
+  if (p->ws2) (void)hipFree(p->ws2);
   delete p;
+}
+
+const char* batch_boundary_check(const BatchPlan* p, int64_t ne, const int64_t* edge_off) {
+  if (!p->strips) return "boundary edges are set on strip plans (face plans find their own)";
+  if (ne < 0 || ne > 0x7fffffff) return "edge count out of range";
+  if (!edge_off) return "null edge offsets";
+  if (edge_off[0] != 0 || edge_off[p->n_meshes] != ne) return "edge offsets must run from 0 to the edge count";
+  for (int m = 0; m < p->n_meshes; ++m)
+    if (edge_off[m + 1] < edge_off[m]) return "edge offsets must be non-decreasing";
+  return nullptr;
+}
+
+hipError_t batch_set_boundary(BatchPlan* p, const int32_t* ids, const double* w, int64_t ne,
+                              const int64_t* edge_off) {
+  if (p->exec) {  // a captured graph holds the previous edges
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  p->boundary = false;
+  if (!p->ws2) {
+    const hipError_t e = plan_boundary(p, nullptr, kBlockThreads);
+    if (e != hipSuccess) return e;
+  }
+  std::vector<int32_t> b((size_t)p->n_meshes), en((size_t)p->n_meshes);
+  for (int m = 0; m < p->n_meshes; ++m) {
+    b[(size_t)m] = (int32_t)edge_off[m];
+    en[(size_t)m] = (int32_t)edge_off[m + 1];
+  }
+  hipError_t e = hipMemcpy(p->d_e_begin, b.data(), sizeof(int32_t) * b.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(p->d_e_end, en.data(), sizeof(int32_t) * en.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  p->b_ids = ids;
+  p->b_w = w;
+  p->b_n = ne;
+  p->boundary = ne > 0;
+  return hipSuccess;
 }
 
 void batch_info(const BatchPlan* p, int64_t* out) {

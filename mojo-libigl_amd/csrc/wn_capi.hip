@@ -292,6 +292,20 @@ struct MeshHandle {
   wn::StripRec64* d_rec64 = nullptr;
   int32_t* d_idx = nullptr;
   int8_t* d_flag = nullptr;
+  // boundary edges (net directed, from F) as record index pairs + weights:
+  // the FP32-compensated forward's boundary correction
+  int64_t nbe = 0;
+  int32_t* d_bpairs = nullptr;
+  double* d_bw = nullptr;
+
+  wn::RibbonSrc boundary() const {
+    wn::RibbonSrc b;
+    b.pts = (const double*)d_rec;
+    b.ids = d_bpairs;
+    b.w = d_bw;
+    b.n = nbe;
+    return b;
+  }
 };
 
 // Device copy of a mesh's strip topology (and its records when V is given on
@@ -301,10 +315,16 @@ int64_t mesh_upload(MeshHandle* m, const double* V, const void* F, int fdtype) {
   std::vector<int8_t> flag;
   m->nrec = wn::strip_topology(F, m->nf, fdtype, idx, flag);
   if (m->nrec == 0) return kOk;
+  std::vector<int32_t> bpairs;
+  std::vector<double> bw;
+  m->nbe = wn::strip_boundary(F, m->nf, fdtype, m->nv, idx.data(), m->nrec, bpairs, bw);
   const size_t rec_b = wn::align_up((size_t)m->nrec * 4 * sizeof(double));
   const size_t rec64_b = wn::align_up((size_t)m->nrec * sizeof(wn::StripRec64));
   const size_t idx_b = wn::align_up((size_t)m->nrec * sizeof(int32_t));
-  hipError_t e = hipMalloc(&m->d_rec, rec_b + rec64_b + idx_b + wn::align_up((size_t)m->nrec));
+  const size_t flag_b = wn::align_up((size_t)m->nrec);
+  const size_t bp_b = wn::align_up((size_t)m->nbe * 2 * sizeof(int32_t));
+  const size_t bw_b = wn::align_up((size_t)m->nbe * sizeof(double));
+  hipError_t e = hipMalloc(&m->d_rec, rec_b + rec64_b + idx_b + flag_b + bp_b + bw_b);
   if (e != hipSuccess) {
     m->d_rec = nullptr;
     return fail(kErrHip, hip_msg("wn_mesh_create", e));
@@ -312,9 +332,16 @@ int64_t mesh_upload(MeshHandle* m, const double* V, const void* F, int fdtype) {
   m->d_rec64 = (wn::StripRec64*)((char*)m->d_rec + rec_b);
   m->d_idx = (int32_t*)((char*)m->d_rec + rec_b + rec64_b);
   m->d_flag = (int8_t*)((char*)m->d_rec + rec_b + rec64_b + idx_b);
+  m->d_bpairs = (int32_t*)((char*)m->d_rec + rec_b + rec64_b + idx_b + flag_b);
+  m->d_bw = (double*)((char*)m->d_rec + rec_b + rec64_b + idx_b + flag_b + bp_b);
   e = hipMemcpy(m->d_idx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(m->d_flag, flag.data(), flag.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && m->nbe > 0)
+    e = hipMemcpy(m->d_bpairs, bpairs.data(), bpairs.size() * sizeof(int32_t),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess && m->nbe > 0)
+    e = hipMemcpy(m->d_bw, bw.data(), bw.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess && V) {
     std::vector<double> rec((size_t)(4 * m->nrec));
     for (int64_t k = 0; k < m->nrec; ++k) {
@@ -692,6 +719,17 @@ int64_t wn_batch_run(int64_t handle, const double* tri, const double* q, double*
   return kOk;
 }
 
+int64_t wn_batch_set_boundary(int64_t handle, const int32_t* pairs, const double* weight,
+                              int64_t ne, const int64_t* edge_off) {
+  wn::BatchPlan* plan = lookup_batch(handle);
+  if (!plan) return fail(kErrArg, "unknown batch handle");
+  if (const char* why = wn::batch_boundary_check(plan, ne, edge_off)) return fail(kErrArg, why);
+  if (ne > 0 && (!pairs || !weight)) return fail(kErrArg, "null device buffer");
+  hipError_t e = wn::batch_set_boundary(plan, pairs, weight, ne, edge_off);
+  if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_batch_set_boundary", e));
+  return kOk;
+}
+
 int64_t wn_batch_info(int64_t handle, int64_t* out6) {
   wn::BatchPlan* plan = lookup_batch(handle);
   if (!plan) return fail(kErrArg, "unknown batch handle");
@@ -826,7 +864,8 @@ int64_t wn_mesh_workspace_bytes(int64_t mesh, int64_t nq, int64_t splits, int64_
   const int64_t st = wn_mesh_resolve(mesh, nq, splits, sort, cfg);
   if (st != kOk) return st;
   if (cfg[1] && device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
-  return (int64_t)wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+  return (int64_t)wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1],
+                                            lookup_mesh(mesh)->nbe);
 }
 
 int64_t wn_mesh_forward(int64_t mesh, const double* q, double* dst, int64_t nq, int64_t splits,
@@ -838,12 +877,13 @@ int64_t wn_mesh_forward(int64_t mesh, const double* q, double* dst, int64_t nq, 
   if (nq == 0) return kOk;
   if (!q || !dst) return fail(kErrArg, "null device buffer");
   MeshHandle* m = lookup_mesh(mesh);
-  const size_t need = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+  const size_t need = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1], m->nbe);
   if (workspace_bytes < 0 || (size_t)workspace_bytes < need || (need > 0 && !workspace))
     return fail(kErrWorkspace, "workspace smaller than wn_mesh_workspace_bytes()");
+  const wn::RibbonSrc b = m->boundary();
   hipError_t e = wn::strip_launch((const double4*)m->d_rec, m->nrec, q, dst, nq, (int)cfg[0],
                                   (int)cfg[1], kappa < 0.0 ? 1.0f : (float)kappa, workspace,
-                                  (size_t)workspace_bytes, (hipStream_t)stream);
+                                  (size_t)workspace_bytes, (hipStream_t)stream, &b);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_forward", e));
   return kOk;
 }
@@ -859,14 +899,14 @@ int64_t wn_mesh_forward_noalloc(int64_t mesh, const double* q, double* dst, int6
   int s = (int)cfg[0];
   const int srt = (int)cfg[1];
   if (splits <= 0 && s > 1 &&
-      wn::strip_workspace_bytes(nq, s, srt) > kNoallocScratchBytes) {
+      wn::strip_workspace_bytes(nq, s, srt, m->nbe) > kNoallocScratchBytes) {
     // default split count: the largest that fits the scratch (partials take
     // 8 bytes per query and split on top of the split-free workspace)
-    const size_t base = wn::strip_workspace_bytes(nq, 1, srt);
+    const size_t base = wn::strip_workspace_bytes(nq, 1, srt, m->nbe);
     const size_t room = base + 256 < kNoallocScratchBytes ? kNoallocScratchBytes - base - 256 : 0;
     s = (int)std::max<size_t>(1, std::min<size_t>((size_t)s, room / ((size_t)nq * sizeof(double))));
   }
-  const size_t need = wn::strip_workspace_bytes(nq, s, srt);
+  const size_t need = wn::strip_workspace_bytes(nq, s, srt, m->nbe);
   if (need > kNoallocScratchBytes)
     return fail(kErrWorkspace, "wn_mesh_forward_noalloc: " + std::to_string(nq) + " queries with " +
                                    std::to_string(s) + " splits need " + std::to_string(need) +
@@ -880,9 +920,10 @@ int64_t wn_mesh_forward_noalloc(int64_t mesh, const double* q, double* dst, int6
   std::string why;
   void* ws = noalloc_scratch(device, why);
   if (!ws) return fail(kErrHip, why);
+  const wn::RibbonSrc b = m->boundary();
   e = wn::strip_launch((const double4*)m->d_rec, m->nrec, q, dst, nq, s, srt,
                        kappa < 0.0 ? 1.0f : (float)kappa, ws, kNoallocScratchBytes,
-                       (hipStream_t)stream);
+                       (hipStream_t)stream, &b);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("wn_mesh_forward_noalloc", e));
   return kOk;
 }
@@ -976,7 +1017,8 @@ int64_t wn_winding_gpu_mesh(const double* V, const void* F, int64_t nv, int64_t 
   }
   const size_t q_b = wn::align_up((size_t)nq * 3 * sizeof(double));
   const size_t dst_b = wn::align_up((size_t)nq * sizeof(double));
-  const size_t ws_b = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1]);
+  const size_t ws_b = wn::strip_workspace_bytes(nq, (int)cfg[0], (int)cfg[1],
+                                                precision == wn::kFP64 ? 0 : m->nbe);
   const size_t v_b = same_v ? 0 : wn::align_up(v_bytes);
   const size_t need = q_b + dst_b + ws_b + v_b;
   if (e == hipSuccess && need > ctx.cap) {
@@ -1012,9 +1054,11 @@ int64_t wn_winding_gpu_mesh(const double* V, const void* F, int64_t nv, int64_t 
       if (precision == wn::kFP64)
         e = wn::strip64_launch(m->d_rec64, (const double4*)m->d_rec, m->nrec, d_q, d_dst, nq,
                                (int)cfg[0], (int)cfg[1], d_ws, ws_b, ctx.stream);
-      else
+      else {
+        const wn::RibbonSrc b = m->boundary();
         e = wn::strip_launch((const double4*)m->d_rec, m->nrec, d_q, d_dst, nq, (int)cfg[0],
-                             (int)cfg[1], 1.0f, d_ws, ws_b, ctx.stream);
+                             (int)cfg[1], 1.0f, d_ws, ws_b, ctx.stream, &b);
+      }
     }
     if (e == hipSuccess)
       e = hipMemcpyAsync(dst, d_dst, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost,
@@ -1286,6 +1330,29 @@ int64_t wn_strip_topology(const void* F, int64_t nv, int64_t nf, int64_t fdtype,
   if (idx && flag && capacity >= n) {
     std::memcpy(idx, ids.data(), ids.size() * sizeof(int32_t));
     std::memcpy(flag, flags.data(), flags.size());
+  }
+  return n;
+}
+
+int64_t wn_strip_boundary(const void* F, int64_t nv, int64_t nf, int64_t fdtype,
+                          const int32_t* idx, int64_t nrec, int32_t* pairs, double* weight,
+                          int64_t capacity) {
+  if (nv < 0 || nf < 0 || nrec < 0) return fail(kErrArg, "negative size");
+  if (nv > (int64_t)0x7fffffff) return fail(kErrArg, "more than 2^31-1 vertices");
+  if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
+  if ((nf > 0 && !F) || (nrec > 0 && !idx)) return fail(kErrArg, "null host buffer");
+  for (int64_t i = 0; i < 3 * nf; ++i) {
+    const int64_t v = fdtype == 0 ? ((const int64_t*)F)[i] : (int64_t)((const int32_t*)F)[i];
+    if (v < 0 || v >= nv) return fail(kErrArg, "face index is outside V");
+  }
+  std::vector<int32_t> pr;
+  std::vector<double> w;
+  const int64_t n = wn::strip_boundary(F, nf, (int)fdtype, nv, idx, nrec, pr, w);
+  for (int32_t r : pr)
+    if (r < 0) return fail(kErrArg, "a boundary vertex has no record (idx is not F's topology)");
+  if (pairs && weight && capacity >= n) {
+    std::memcpy(pairs, pr.data(), pr.size() * sizeof(int32_t));
+    std::memcpy(weight, w.data(), w.size() * sizeof(double));
   }
   return n;
 }

@@ -52,4 +52,20 @@ int64_t boundary_edges(const void* F, int64_t nf, int fdtype, std::vector<int32_
   return (int64_t)weight.size();
 }
 
+int64_t strip_boundary(const void* F, int64_t nf, int fdtype, int64_t nv, const int32_t* idx,
+                       int64_t nrec, std::vector<int32_t>& pairs, std::vector<double>& weight) {
+  std::vector<int32_t> edges;
+  const int64_t n = boundary_edges(F, nf, fdtype, edges, weight);
+  pairs.assign(edges.size(), -1);
+  if (n == 0) return 0;
+  // first record of each vertex (every face corner is some record's vertex)
+  std::vector<int32_t> first((size_t)nv, -1);
+  for (int64_t k = nrec - 1; k >= 0; --k) {
+    const int32_t v = idx[k];
+    if (v >= 0 && v < nv) first[(size_t)v] = (int32_t)k;
+  }
+  for (size_t i = 0; i < edges.size(); ++i) pairs[i] = first[(size_t)edges[i]];
+  return n;
+}
+
 }  // namespace wn

@@ -260,13 +260,17 @@ const char* fwd_check(int64_t nf, int64_t nq, const FwdConfig& cfg) {
 }
 
 size_t fwd_workspace_bytes(int64_t nf, int64_t nq, const FwdConfig& cfg) {
-  (void)nf;
   size_t bytes = 0;
   if (cfg.splits > 1) bytes += align_up((size_t)nq * (size_t)cfg.splits * sizeof(double));
   if (cfg.precision == kFP32C && cfg.sort) {
     bytes += align_up((size_t)nq * sizeof(int32_t));
     bytes += align_up((size_t)nq * 3 * sizeof(double));
     bytes += morton_workspace_bytes(nq);
+  }
+  if (cfg.precision == kFP32C && nf > 0) {
+    // boundary edges from the corner coordinates, then the correction
+    bytes += soa_boundary_workspace_bytes(nf, 0);
+    bytes += ribbon_workspace_bytes(nq, ribbon_splits(nq, cfg.qpl, 3 * nf));
   }
   return bytes;
 }
@@ -291,6 +295,7 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
     hipError_t e = morton_order(q, nq, perm, cursor, morton_workspace_bytes(nq), stream);
     if (e == hipSuccess) e = gather_queries_launch(q, perm, nq, qs, stream);
     if (e != hipSuccess) return e;
+    cursor += morton_workspace_bytes(nq);
     q = qs;
   }
   double* out = cfg.splits > 1 ? partial : dst;
@@ -323,7 +328,23 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (cfg.splits > 1) e = reduce_splits_launch(partial, perm, dst, nq, cfg.splits, stream);
-  return e;
+  if (e != hipSuccess || cfg.precision != kFP32C) return e;
+  // boundary correction of an open mesh: its boundary edges from the corner
+  // coordinates (none for a closed mesh), then the same query blocks
+  const size_t sb_bytes = soa_boundary_workspace_bytes(nf, 0);
+  SoaBoundary sb;
+  e = soa_boundary_launch(tri, nf, nullptr, 0, cursor, sb_bytes, &sb, stream);
+  if (e != hipSuccess) return e;
+  cursor += sb_bytes;
+  RibbonSrc src;
+  src.pts = tri;
+  src.stride = nf;
+  src.ids = sb.ids;
+  src.w = sb.w;
+  src.count = sb.count;
+  const int rs = ribbon_splits(nq, cfg.qpl, 3 * nf);
+  return ribbon_launch(src, 1, cfg.qpl, q, perm, dst, nq, rs, cursor,
+                       ribbon_workspace_bytes(nq, rs), stream);
 }
 
 }  // namespace wn

@@ -55,8 +55,10 @@ int64_t strip_topology(const void* F, int64_t nf, int fdtype, std::vector<int32_
 hipError_t strip_gather_launch(const void* V, int vdtype, int64_t nv, const int32_t* idx,
                                const int8_t* flag, int64_t nrec, double4* rec,
                                hipStream_t stream);
+struct RibbonSrc;
 int strip_resident_blocks();
-size_t strip_workspace_bytes(int64_t nq, int splits, int sort);
+// nbe: boundary edges of the mesh (the correction's partials when it splits)
+size_t strip_workspace_bytes(int64_t nq, int splits, int sort, int64_t nbe = 0);
 // FP64 strip forward (wn_strip64.hip): sweep records from the records, and
 // the launch (same workspace layout and size as strip_launch).
 hipError_t strip64_prep_launch(const double4* rec, int64_t nrec, StripRec64* out,
@@ -64,9 +66,11 @@ hipError_t strip64_prep_launch(const double4* rec, int64_t nrec, StripRec64* out
 hipError_t strip64_launch(const StripRec64* rec64, const double4* rec, int64_t nrec,
                           const double* q, double* dst, int64_t nq, int splits, int sort,
                           void* ws, size_t ws_bytes, hipStream_t stream);
+// boundary (optional): the mesh's boundary edges as record pairs (pts is
+// taken to be rec); the correction runs after the sweep when it has edges.
 hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, double* dst,
                         int64_t nq, int splits, int sort, float kappa, void* ws, size_t ws_bytes,
-                        hipStream_t stream);
+                        hipStream_t stream, const RibbonSrc* boundary = nullptr);
 
 // Forward pass (wn_forward.hip).
 // Blocks of the forward kernel (precision, qpl) resident at once on the
@@ -110,6 +114,11 @@ hipError_t scatter_corners_launch(const double* gtri, const void* F, int fdtype,
 // vertex ids) and their positive weights (net counts); returns ne.
 int64_t boundary_edges(const void* F, int64_t nf, int fdtype, std::vector<int32_t>& edges,
                        std::vector<double>& weight);
+// The same edges as strip record indices (interleaved pairs: the first
+// record holding each endpoint; idx = vertex id per record), for the
+// boundary correction of the strip paths.
+int64_t strip_boundary(const void* F, int64_t nf, int fdtype, int64_t nv, const int32_t* idx,
+                       int64_t nrec, std::vector<int32_t>& pairs, std::vector<double>& weight);
 // Edge adjoint (wn_backward.hip): grad_q (nq x 3) and grad_V (nv x 3, vdtype
 // 0 = float64, 1 = float32 V) overwritten from the ne weighted edges.
 const char* bwd_edge_check(int64_t ne, int64_t nq, int qpl, int splits);
@@ -117,6 +126,62 @@ hipError_t bwd_edge_launch(const void* V, int vdtype, int64_t nv, const int32_t*
                            const double* weight, int64_t ne, const double* q, const double* gw,
                            double* gq, double* gv, int64_t nq, int qpl, int splits,
                            hipStream_t stream);
+
+}  // namespace wn
+
+namespace wn {
+
+// Boundary correction of the FP32-compensated mode (wn_ribbon.hip): adds,
+// per block of queries, the angle of the thin quads between each boundary
+// edge and its FP32-rounded copy (rounded relative to the block's origin as
+// the forward kernels round it) to dst, w += sum / (2 pi) through perm.
+//
+// Edge source: interleaved id pairs and weights. Records (soa = 0): both ids
+// index strip records (4 doubles each) in pts. Face SoA (soa = 1): (face,
+// corner) of the 9-row SoA pts with row stride `stride`; the edge runs corner
+// c -> c + 1 mod 3. The edge count is *count (device) when count is set,
+// else n.
+struct RibbonSrc {
+  const double* pts = nullptr;
+  int64_t stride = 0;
+  const int32_t* ids = nullptr;
+  const double* w = nullptr;
+  const int32_t* count = nullptr;
+  int64_t n = 0;
+};
+
+// Edge splits of a single-mesh correction (deterministic in its arguments)
+// and the partials it needs.
+int ribbon_splits(int64_t nq, int qpl, int64_t ne_bound);
+size_t ribbon_workspace_bytes(int64_t nq, int splits);
+// Single mesh: query blocks of 256 x qpl queries of q (launch order), the
+// blocks of the forward kernel that ran before it on the same stream.
+hipError_t ribbon_launch(const RibbonSrc& src, int soa, int qpl, const double* q,
+                         const int32_t* perm, double* dst, int64_t nq, int splits, void* ws,
+                         size_t ws_bytes, hipStream_t s);
+// Batch: one block per (mesh, query tile) in blocks; mesh m's edges are
+// [e_begin[m], e_end[m]).
+struct BatchMesh;
+hipError_t ribbon_batch_launch(const RibbonSrc& src, int soa, int qpl, const int32_t* e_begin,
+                               const int32_t* e_end, const int2* blocks, int64_t nblocks,
+                               const BatchMesh* meshes, const double* q, const int32_t* perm,
+                               double* dst, hipStream_t s);
+
+// Boundary edges of a face SoA from its corner coordinates (device):
+// interleaved (face, corner) pairs, weights, the count, and per segment
+// (faces [face_off[s], face_off[s+1]); face_off null = one segment) the
+// range [e_begin[s], e_end[s]) of its edges. Capacity 3 nf edges.
+struct SoaBoundary {
+  int32_t* ids = nullptr;
+  double* w = nullptr;
+  int32_t* count = nullptr;
+  int32_t* e_begin = nullptr;
+  int32_t* e_end = nullptr;
+  int64_t cap = 0;
+};
+size_t soa_boundary_workspace_bytes(int64_t nf, int nseg);
+hipError_t soa_boundary_launch(const double* tri, int64_t nf, const int64_t* face_off, int nseg,
+                               void* ws, size_t ws_bytes, SoaBoundary* out, hipStream_t s);
 
 }  // namespace wn
 
@@ -176,6 +241,13 @@ hipError_t batch_create_strips(const int64_t* rec_off, const int64_t* query_off,
                                int64_t chunk_tiles, int sort, float kappa, int precision,
                                BatchPlan** out);
 void batch_destroy(BatchPlan* p);
+// Strip plans: the meshes' boundary edges for the FP32-compensated boundary
+// correction, as interleaved record pairs (ids, 2 x ne) and weights (w) in
+// device memory the caller keeps; mesh m owns edges [edge_off[m],
+// edge_off[m+1]) (host). nullptr from the check if the arguments fit.
+const char* batch_boundary_check(const BatchPlan* p, int64_t ne, const int64_t* edge_off);
+hipError_t batch_set_boundary(BatchPlan* p, const int32_t* ids, const double* w, int64_t ne,
+                              const int64_t* edge_off);
 // out6 = {items, partial slots, workspace bytes, persistent grid, faces, queries}
 void batch_info(const BatchPlan* p, int64_t* out6);
 
@@ -217,3 +289,4 @@ hipError_t fwn_query_launch(const void* dev, int64_t nnodes, const double* q, do
                             hipStream_t stream);
 
 }  // namespace wn
+

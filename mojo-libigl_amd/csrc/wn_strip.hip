@@ -335,20 +335,22 @@ int strip_resident_blocks() {
   return cus * per_cu;
 }
 
-size_t strip_workspace_bytes(int64_t nq, int splits, int sort) {
+size_t strip_workspace_bytes(int64_t nq, int splits, int sort, int64_t nbe) {
   size_t bytes = 0;
   if (splits > 1) bytes += align_up((size_t)nq * (size_t)splits * sizeof(double));
   if (sort)
     bytes += align_up((size_t)nq * sizeof(int32_t)) + align_up((size_t)nq * 3 * sizeof(double)) +
              morton_workspace_bytes(nq);
+  if (nbe > 0) bytes += ribbon_workspace_bytes(nq, ribbon_splits(nq, 1, nbe));
   return bytes;
 }
 
 hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, double* dst,
                         int64_t nq, int splits, int sort, float kappa, void* ws, size_t ws_bytes,
-                        hipStream_t stream) {
+                        hipStream_t stream, const RibbonSrc* boundary) {
   if (nq == 0) return hipSuccess;
-  if (ws_bytes < strip_workspace_bytes(nq, splits, sort)) return hipErrorInvalidValue;
+  const int64_t nbe = boundary ? boundary->n : 0;
+  if (ws_bytes < strip_workspace_bytes(nq, splits, sort, nbe)) return hipErrorInvalidValue;
   const int64_t ntiles = nrec / kStripTileRecords;
   char* cursor = (char*)ws;
   double* partial = nullptr;
@@ -365,6 +367,7 @@ hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, doubl
     hipError_t e = morton_order(q, nq, perm, cursor, morton_workspace_bytes(nq), stream);
     if (e == hipSuccess) e = gather_queries_launch(q, perm, nq, qs, stream);
     if (e != hipSuccess) return e;
+    cursor += morton_workspace_bytes(nq);
     q = qs;
   }
   if (ntiles == 0) return fill_launch(dst, nq, 0.0, stream);
@@ -375,8 +378,15 @@ hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, doubl
                                                        used_splits > 1 ? partial : dst, nq, tps,
                                                        used_splits, kappa);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess || used_splits == 1) return e;
-  return reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
+  if (e == hipSuccess && used_splits > 1)
+    e = reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
+  if (e != hipSuccess || nbe <= 0) return e;
+  // boundary correction of an open mesh, for the same query blocks
+  RibbonSrc src = *boundary;
+  src.pts = (const double*)rec;
+  const int rs = ribbon_splits(nq, 1, nbe);
+  return ribbon_launch(src, 0, 1, q, perm, dst, nq, rs, cursor, ribbon_workspace_bytes(nq, rs),
+                       stream);
 }
 
 namespace {

@@ -149,9 +149,9 @@ def tune_batch(face_counts, query_counts, tri, Q, precision: int, repeats: int =
 
 def tune_batch_strips(record_counts, query_counts, rec, Q, repeats: int = 2,
                       use_cache: bool = True, chunks=(2, 4, 8, 16, 32),
-                      precision="fp32c") -> int:
+                      precision="fp32c", boundary=None) -> int:
     """Best chunk_tiles (255-record tiles per work item) of a strip batch
-    plan in the given precision."""
+    plan in the given precision (with its BatchBoundary, as it will run)."""
     from .batch import StripBatch, _mode
 
     rc = np.asarray(record_counts, dtype=np.int64)
@@ -159,7 +159,7 @@ def tune_batch_strips(record_counts, query_counts, rec, Q, repeats: int = 2,
     mode = _mode(precision)
 
     def measure(chunk):
-        plan = StripBatch(rc, qc, chunk_tiles=chunk, precision=precision)
+        plan = StripBatch(rc, qc, chunk_tiles=chunk, precision=precision, boundary=boundary)
         try:
             return _time(lambda: plan.run(rec, Q, use_graph=False), repeats)
         finally:

@@ -45,6 +45,8 @@ _SIGNATURES = {
     "wn_mesh_set_vertices": ([I] * 5, I),
     "wn_strip_topology": ([I] * 7, I),
     "wn_strip_gather": ([I] * 8, I),
+    "wn_strip_boundary": ([I] * 9, I),
+    "wn_batch_set_boundary": ([I] * 5, I),
     "wn_boundary_edges": ([I] * 7, I),
     "wn_backward_edges": ([I] * 14, I),
     "wn_mesh_info": ([I] * 2, I),

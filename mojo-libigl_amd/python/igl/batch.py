@@ -15,8 +15,8 @@ Strip plans (``StripBatch``) take the meshes as triangle-strip records built
 once on the host (``pack_batch_strips``) and run the strip kernel, which
 evaluates one vertex per face instead of three:
 
-    rec, counts = pack_batch_strips([V0, V1, ...], [F0, F1, ...])
-    plan = StripBatch(counts, query_counts)
+    rec, counts, boundary = pack_batch_strips([V0, V1, ...], [F0, F1, ...], boundary=True)
+    plan = StripBatch(counts, query_counts, boundary=boundary)
     w = plan.run(rec, Q_all)
 """
 
@@ -107,13 +107,49 @@ class WindingBatch:
             pass
 
 
+class BatchBoundary:
+    """Boundary edges of a strip batch's meshes on the device: record pairs
+    ((E, 2) int32, indices into the batch's records), weights ((E,)
+    float64) and per-mesh edge counts (host). The FP32-compensated strip plan
+    adds their boundary correction (wn_batch_set_boundary)."""
+
+    def __init__(self, pairs, weight, edge_counts):
+        self.pairs = pairs
+        self.weight = weight
+        self.edge_counts = np.asarray(edge_counts, dtype=np.int64)
+        self.edge_off = np.concatenate([[0], np.cumsum(self.edge_counts)]).astype(np.int64)
+
+    @classmethod
+    def from_host(cls, pairs_list, weight_list, record_counts, device="cuda"):
+        """Per-mesh (pairs, weight) with mesh-local record indices ->
+        one device list with batch record indices."""
+        import torch
+
+        off = np.concatenate([[0], np.cumsum(np.asarray(record_counts, dtype=np.int64))])
+        pairs = [np.asarray(p, dtype=np.int64) + off[k] for k, p in enumerate(pairs_list)]
+        if off[-1] > np.iinfo(np.int32).max:
+            raise ValueError("more than 2^31-1 records in the batch")
+        cat = (np.concatenate(pairs).astype(np.int32) if pairs else np.zeros((0, 2), np.int32))
+        w = np.concatenate(weight_list) if weight_list else np.zeros(0)
+        return cls(torch.as_tensor(np.ascontiguousarray(cat.reshape(-1, 2)), device=device),
+                   torch.as_tensor(np.ascontiguousarray(w, dtype=np.float64), device=device),
+                   [len(p) for p in pairs_list])
+
+    @property
+    def n_edges(self) -> int:
+        return int(self.edge_off[-1])
+
+
 class StripBatch(WindingBatch):
     """A planned batch over meshes in strip-record form: FP32-compensated, or
     FP64 with precision="fp64" (the FP64 strip kernel; kappa does not
-    apply)."""
+    apply). ``boundary`` (a BatchBoundary, from pack_batch_strips(...,
+    boundary=True) or BatchTopology.boundary) gives the FP32-compensated plan
+    its open meshes' boundary correction; without it open meshes keep the
+    FP32 rounding of their boundary (errors up to a few 1e-6)."""
 
     def __init__(self, record_counts, query_counts, chunk_tiles=0, sort=-1, kappa=-1.0,
-                 precision="fp32c"):
+                 precision="fp32c", boundary=None):
         rc = np.asarray(record_counts, dtype=np.int64)
         qc = np.asarray(query_counts, dtype=np.int64)
         if rc.ndim != 1 or rc.shape != qc.shape or len(rc) == 0:
@@ -139,6 +175,21 @@ class StripBatch(WindingBatch):
         if handle <= 0:
             raise RuntimeError(f"wn_batch_create_strips failed: {last_error()}")
         self.handle = handle
+        self.boundary = None
+        if boundary is not None:
+            self.set_boundary(boundary)
+
+    def set_boundary(self, boundary) -> None:
+        """Boundary edges of the meshes (a BatchBoundary with one edge count
+        per mesh); the plan keeps a reference while it lives."""
+        if len(boundary.edge_counts) != len(self.query_off) - 1:
+            raise ValueError("boundary must have one edge count per mesh")
+        st = lib().wn_batch_set_boundary(self.handle, boundary.pairs.data_ptr(),
+                                         boundary.weight.data_ptr(), boundary.n_edges,
+                                         boundary.edge_off.ctypes.data)
+        if st != 0:
+            raise RuntimeError(f"wn_batch_set_boundary failed: {last_error()}")
+        self.boundary = boundary
 
     @property
     def n_records(self) -> int:
@@ -155,19 +206,28 @@ def _workers(n: int) -> int:
     return max(1, min(n, 16, os.cpu_count() or 1))
 
 
-def pack_batch_strips(vertices, faces, device="cuda"):
+def pack_batch_strips(vertices, faces, device="cuda", boundary=False):
     """Strip records of every mesh (host stripifier), concatenated on the
-    device: ((R_total, 4) float64 tensor, per-mesh record counts)."""
+    device: ((R_total, 4) float64 tensor, per-mesh record counts), and with
+    boundary=True also the meshes' BatchBoundary."""
     import torch
 
-    from .mesh import strip_records
+    from .mesh import strip_records, strip_records_boundary
 
     # meshes stripify independently; the library calls release the GIL
     with ThreadPoolExecutor(max_workers=_workers(len(vertices))) as pool:
-        recs = list(pool.map(strip_records, vertices, faces))
+        if boundary:
+            parts = list(pool.map(strip_records_boundary, vertices, faces))
+            recs = [p[0] for p in parts]
+        else:
+            recs = list(pool.map(strip_records, vertices, faces))
     counts = np.array([len(r) for r in recs], dtype=np.int64)
     rec = np.concatenate(recs) if recs else np.zeros((0, 4))
-    return torch.as_tensor(np.ascontiguousarray(rec), device=device), counts
+    rec = torch.as_tensor(np.ascontiguousarray(rec), device=device)
+    if not boundary:
+        return rec, counts
+    return rec, counts, BatchBoundary.from_host([p[1] for p in parts], [p[2] for p in parts],
+                                                counts, device=device)
 
 
 class BatchTopology:
@@ -177,7 +237,7 @@ class BatchTopology:
     topology) without re-stripifying.
 
         topo = BatchTopology([F0, F1, ...], [nv0, nv1, ...])
-        plan = StripBatch(topo.record_counts, query_counts)
+        plan = StripBatch(topo.record_counts, query_counts, boundary=topo.boundary)
         w = plan.run(topo.records(V_all), Q_all)
     """
 
@@ -185,31 +245,19 @@ class BatchTopology:
         import torch
 
         from . import _faces
-
-        h = lib()
+        from .mesh import strip_boundary, strip_topology
 
         def topology(F, nv):
             Fa = _faces(F)
-            cap = len(Fa) + len(Fa) // 4 + 2 * 255  # one call when it fits
-            idx = np.empty(cap, dtype=np.int32)
-            flag = np.empty(cap, dtype=np.int8)
-            n = h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, idx.ctypes.data, flag.ctypes.data,
-                                    cap)
-            if n < 0:
-                raise (IndexError if "outside V" in last_error() else ValueError)(last_error())
-            if n > cap:
-                idx = np.empty(n, dtype=np.int32)
-                flag = np.empty(n, dtype=np.int8)
-                if h.wn_strip_topology(addr(Fa), nv, len(Fa), 0, idx.ctypes.data,
-                                       flag.ctypes.data, n) != n:
-                    raise RuntimeError(last_error())
-            return idx[:n], flag[:n]
+            idx, flag = strip_topology(Fa, nv)
+            pairs, weight = strip_boundary(Fa, nv, idx)
+            return idx, flag, pairs, weight
 
         nvs = [int(nv) for nv in vertex_counts]
         with ThreadPoolExecutor(max_workers=_workers(len(nvs))) as pool:
             tops = list(pool.map(topology, faces, nvs))
         ids, flags, counts, off = [], [], [], 0
-        for (idx, flag), nv in zip(tops, nvs):
+        for (idx, flag, _, _), nv in zip(tops, nvs):
             ids.append(idx.astype(np.int64) + off)
             flags.append(flag)
             counts.append(len(idx))
@@ -222,6 +270,10 @@ class BatchTopology:
         self.idx = torch.as_tensor(cat, device=device)
         self.flag = torch.as_tensor(np.concatenate(flags) if flags else np.zeros(0, np.int8),
                                     device=device)
+        # boundary edges as record pairs: they follow the records, so moved
+        # vertices need no new boundary
+        self.boundary = BatchBoundary.from_host([t[2] for t in tops], [t[3] for t in tops],
+                                                self.record_counts, device=device)
 
     def records(self, V_all, out=None):
         """(R_total, 4) float64 records for (n_vertices, 3) float32/float64
@@ -269,8 +321,8 @@ def winding_number_batch(vertices, faces, queries, precision="fp32c", use_graph=
                                         for o in queries]), device="cuda")
     qc = [len(np.asarray(o).reshape(-1, 3)) for o in queries]
     if strips:
-        tri, counts = pack_batch_strips(vertices, faces)
-        plan = StripBatch(counts, qc, precision=precision, **config)
+        tri, counts, boundary = pack_batch_strips(vertices, faces, boundary=True)
+        plan = StripBatch(counts, qc, precision=precision, boundary=boundary, **config)
     else:
         tri = pack_batch(vertices, faces)
         plan = WindingBatch([len(f) for f in faces], qc, precision=precision, **config)

@@ -40,6 +40,67 @@ def strip_records(V, F) -> np.ndarray:
     return out[:n]
 
 
+def strip_topology(F, n_vertices: int):
+    """(vertex id per record int32, flag per record int8) of F's strips
+    (wn_strip_topology)."""
+    from . import _faces
+
+    faces = _faces(F)
+    h = lib()
+    cap = len(faces) + len(faces) // 4 + 2 * 255  # one call when it fits
+    idx = np.empty(cap, dtype=np.int32)
+    flag = np.empty(cap, dtype=np.int8)
+    n = h.wn_strip_topology(addr(faces), int(n_vertices), len(faces), 0, idx.ctypes.data,
+                            flag.ctypes.data, cap)
+    if n < 0:
+        raise (IndexError if "outside V" in last_error() else ValueError)(last_error())
+    if n > cap:
+        idx = np.empty(n, dtype=np.int32)
+        flag = np.empty(n, dtype=np.int8)
+        if h.wn_strip_topology(addr(faces), int(n_vertices), len(faces), 0, idx.ctypes.data,
+                               flag.ctypes.data, n) != n:
+            raise RuntimeError(last_error())
+    return idx[:n], flag[:n]
+
+
+def strip_boundary(F, n_vertices: int, idx) -> tuple[np.ndarray, np.ndarray]:
+    """Net directed boundary edges of F as record pairs of the strip topology
+    idx ((count, 2) int32) and their weights ((count,) float64)
+    (wn_strip_boundary)."""
+    from . import _faces
+
+    faces = _faces(F)
+    idx = np.ascontiguousarray(idx, dtype=np.int32)
+    h = lib()
+    cap = max(16, len(faces) // 8)
+    for _ in range(2):
+        pairs = np.empty((cap, 2), dtype=np.int32)
+        weight = np.empty(cap, dtype=np.float64)
+        n = h.wn_strip_boundary(addr(faces), int(n_vertices), len(faces), 0, idx.ctypes.data,
+                                len(idx), pairs.ctypes.data, weight.ctypes.data, cap)
+        if n < 0:
+            raise (IndexError if "outside V" in last_error() else ValueError)(last_error())
+        if n <= cap:
+            return pairs[:n], weight[:n]
+        cap = n
+    raise RuntimeError("wn_strip_boundary: edge count changed between calls")
+
+
+def strip_records_boundary(V, F):
+    """Strip records of (V, F) as strip_records gives them, with the mesh's
+    boundary edges as record pairs and weights (the FP32-compensated boundary
+    correction of a strip batch plan)."""
+    from . import _mesh
+
+    vertices, faces = _mesh(V, F, three_dimensional=True)
+    idx, flag = strip_topology(faces, len(vertices))
+    rec = np.empty((len(idx), 4), dtype=np.float64)
+    rec[:, :3] = vertices[idx]
+    rec[:, 3] = flag
+    pairs, weight = strip_boundary(faces, len(vertices), idx)
+    return rec, pairs, weight
+
+
 class Mesh:
     """A mesh resident on the GPU in strip-record form."""
 

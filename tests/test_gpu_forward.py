@@ -123,15 +123,15 @@ def test_ragged_sizes(require_gpu, c1, nf_take, nq):
     """Odd face and query counts (tails of the LDS tile and of the query
     block). Prefixes of the face list are OPEN patches: there w is not
     locally constant and the FP32 rounding of the block-local coordinates
-    (~6e-8 * block extent, here the whole [-1.5, 1.5]^3 box) moves w by
-    |grad w| times that, ~1e-6 within 1e-2 of a boundary edge; the
-    FP32-compensated bar for open patches is therefore 1e-5."""
+    (~6e-8 * block extent, here the whole [-1.5, 1.5]^3 box) moves the
+    boundary, which the boundary correction undoes; the FP32-compensated bar
+    is 1e-6 as for closed meshes."""
     V, F, Q, tri, ref = c1
     Fs = F[:nf_take]
     Qs = Q[:nq]
     t = oracle.pack_soa(V, Fs)
     r = oracle.winding_soa(t, Qs)
-    tol32 = FP32C_TOL if nf_take == len(F) else 1e-5
+    tol32 = FP32C_TOL
     for precision, tol in ((0, FP64_TOL), (1, tol32)):
         for splits in (1, 3):
             out = host_call(t, Qs, precision, 1, splits, 0)

@@ -106,11 +106,14 @@ def soup(seed):
 @pytest.mark.parametrize("seed", [1, 2])
 def test_triangle_soup(require_gpu, path, seed):
     """Every GPU path on a non-manifold soup against the oracle: FP64 at its
-    bar, FP32c at the open-mesh bar 1e-5 (no closed surface, so w is not an
-    integer and near-edge queries move with the FP32 block-local rounding)."""
+    bar, FP32c at 2e-6. The soup's 6,400 random faces overlap at every query
+    (sum of |angle| over the faces ~440 rad per query, against 2 pi for a
+    closed surface), so the FP32 rounding of each pair's angle (~eps |angle|)
+    adds up to ~3.4e-7 rms in w; the boundary rounding itself is corrected
+    (before the boundary correction these soups read up to 7.3e-6)."""
     V, F, Q = soup(seed)
     ref = oracle.winding(V, F, Q)
     err = np.abs(run(path, V, F, Q) - ref).max()
-    bar = BARS[path] if path.startswith("fp64") else 1e-5
+    bar = BARS[path] if path.startswith("fp64") else 2e-6
     print(f"soup {seed} {path}: max|w - oracle| {err:.3e}", flush=True)
     assert err <= bar
