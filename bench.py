@@ -29,6 +29,7 @@ Other workloads (one JSON line each, same timing method):
   --workload fast    hierarchical winding number on the 1.31M-face sphere x
                      1M queries, per accuracy parameter, vs all-pairs
   --workload copy    256 MiB device copy (calibrates the byte counters)
+  --workload plain64 the plain FP64 kernel alone on the torus workload
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
 """
@@ -168,6 +169,24 @@ def timed(fn, steps, warmup):
     return start.elapsed_time(end) / steps, wall / steps * 1e3
 
 
+def timed_kernel(h, fn, steps, warmup):
+    """timed(), plus the average device duration of the dominant kernel of
+    each call over the timed steps: the library's kernel timer records HIP
+    events on the stream that kernel is launched on, right before and after
+    it. Returns (ms/step, wall ms/step, kernel ms/launch, launches)."""
+    import numpy as np
+
+    for _ in range(warmup):
+        fn()
+    out = np.zeros(2)
+    h.wn_kernel_timer(1, 0)
+    ms, wall = timed(fn, steps, 0)
+    h.wn_kernel_timer(2, out.ctypes.data)
+    h.wn_kernel_timer(0, 0)
+    n = int(out[0])
+    return ms, wall, (out[1] / n if n else float("nan")), n
+
+
 def base_line(args, metric, value, unit, ms, dtype, workload, **extra):
     line = {
         "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
@@ -199,10 +218,18 @@ def measured_traffic(kernel: str):
     return None
 
 
-def roofline(pairs, ms, peak=FP32_VECTOR_PEAK_TFLOPS, kernel=None):
+def roofline(pairs, ms, peak=FP32_VECTOR_PEAK_TFLOPS, kernel=None, launches=None,
+             torus_traffic=True):
+    """66 algorithmic FLOP per pair over `ms`, the kernel's average launch
+    duration (kernel timer) where one is given, against `peak`; traffic from
+    the torus workload's counter summary when the workload is the torus."""
     achieved = pairs * FLOP_PER_PAIR / (ms * 1e-3) / 1e12
-    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": measured_traffic(kernel) if kernel else None}
+    traffic = measured_traffic(kernel) if kernel and torus_traffic else None
+    line = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic}
+    if kernel:
+        line.update(kernel=kernel, kernel_ms=ms, launches=launches)
+    return line
 
 
 # ---------------------------------------------------------------------------
@@ -265,42 +292,44 @@ def run_torus(args, h):
     strip = MeshForward(h, mesh, Qd, args.splits)
     log(f"strip path: {info['records']} records for {nf} faces (built in {create_ms:.1f} ms), "
         f"splits={strip.splits} sort={strip.sort}; {args.warmup} warm-up + {args.steps} timed steps")
-    ms_strip, wall_strip = timed(strip, args.steps, args.warmup)
-    log(f"fp32c strip {ms_strip:.3f} ms/step (device), {wall_strip:.3f} ms/step (wall)")
+    ms_strip, wall_strip, k_strip, n_strip = timed_kernel(h, strip, args.steps, args.warmup)
+    log(f"fp32c strip {ms_strip:.3f} ms/step (device), {wall_strip:.3f} ms/step (wall), "
+        f"fwd_strip_kernel {k_strip:.3f} ms/launch over {n_strip} launches")
     w_strip = strip.out.cpu().numpy()
 
     face = Forward(h, trid, Qd, 1, args.qpl, args.splits)
-    ms_face, _ = timed(face, args.steps, args.warmup)
-    log(f"fp32c face-record kernel {ms_face:.3f} ms/step (qpl={face.qpl} splits={face.splits})")
+    ms_face, _, k_face, _ = timed_kernel(h, face, args.steps, args.warmup)
+    log(f"fp32c face-record kernel {ms_face:.3f} ms/step (qpl={face.qpl} splits={face.splits}), "
+        f"{k_face:.3f} ms/launch")
     w_face = face.out.cpu().numpy()
 
     result_fp64, acc = None, {}
     if not args.no_fp64:
         f64 = Forward(h, trid, Qd, 0)
-        ms64, _ = timed(f64, args.steps, args.warmup)
-        log(f"fp64 plain {ms64:.3f} ms/step (device)")
+        ms64, _, k64, _ = timed_kernel(h, f64, args.steps, args.warmup)
+        log(f"fp64 plain {ms64:.3f} ms/step (device), fwd_f64_kernel {k64:.3f} ms/launch")
         w64 = f64.out.cpu().numpy()
         d = np.abs(w_strip - w64)
         acc = {"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw_fp32c_vs_fp64": float(d.mean()),
                "max_abs_dw_face_kernel_vs_fp64": float(np.abs(w_face - w64).max()),
                "fp64_max_dist_from_integer": float(np.abs(w64 - np.round(w64)).max())}
         result_fp64 = {"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64, "qpl": f64.qpl,
-                       "splits": f64.splits,
-                       "roofline_frac_fp64_peak": roofline(nf * nq, ms64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
+                       "splits": f64.splits, "kernel_ms": k64,
+                       "roofline_frac_fp64_peak": roofline(nf * nq, k64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
         sub = np.arange(0, nq, max(1, nq // 512))
         ref = oracle.winding_soa(tri, Q[sub])
         acc["max_abs_dw_fp64_vs_oracle_sub512"] = float(np.abs(w64[sub] - ref).max())
         acc["max_abs_dw_fp32c_vs_oracle_sub512"] = float(np.abs(w_strip[sub] - ref).max())
         s64 = MeshForward(h, mesh, Qd, args.splits, fp64=True)
-        ms_s64, _ = timed(s64, args.steps, args.warmup)
-        log(f"fp64 strip {ms_s64:.3f} ms/step (device)")
+        ms_s64, _, k_s64, _ = timed_kernel(h, s64, args.steps, args.warmup)
+        log(f"fp64 strip {ms_s64:.3f} ms/step (device), {k_s64:.3f} ms/launch")
         w_s64 = s64.out.cpu().numpy()
         acc["max_abs_dw_fp64_strip_vs_fp64"] = float(np.abs(w_s64 - w64).max())
         acc["max_abs_dw_fp64_strip_vs_oracle_sub512"] = float(np.abs(w_s64[sub] - ref).max())
         result_fp64["strip"] = {"value": nf * nq / (ms_s64 * 1e-3) / 1e9, "ms_per_step": ms_s64,
-                                "speedup_vs_fp64_plain": ms64 / ms_s64,
+                                "kernel_ms": k_s64, "speedup_vs_fp64_plain": ms64 / ms_s64,
                                 "roofline_frac_fp64_peak":
-                                    roofline(nf * nq, ms_s64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
+                                    roofline(nf * nq, k_s64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
         # the drop-in call from host arrays, end to end (large calls take the
         # strip route: host stripification and upload unless the library still
         # holds this mesh from the previous call, strip kernel, download):
@@ -321,8 +350,21 @@ def run_torus(args, h):
         result_fp64["host_call_end_to_end_ms"] = e2e64
         result_fp64["host_call_first_ms"] = first64
         acc["max_abs_dw_host_call_vs_fp64"] = float(np.abs(w_e2e - w64).max())
+        kt = np.zeros(2)
+        h.wn_kernel_timer(1, 0)
         first32, e2e32, w_e2e32 = host_calls("fp32c")
+        h.wn_kernel_timer(2, kt.ctypes.data)
+        h.wn_kernel_timer(0, 0)
         acc["max_abs_dw_host_call_fp32c_vs_fp64"] = float(np.abs(w_e2e32 - w64).max())
+        # the same strip step again, after the FP64 runs and the host calls:
+        # the kernel's duration on a device that has just run FP64 work
+        _, _, k_after, n_after = timed_kernel(h, strip, max(1, args.steps // 2), 0)
+        log(f"fwd_strip_kernel in the host calls {kt[1] / max(kt[0], 1):.3f} ms/launch "
+            f"({int(kt[0])} launches); strip step again after them {k_after:.3f} ms/launch")
+        strip_kernel_later = {"host_call_launches": int(kt[0]),
+                              "host_call_kernel_ms": kt[1] / max(kt[0], 1),
+                              "strip_step_again_kernel_ms": k_after,
+                              "strip_step_again_launches": n_after}
     mesh.close()
 
     line = base_line(
@@ -330,17 +372,24 @@ def run_torus(args, h):
         f"closed torus R=1 r=0.35 120x118 ({nf} faces) x {nq} queries uniform in the 20%-padded "
         "bbox; FP32-compensated forward (FP32 pairs, FP64 accumulation, FP64 re-evaluation of "
         "ill-conditioned pairs) over the resident triangle-strip mesh, inputs resident in HBM",
-        roofline=roofline(nf * nq, ms_strip, kernel="wn::fwd_strip_kernel"), fp64_plain=result_fp64,
+        roofline=roofline(nf * nq, k_strip, kernel="wn::fwd_strip_kernel", launches=n_strip),
+        fp64_plain=result_fp64,
         traffic_source="bytes per launch of wn::fwd_strip_kernel from profiles/fwd_torus28k_1M_counters.md "
                        "(rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 256 MiB copy)",
         speedup_fp32c_vs_fp64=(result_fp64["ms_per_step"] / ms_strip) if result_fp64 else None,
+        # the metric's ratio is FP32c strips over the plain FP64 kernel; the
+        # same-algorithm ratios put each FP32c kernel beside its FP64 twin
+        speedup_same_algorithm=({"face_vs_face": result_fp64["ms_per_step"] / ms_face,
+                                 "strip_vs_strip": result_fp64["strip"]["ms_per_step"] / ms_strip}
+                                if result_fp64 else None),
         fp32c_face_kernel={"value": nf * nq / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face,
-                           "qpl": face.qpl, "splits": face.splits},
+                           "kernel_ms": k_face, "qpl": face.qpl, "splits": face.splits},
         accuracy=acc)
     line["config"].update(faces=nf, queries=nq, strip_records=info["records"], splits=strip.splits,
                           morton_sort=bool(strip.sort), mesh_build_ms=create_ms,
                           wall_ms_per_step=wall_strip)
     if result_fp64 is not None:
+        line["strip_kernel_after_fp64"] = strip_kernel_later
         # igl.winding_number(V, F, Q, device="gpu", precision="fp32c") from
         # host arrays, repeated on the same mesh (kernel, query upload, result
         # download) and first on it (plus stripify and mesh upload)
@@ -396,22 +445,34 @@ def run_batch(args, h):
     log(f"batched face records fp32c (graph) {ms_face:.3f} ms/step; qpl={qpl} chunk={chunk}")
     w_face = out.cpu().numpy()
 
-    # 512 separate launches of the single-mesh forward (per-mesh SoA packed
-    # beforehand, outside the timed region)
-    singles = []
+    # 512 separate launches of the same strip kernels (per-mesh resident
+    # strip meshes, built outside the timed region), and of the face-record
+    # forward (per-mesh SoA packed beforehand; each call also finds its
+    # mesh's boundary edges)
+    from igl.mesh import Mesh
+
+    meshes, singles, faces_single = [], [], []
     off = 0
     for V, F, Qm in B:
+        Qs = Q[off:off + len(Qm)]
+        if len(F):
+            meshes.append(Mesh(V, F))
+            singles.append(MeshForward(h, meshes[-1], Qs))
         tri_m = pack_faces(torch.tensor(V, device="cuda"), torch.tensor(F, device="cuda"))
-        singles.append(Forward(h, tri_m, Q[off:off + len(Qm)], 1))
+        faces_single.append(Forward(h, tri_m, Qs, 1))
         off += len(Qm)
 
-    def all_single():
-        for f in singles:
+    def run_all(calls):
+        for f in calls:
             f()
 
-    ms_sep, _ = timed(all_single, args.steps, args.warmup)
-    log(f"512 separate launches {ms_sep:.3f} ms/step")
-    w_sep = np.concatenate([f.out.cpu().numpy() for f in singles])
+    ms_sep, _ = timed(lambda: run_all(singles), args.steps, args.warmup)
+    log(f"512 separate strip launches {ms_sep:.3f} ms/step")
+    ms_sep_face, _ = timed(lambda: run_all(faces_single), args.steps, args.warmup)
+    log(f"512 separate face-record launches {ms_sep_face:.3f} ms/step")
+    w_sep = np.concatenate([f.out.cpu().numpy() for f in faces_single])
+    for m in meshes:
+        m.close()
     plan64 = WindingBatch(fc, qc, precision="fp64", qpl=1, chunk_faces=chunk)
     w64 = plan64.run(tri, Q, use_graph=False).cpu().numpy()
     ms64, _ = timed(lambda: plan64.run(tri, Q, out=out, use_graph=True), max(1, args.steps // 2), 1)
@@ -433,7 +494,11 @@ def run_batch(args, h):
         "replayed from a hipGraph",
         roofline=roofline(pairs, ms_graph),
         face_records_batched={"value": pairs / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face},
-        separate_launches={"value": pairs / (ms_sep * 1e-3) / 1e9, "ms_per_step": ms_sep},
+        separate_launches={"value": pairs / (ms_sep * 1e-3) / 1e9, "ms_per_step": ms_sep,
+                           "kind": "wn_mesh_forward per mesh (the batch's strip kernels)"},
+        separate_face_launches={"value": pairs / (ms_sep_face * 1e-3) / 1e9,
+                                "ms_per_step": ms_sep_face,
+                                "kind": "wn_forward FP32c per mesh (face records)"},
         speedup_batched_vs_separate=ms_sep / ms_graph,
         fp64_batched={"value": pairs / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64,
                       "strips": {"value": pairs / (ms64s * 1e-3) / 1e9, "ms_per_step": ms64s,
@@ -450,6 +515,29 @@ def run_batch(args, h):
                   "max_abs_dw_closed_meshes": float(d[~open_mesh].max(initial=0.0)),
                   "max_abs_dw_face_batch_vs_fp64": float(np.abs(w_face - w64).max()),
                   "max_abs_dw_batched_vs_separate": float(np.abs(w_face - w_sep).max())})
+
+
+def run_plain64(args, h):
+    """The plain FP64 face kernel alone on the torus workload (wn_forward,
+    precision 0): the comparison point of the metric, for profiling."""
+    import torch
+
+    import igl
+    import wn_workloads as W
+
+    V, F, Q = W.config2(args.queries)
+    tri = igl.pack_faces(V, F)
+    nf, nq = len(F), len(Q)
+    f64 = Forward(h, torch.tensor(tri, device="cuda"), torch.tensor(Q, device="cuda"), 0)
+    ms, _, k, n = timed_kernel(h, f64, args.steps, args.warmup)
+    log(f"fp64 plain {ms:.3f} ms/step, fwd_f64_kernel {k:.3f} ms/launch")
+    return base_line(
+        args, "G query-face pairs/s, FP64 plain kernel, 1 MI355X", nf * nq / (ms * 1e-3) / 1e9,
+        "G query-face pairs/s", ms, "f64",
+        f"closed torus 120x118 ({nf} faces) x {nq} queries; plain FP64 face kernel (wn_forward "
+        f"precision 0, qpl {f64.qpl}, {f64.splits} splits)",
+        roofline=roofline(nf * nq, k, FP64_VECTOR_PEAK_TFLOPS, kernel="wn::fwd_f64_kernel<1>",
+                          launches=n, torus_traffic=False))
 
 
 def run_stress(args, h):
@@ -471,7 +559,7 @@ def run_stress(args, h):
     mesh = Mesh(V, F)
     create_ms = (time.perf_counter() - t0) * 1e3
     strip = MeshForward(h, mesh, Qd)
-    ms32, _ = timed(strip, args.steps, min(args.warmup, 1))
+    ms32, _, k32, n32 = timed_kernel(h, strip, args.steps, min(args.warmup, 1))
     w32 = strip.out.cpu().numpy()
     face = Forward(h, trid, Qd, 1)
     ms_face, _ = timed(face, max(1, args.steps // 2), min(args.warmup, 1))
@@ -500,7 +588,8 @@ def run_stress(args, h):
         nf * nq / (ms32 * 1e-3) / 1e9, "G query-face pairs/s", ms32, "f32",
         f"displaced icosphere level 8 ({nf} faces) x {nq} queries (half uniform, half at "
         "|d| in [1e-7, 1e-2] along the normal); FP32-compensated strip kernel",
-        roofline=roofline(nf * nq, ms32),
+        roofline=roofline(nf * nq, k32, kernel="wn::fwd_strip_kernel", launches=n32,
+                          torus_traffic=False),
         fp64_plain={"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64},
         fp32c_face_kernel={"value": nf * nq / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face},
         speedup_fp32c_vs_fp64=ms64 / ms32, mesh_build_ms=create_ms,
@@ -783,7 +872,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="torus",
-                    choices=("torus", "batch", "stress", "grad", "readme", "aabb", "fast", "copy"))
+                    choices=("torus", "batch", "stress", "grad", "readme", "aabb", "fast", "copy",
+                             "plain64"))
     ap.add_argument("--queries", type=int, default=1_048_576)
     ap.add_argument("--qpl", type=int, default=0, help="queries per lane (0 = library default)")
     ap.add_argument("--splits", type=int, default=0, help="face splits (0 = library default)")
@@ -806,7 +896,7 @@ def main() -> int:
         return 1
     runner = {"torus": run_torus, "batch": run_batch, "stress": run_stress, "grad": run_grad,
               "aabb": run_aabb, "fast": run_fast, "readme": run_readme,
-              "copy": run_copy}[args.workload]
+              "copy": run_copy, "plain64": run_plain64}[args.workload]
     print(json.dumps(runner(args, h)), flush=True)
     return 0
 

@@ -40,9 +40,13 @@ void mli_winding(const double *tri, const double *q, double *dst, int64_t nf,
                  int64_t nq, int64_t q0, int64_t q1);
 
 /* Replaces mli_winding_gpu, src/capi.mojo:945-947 (python/igl/_lib.py:23,
- * 5 x c_int64 -> c_int64). GPU, FP64 plain. (host) tri, q, dst. Returns 1 on
- * success, 0 on any failure (reason in wn_last_error()), as the reference's
- * try_winding_gpu does (src/capi.mojo:366-398). Thread-safe. */
+ * 5 x c_int64 -> c_int64). GPU, FP64. (host) tri, q, dst. Below 65,536
+ * queries the plain FP64 face kernel; from there the FP64 strip kernel (the
+ * faces' shared corners found by coordinate equality, the strip mesh kept
+ * for a repeated call as wn_winding_gpu_mesh keeps it), which agrees with
+ * the plain kernel to ~1e-15. Returns 1 on success, 0 on any failure (reason
+ * in wn_last_error()), as the reference's try_winding_gpu does
+ * (src/capi.mojo:366-398). Thread-safe. */
 int64_t mli_winding_gpu(const double *tri, const double *q, double *dst,
                         int64_t nf, int64_t nq);
 
@@ -75,6 +79,14 @@ const char *wn_last_error(void);
 
 /* ABI version (2: adds wn_mesh_forward_noalloc). */
 int64_t wn_version(void);
+
+/* No reference counterpart: measurement. op 1 starts timing the dominant
+ * kernel of every forward call (face, strip and FP64 strip kernels; HIP
+ * events recorded on the launch stream right before and after the kernel;
+ * nothing on a stream under graph capture), op 0 stops, op 2 writes out2 =
+ * {launches, total milliseconds} of the launches timed since the last op
+ * (waiting for their events) and starts a new count. */
+int64_t wn_kernel_timer(int64_t op, double *out2);
 
 /* ---- Extended forward --------------------------------------------------- */
 

@@ -23,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "wn.h"
 #include "wn_internal.h"
 
 namespace {
@@ -485,9 +486,96 @@ void* noalloc_scratch(int device, std::string& why) {
   return ptr[device];
 }
 
+// The kernel timer (wn_kernel_timer): event pairs of the timed launches.
+struct KernelTimer {
+  std::mutex mu;
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> done;
+};
+
+KernelTimer& kernel_timer() {
+  static KernelTimer t;
+  return t;
+}
+
+thread_local hipEvent_t t_open = nullptr;  // begin event of this thread's launch
+
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+void timer_clear(KernelTimer& t) {
+  for (auto& p : t.done) {
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  t.done.clear();
+}
+
 }  // namespace
 
+namespace wn {
+
+void timer_begin(hipStream_t s) {
+  KernelTimer& t = kernel_timer();
+  {
+    std::lock_guard<std::mutex> lock(t.mu);
+    if (!t.on) return;
+  }
+  if (capturing(s)) return;
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  if (hipEventRecord(e, s) != hipSuccess) {
+    (void)hipEventDestroy(e);
+    return;
+  }
+  t_open = e;
+}
+
+void timer_end(hipStream_t s) {
+  if (!t_open) return;
+  hipEvent_t b = t_open, e = nullptr;
+  t_open = nullptr;
+  if (hipEventCreate(&e) != hipSuccess || hipEventRecord(e, s) != hipSuccess) {
+    (void)hipEventDestroy(b);
+    if (e) (void)hipEventDestroy(e);
+    return;
+  }
+  KernelTimer& t = kernel_timer();
+  std::lock_guard<std::mutex> lock(t.mu);
+  t.done.push_back({b, e});
+}
+
+}  // namespace wn
+
 extern "C" {
+
+int64_t wn_kernel_timer(int64_t op, double* out2) {
+  KernelTimer& t = kernel_timer();
+  std::lock_guard<std::mutex> lock(t.mu);
+  if (op == 0 || op == 1) {
+    timer_clear(t);
+    t.on = op == 1;
+    return kOk;
+  }
+  if (op != 2) return fail(kErrArg, "op must be 0 (off), 1 (on) or 2 (read)");
+  if (!out2) return fail(kErrArg, "null output");
+  double total = 0.0;
+  int64_t n = 0;
+  for (auto& p : t.done) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(p.second) == hipSuccess &&
+        hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess) {
+      total += ms;
+      ++n;
+    }
+  }
+  timer_clear(t);
+  out2[0] = (double)n;
+  out2[1] = total;
+  return kOk;
+}
 
 int64_t wn_version(void) { return 2; }
 
@@ -503,8 +591,22 @@ int64_t wn_device_free_bytes(void) {
   return (int64_t)free_b;
 }
 
+// mli_winding_gpu calls with at least this many queries take the FP64 strip
+// kernel (the corners' vertex identity recovered from the face SoA by
+// coordinate equality, then the host strip route of wn_winding_gpu_mesh,
+// which keeps the mesh for the next call): the strip kernel saves ~2.3 ps
+// per query-face pair over the plain FP64 kernel and the host side costs
+// ~0.3 us per face, the break-even of igl.winding_number's strip route.
+constexpr int64_t kTriStripMinQueries = 65536;
+
 int64_t mli_winding_gpu(const double* tri, const double* q, double* dst, int64_t nf,
                         int64_t nq) {
+  if (nq >= kTriStripMinQueries && nf > 0 && nf <= (int64_t)0x7fffffff && tri && q && dst) {
+    std::vector<double> V;
+    std::vector<int32_t> F;
+    const int64_t nv = wn::soa_vertices(tri, nf, V, F);
+    return wn_winding_gpu_mesh(V.data(), F.data(), nv, nf, 1, q, dst, nq, wn::kFP64) == kOk ? 1 : 0;
+  }
   const wn::FwdConfig cfg = make_config(nf, nq, wn::kFP64, 0, 0, 0, 1.0);
   HostMesh mesh;
   mesh.tri = tri;

@@ -309,6 +309,7 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
                       kBlockThreads, 0, stream>>>(dst, nq, 0.0);
     return hipGetLastError();
   }
+  timer_begin(stream);
   if (cfg.precision == kFP64) {
     if (cfg.qpl == 1)
       fwd_f64_kernel<1><<<grid, kBlockThreads, 0, stream>>>(tri, q, out, nf, nq, fps, cfg.splits);
@@ -326,6 +327,7 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
                                                              cfg.splits, cfg.kappa);
   }
   hipError_t e = hipGetLastError();
+  timer_end(stream);
   if (e != hipSuccess) return e;
   if (cfg.splits > 1) e = reduce_splits_launch(partial, perm, dst, nq, cfg.splits, stream);
   if (e != hipSuccess || cfg.precision != kFP32C) return e;

@@ -34,6 +34,12 @@ struct FwdConfig {
   float kappa = 1.0f;     // kFP32C: conditioning threshold for the FP64 re-evaluation
 };
 
+// Device timing of the dominant kernel of each forward call, off unless
+// enabled (wn_kernel_timer): HIP events recorded on the launch stream right
+// before and after the kernel; nothing is recorded on a capturing stream.
+void timer_begin(hipStream_t s);
+void timer_end(hipStream_t s);
+
 // Rounds to 256 bytes so sub-buffers stay aligned.
 inline size_t align_up(size_t n, size_t a = 256) { return (n + a - 1) / a * a; }
 
@@ -52,6 +58,10 @@ int64_t strip_records(const double* V, const void* F, int64_t nf, int fdtype,
 // float64, 1 = float32): rec[k] = (V[idx[k]], flag[k]).
 int64_t strip_topology(const void* F, int64_t nf, int fdtype, std::vector<int32_t>& idx,
                        std::vector<int8_t>& flag);
+// Distinct corners of a face SoA (tri, 9 x nf) by exact coordinate equality
+// as V (nv x 3) and the faces' corner ids as F (nf x 3 int32); returns nv.
+int64_t soa_vertices(const double* tri, int64_t nf, std::vector<double>& V,
+                     std::vector<int32_t>& F);
 hipError_t strip_gather_launch(const void* V, int vdtype, int64_t nv, const int32_t* idx,
                                const int8_t* flag, int64_t nrec, double4* rec,
                                hipStream_t stream);

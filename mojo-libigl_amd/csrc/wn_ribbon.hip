@@ -292,7 +292,7 @@ __device__ __forceinline__ void ribbon_block(const RibbonSrc& src, const double*
 // evenly by the splits. One split adds w += sum / (2 pi) through perm; more
 // write partials part[split * nq + slot] for ribbon_reduce_kernel.
 template <int SRC, int QPL>
-__global__ __launch_bounds__(kBlockThreads) void ribbon_kernel(
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8))) void ribbon_kernel(
     RibbonSrc src, const double* __restrict__ q, const int32_t* __restrict__ perm,
     double* __restrict__ dst, double* __restrict__ part, int64_t nq) {
   __shared__ RibbonSmem sm;
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kBlockThreads) void ribbon_reduce_kernel(
 // Batch: one block per (mesh, query tile) of the plan's table; the mesh's
 // edges [e_begin[m], e_end[m]).
 template <int SRC, int QPL>
-__global__ __launch_bounds__(kBlockThreads) void ribbon_batch_kernel(
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8))) void ribbon_batch_kernel(
     RibbonSrc src, const int32_t* __restrict__ e_begin, const int32_t* __restrict__ e_end,
     const int2* __restrict__ blocks, const BatchMesh* __restrict__ meshes,
     const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ dst) {
@@ -523,11 +523,18 @@ __global__ __launch_bounds__(kBlockThreads) void edge_bounds_kernel(
   if (i == n - 1 || seg_of_face(face_off, nseg, ids[2 * i + 2]) != s) e_end[s] = (int32_t)(i + 1);
 }
 
-size_t sort_temp64(int64_t n) {
+// Key bits the sort must order: the 40-bit hash plus the segment's bits.
+int key_bits(int nseg) {
+  int b = 40;
+  while (nseg > 1 && (1 << (b - 40)) < nseg) ++b;
+  return b;
+}
+
+size_t sort_temp64(int64_t n, int bits) {
   size_t t = 0;
   (void)hipcub::DeviceRadixSort::SortPairs(nullptr, t, (const unsigned long long*)nullptr,
                                      (unsigned long long*)nullptr, (const int32_t*)nullptr,
-                                     (int32_t*)nullptr, (int)n, 0, 64);
+                                     (int32_t*)nullptr, (int)n, 0, bits);
   return t;
 }
 
@@ -549,7 +556,7 @@ size_t soa_boundary_workspace_bytes(int64_t nf, int nseg) {
   b += 2 * align_up(sizeof(unsigned long long) * (size_t)ne);  // keys in / out
   b += 2 * align_up(sizeof(int32_t) * (size_t)ne);             // vals in / out
   b += 2 * align_up(sizeof(int32_t) * (size_t)ne);             // counts, offsets
-  b += align_up(std::max(sort_temp64(ne), scan_temp(ne)));
+  b += align_up(std::max(sort_temp64(ne, key_bits(nseg)), scan_temp(ne)));
   b += 2 * align_up(sizeof(int32_t) * (size_t)ne);  // ids (2 x ne)
   b += align_up(sizeof(double) * (size_t)ne);       // weights
   b += align_up(sizeof(int32_t));                   // count
@@ -583,7 +590,8 @@ hipError_t soa_boundary_launch(const double* tri, int64_t nf, const int64_t* fac
   int32_t* v_out = (int32_t*)take(sizeof(int32_t) * ne);
   int32_t* counts = (int32_t*)take(sizeof(int32_t) * ne);
   int32_t* offs = (int32_t*)take(sizeof(int32_t) * ne);
-  const size_t temp_b = std::max(sort_temp64(ne), scan_temp(ne));
+  const int bits = key_bits(face_off ? nseg : 1);
+  const size_t temp_b = std::max(sort_temp64(ne, bits), scan_temp(ne));
   void* temp = take(temp_b);
   r.ids = (int32_t*)take(2 * sizeof(int32_t) * ne);
   r.w = (double*)take(sizeof(double) * ne);
@@ -595,7 +603,7 @@ hipError_t soa_boundary_launch(const double* tri, int64_t nf, const int64_t* fac
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   size_t tb = temp_b;
-  e = hipcub::DeviceRadixSort::SortPairs(temp, tb, k_in, k_out, v_in, v_out, (int)ne, 0, 64, s);
+  e = hipcub::DeviceRadixSort::SortPairs(temp, tb, k_in, k_out, v_in, v_out, (int)ne, 0, bits, s);
   if (e != hipSuccess) return e;
   edge_net_kernel<false><<<grid_of(ne), kBlockThreads, 0, s>>>(tri, nf, ne, k_out, v_out, counts,
                                                                 nullptr, nullptr, nullptr);

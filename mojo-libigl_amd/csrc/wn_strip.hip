@@ -374,10 +374,12 @@ hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, doubl
   const int64_t tps = (ntiles + splits - 1) / splits;
   const int used_splits = (int)((ntiles + tps - 1) / tps);
   const dim3 grid((unsigned)((nq + kBlockThreads - 1) / kBlockThreads), (unsigned)used_splits);
+  timer_begin(stream);
   fwd_strip_kernel<<<grid, kBlockThreads, 0, stream>>>(rec, ntiles, q, perm,
                                                        used_splits > 1 ? partial : dst, nq, tps,
                                                        used_splits, kappa);
   hipError_t e = hipGetLastError();
+  timer_end(stream);
   if (e == hipSuccess && used_splits > 1)
     e = reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
   if (e != hipSuccess || nbe <= 0) return e;

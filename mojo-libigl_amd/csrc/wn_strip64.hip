@@ -306,9 +306,11 @@ hipError_t strip64_launch(const StripRec64* rec64, const double4* rec, int64_t n
   const int64_t tps = (ntiles + splits - 1) / splits;
   const int used_splits = (int)((ntiles + tps - 1) / tps);
   const dim3 grid((unsigned)((nq + kBlockThreads - 1) / kBlockThreads), (unsigned)used_splits);
+  timer_begin(stream);
   fwd_strip64_kernel<<<grid, kBlockThreads, 0, stream>>>(
       rec64, rec, ntiles, q, perm, used_splits > 1 ? partial : dst, nq, tps, used_splits);
   hipError_t e = hipGetLastError();
+  timer_end(stream);
   if (e != hipSuccess || used_splits == 1) return e;
   return reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
 }

@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "wn_radix.h"
@@ -239,6 +241,56 @@ int64_t strip_records(const double* V, const void* F, int64_t nf, int fdtype,
     out[(size_t)(4 * k + 3)] = (double)flag[(size_t)k];
   }
   return n;
+}
+
+// Vertices of a face SoA (9 x nf, rows ax..cz) by exact coordinate equality
+// (-0 and +0 count as equal): V gets each distinct corner once, in order of
+// first appearance (face by face, corners a, b, c), F the corner ids. The
+// corners that the kernels round identically are exactly these.
+int64_t soa_vertices(const double* tri, int64_t nf, std::vector<double>& V,
+                     std::vector<int32_t>& F) {
+  struct Key {
+    uint64_t x, y, z;
+    bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
+  };
+  struct Hash {
+    size_t operator()(const Key& k) const {
+      uint64_t h = k.x * 0x9e3779b97f4a7c15ull;
+      h ^= (k.y + 0x632be59bd9b4e019ull + (h << 6) + (h >> 2));
+      h *= 0xbf58476d1ce4e5b9ull;
+      h ^= (k.z + 0x94d049bb133111ebull + (h << 6) + (h >> 2));
+      return (size_t)(h ^ (h >> 31));
+    }
+  };
+  auto bits = [](double v) {
+    v += 0.0;  // -0 -> +0
+    uint64_t b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+  };
+  std::unordered_map<Key, int32_t, Hash> ids;
+  ids.reserve((size_t)(nf * 3 / 2 + 16));
+  V.clear();
+  F.assign((size_t)(3 * nf), 0);
+  for (int64_t f = 0; f < nf; ++f)
+    for (int c = 0; c < 3; ++c) {
+      const double x = tri[(3 * c) * nf + f], y = tri[(3 * c + 1) * nf + f],
+                   z = tri[(3 * c + 2) * nf + f];
+      const Key k{bits(x), bits(y), bits(z)};
+      auto it = ids.find(k);
+      int32_t id;
+      if (it == ids.end()) {
+        id = (int32_t)(V.size() / 3);
+        ids.emplace(k, id);
+        V.push_back(x + 0.0);
+        V.push_back(y + 0.0);
+        V.push_back(z + 0.0);
+      } else {
+        id = it->second;
+      }
+      F[(size_t)(3 * f + c)] = id;
+    }
+  return (int64_t)(V.size() / 3);
 }
 
 }  // namespace wn

@@ -33,6 +33,7 @@ _SIGNATURES = {
     "wn_device_free_bytes": ([], I),
     "wn_last_error": ([], S),
     "wn_version": ([], I),
+    "wn_kernel_timer": ([I] * 2, I),
     "wn_winding_gpu_ex": ([I] * 9 + [D], I),
     "wn_winding_gpu_vf": ([I] * 9, I),
     "wn_resolve_config": ([I] * 6 + [D, I], I),

@@ -295,6 +295,44 @@ def test_large_calls_use_strip_kernels(require_gpu, monkeypatch):
     assert np.array_equal(igl.winding_number(V, F32, Q, device="gpu"), w64)
 
 
+def test_reference_entry_large_calls_use_fp64_strips(require_gpu):
+    """mli_winding_gpu (the reference's ABI entry, face SoA only) with >=
+    65,536 queries finds the faces' shared corners by coordinate equality and
+    runs the FP64 strip kernel: within 1e-12 of the plain FP64 face kernel
+    (wn_winding_gpu_ex, precision 0) and within the FP64 bar of the oracle; a
+    repeated call (kept strip mesh) gives the same bits; a non-manifold soup
+    (repeated and reversed faces, coordinate-duplicated vertices) too."""
+    from igl import pack_faces
+    from igl._lib import addr, lib
+
+    h = lib()
+    V, F, Q = W.config2(1 << 20)
+    Q = np.ascontiguousarray(Q[:70_000])
+    soup_V, soup_F = W.icosphere(2)
+    soup_V = np.vstack([soup_V, soup_V[:40]])  # same coordinates, other ids
+    soup_F = np.vstack([soup_F, soup_F[:30, ::-1], soup_F[5:9]])
+    soup_F[::7, 0] = np.where(soup_F[::7, 0] < 40, soup_F[::7, 0] + len(soup_V) - 40,
+                              soup_F[::7, 0])
+    soup_Q = np.ascontiguousarray(np.random.default_rng(9).uniform(-1.5, 1.5, size=(65_536, 3)))
+    for name, (tri, Qc) in {"torus": (pack_faces(V, F), Q),
+                            "soup": (pack_faces(soup_V, soup_F), soup_Q)}.items():
+        face = np.full(len(Qc), np.nan)
+        assert h.wn_winding_gpu_ex(addr(tri), addr(Qc), addr(face), tri.shape[1], len(Qc), 0, 0,
+                                   0, 0, -1.0) == 0, igl._lib.last_error()
+        out = np.full(len(Qc), np.nan)
+        assert h.mli_winding_gpu(addr(tri), addr(Qc), addr(out), tri.shape[1], len(Qc)) == 1
+        again = np.full(len(Qc), np.nan)
+        assert h.mli_winding_gpu(addr(tri), addr(Qc), addr(again), tri.shape[1], len(Qc)) == 1
+        sub = np.arange(0, len(Qc), 331)
+        ref = oracle.winding_soa(tri, Qc[sub])
+        print(f"mli_winding_gpu strip route, {name}: vs face kernel "
+              f"{np.abs(out - face).max():.3e}, vs oracle {np.abs(out[sub] - ref).max():.3e}",
+              flush=True)
+        assert np.array_equal(out, again)
+        assert np.abs(out - face).max() <= 1e-12
+        assert np.abs(out[sub] - ref).max() <= FP64_TOL
+
+
 def test_vf_entry_matches_host_packing(require_gpu, c1, c2):
     """wn_winding_gpu_vf (V and F uploaded, face SoA gathered on the device;
     the path of igl.winding_number below the strip threshold) equals the
