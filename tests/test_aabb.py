@@ -16,6 +16,8 @@ igl.AABB) equals the oracle bit for bit, on the reference tests' inputs
 (tests/test_parity.py:131-177, 425-445) and contracts (:270-290, 340-350).
 """
 
+import os
+
 import numpy as np
 import pytest
 
@@ -320,3 +322,33 @@ def test_convenience_call_keeps_its_tree():
     assert same(igl.point_mesh_squared_distance(P, V, F2), fresh(V, F2))
     AB.release()
     assert AB._kept.tree is None
+
+
+def _tree_golden():
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "aabb_tree_golden.npz"))
+    names = sorted({k.split("_")[0] for k in g.files})
+    return g, names
+
+
+@pytest.mark.parametrize("leaf", [8, 3])
+def test_tree_layout_matches_reference_builder(leaf):
+    """The reference's own tree (its NumPy _build_aabb,
+    python/igl/__init__.py:286-319, run once by
+    tests/golden/make_aabb_tree_golden.py on the meshes of its AABB tests):
+    boxes, preorder node links, face order (stable ties) and stack size equal
+    bit for bit to the oracle's restatement and to the library builder
+    (wn_aabb_build), which mli_aabb_squared_distance's callers receive."""
+    from igl.aabb import _build_tree
+
+    g, names = _tree_golden()
+    assert {"grid8", "grid9", "grid10", "grid60", "flat12", "soup"} <= set(names)
+    for name in names:
+        V, F = g[f"{name}_V"], g[f"{name}_F"]
+        key = f"{name}_leaf{leaf}"
+        expect = (g[f"{key}_boxes"], g[f"{key}_nodes"], g[f"{key}_order"], int(g[f"{key}_stack"]))
+        for got in (oracle.build_aabb(V, F, leaf), _build_tree(V, F, leaf)):
+            assert np.array_equal(got[0], expect[0]), name
+            assert np.array_equal(got[1], expect[1]), name
+            assert np.array_equal(got[2], expect[2]), name
+            assert got[3] == expect[3], name
