@@ -346,6 +346,26 @@ def run_torus(args, h):
                 again.append((time.perf_counter() - t0) * 1e3)
             return min(first), min(again), w
 
+        # the reference's own ABI entry (face SoA from the host, FP64): from
+        # 65,536 queries it matches corners, stripifies and runs the FP64
+        # strip kernel, keeping the strip mesh for a repeated call
+        from igl._lib import addr
+
+        w_abi = np.empty(nq)
+        abi_ms = []
+        for _ in range(2):
+            igl.lib().wn_release_host_buffers()
+            for _ in range(2):
+                t0 = time.perf_counter()
+                assert h.mli_winding_gpu(addr(tri), addr(Q), addr(w_abi), nf, nq) == 1
+                abi_ms.append((time.perf_counter() - t0) * 1e3)
+        result_fp64["reference_abi_mli_winding_gpu"] = {
+            "first_call_ms": min(abi_ms[0::2]), "repeated_call_ms": min(abi_ms[1::2]),
+            "repeated_value": nf * nq / (min(abi_ms[1::2]) * 1e-3) / 1e9,
+            "note": "host arrays in and out (PCIe included), G pairs/s end to end"}
+        acc["max_abs_dw_reference_abi_vs_fp64"] = float(np.abs(w_abi - w64).max())
+        log(f"mli_winding_gpu 1M queries: first {min(abi_ms[0::2]):.2f} ms, repeated "
+            f"{min(abi_ms[1::2]):.2f} ms")
         first64, e2e64, w_e2e = host_calls("fp64")
         result_fp64["host_call_end_to_end_ms"] = e2e64
         result_fp64["host_call_first_ms"] = first64

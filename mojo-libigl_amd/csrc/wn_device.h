@@ -71,11 +71,14 @@ __device__ __forceinline__ double pair_f64(double ax, double ay, double az,
 // ---------------------------------------------------------------------------
 // FP32-compensated face record, 64 bytes, staged in LDS per block.
 // Coordinates are relative to the block's local origin (its query bbox
-// centre), rounded once to FP32; N2 = 2 (B'-A') x (C'-A') and the squared edge
-// lengths are computed in FP64 from those same rounded corners, so every
-// quantity the pair uses describes one consistent (rounded) triangle.
+// centre), rounded once to FP32; N8 = 8 (B'-A') x (C'-A') and the halved
+// squared edge lengths are computed in FP64 from those same rounded corners,
+// so every quantity the pair uses describes one consistent (rounded)
+// triangle. The scalings are powers of two, exact: the lane's dot product
+// with N8 is 8 det, and a law-of-cosines dot product is one add and one fma
+// with 0.5.
 //   r0 = (A.x, A.y, A.z, B.x)   r1 = (B.y, B.z, C.x, C.y)
-//   r2 = (C.z, N2.x, N2.y, N2.z) r3 = (|B-A|^2, |C-B|^2, |A-C|^2, 0)
+//   r2 = (C.z, N8.x, N8.y, N8.z) r3 = (-|B-A|^2/2, -|C-B|^2/2, -|A-C|^2/2, 0)
 // ---------------------------------------------------------------------------
 struct FaceRec32 {
   float4 r0, r1, r2, r3;
@@ -95,16 +98,16 @@ __device__ __forceinline__ FaceRec32 make_face_rec32(double Ax, double Ay, doubl
                e2z = (double)cz - (double)az;
   const double e3x = (double)cx - (double)bx, e3y = (double)cy - (double)by,
                e3z = (double)cz - (double)bz;
-  const double nx = 2.0 * (e1y * e2z - e1z * e2y);
-  const double ny = 2.0 * (e1z * e2x - e1x * e2z);
-  const double nz = 2.0 * (e1x * e2y - e1y * e2x);
+  const double nx = 8.0 * (e1y * e2z - e1z * e2y);
+  const double ny = 8.0 * (e1z * e2x - e1x * e2z);
+  const double nz = 8.0 * (e1x * e2y - e1y * e2x);
   FaceRec32 r;
   r.r0 = make_float4(ax, ay, az, bx);
   r.r1 = make_float4(by, bz, cx, cy);
   r.r2 = make_float4(cz, (float)nx, (float)ny, (float)nz);
-  r.r3 = make_float4((float)(e1x * e1x + e1y * e1y + e1z * e1z),
-                     (float)(e3x * e3x + e3y * e3y + e3z * e3z),
-                     (float)(e2x * e2x + e2y * e2y + e2z * e2z), 0.0f);
+  r.r3 = make_float4((float)(-0.5 * (e1x * e1x + e1y * e1y + e1z * e1z)),
+                     (float)(-0.5 * (e3x * e3x + e3y * e3y + e3z * e3z)),
+                     (float)(-0.5 * (e2x * e2x + e2y * e2y + e2z * e2z)), 0.0f);
   return r;
 }
 
@@ -140,20 +143,16 @@ __device__ __forceinline__ float atan_small_x8_f32(float x) {
   return x * p;
 }
 
-// Geometry of one pair in FP32 (see pair_rec32): doubled det and den and the
-// doubled scale 2 la lb lc.
+// Geometry of one pair in FP32 from a record. Algebra (exact identities,
+// cheaper and better conditioned than the direct form):
+//   det = N . a              with N = (B-A) x (C-A)  (det(a,b,c) = det(a,b-a,c-a))
+//   a.b = (la^2 + lb^2)/2 - |B-A|^2/2   (law of cosines), likewise b.c, c.a
+//   den = la lb lc + dp,  dp = lc a.b + la b.c + lb c.a
+// det8 = 8 det (the record's normal is 8 N); pql = la lb lc, the scale of the
+// conditioning test. A distant face has den -> 4 la lb lc.
 struct PairGeom32 {
-  float det2, den2, s2;
+  float det8, den, dp, pql;
 };
-
-// A pair is "far" when its angle is small and well conditioned:
-// |det| < kFarTau * den (so den > 0 and |atan argument| < kFarTau) and
-// den > kFarKappa * (2 la lb lc) (away from every edge line; a distant face
-// has den -> 4 la lb lc). A wave whose 64 pairs are all far evaluates
-// atan_small3_f32(det / den) instead of the octant-reduced atan2 and skips
-// the conditioning test.
-constexpr float kFarTau = 0.125f;
-constexpr float kFarKappa = 1.0f;
 
 __device__ __forceinline__ PairGeom32 pair_geom32(const float4 r0, const float4 r1,
                                                   const float4 r2, const float4 r3,
@@ -168,51 +167,15 @@ __device__ __forceinline__ PairGeom32 pair_geom32(const float4 r0, const float4 
   const float lb = __builtin_amdgcn_sqrtf(lb2);
   const float lc = __builtin_amdgcn_sqrtf(lc2);
   PairGeom32 g;
-  g.det2 = __builtin_fmaf(r2.w, az, __builtin_fmaf(r2.z, ay, r2.y * ax));
-  const float sab = (la2 + lb2) - r3.x;
-  const float sbc = (lb2 + lc2) - r3.y;
-  const float sca = (lc2 + la2) - r3.z;
-  const float u2 = 2.0f * (la * lb);
-  g.s2 = u2 * lc;
-  g.den2 = __builtin_fmaf(lc, u2 + sab, __builtin_fmaf(la, sbc, lb * sca));
+  g.det8 = __builtin_fmaf(r2.w, az, __builtin_fmaf(r2.z, ay, r2.y * ax));
+  const float ab = __builtin_fmaf(la2 + lb2, 0.5f, r3.x);
+  const float bc = __builtin_fmaf(lb2 + lc2, 0.5f, r3.y);
+  const float ca = __builtin_fmaf(lc2 + la2, 0.5f, r3.z);
+  const float pq = la * lb;
+  g.dp = __builtin_fmaf(lc, ab, __builtin_fmaf(la, bc, lb * ca));
+  g.den = __builtin_fmaf(pq, lc, g.dp);
+  g.pql = pq * lc;
   return g;
-}
-
-// FP32 pair from a record. Algebra (exact identities, cheaper and better
-// conditioned than the direct form):
-//   det = N . a              with N = (B-A) x (C-A)  (det(a,b,c) = det(a,b-a,c-a))
-//   2 a.b = la^2 + lb^2 - |B-A|^2   (law of cosines), likewise b.c, c.a
-// so with det2 = 2 det and den2 = 2 den (atan2 is invariant under the common
-// positive factor):
-//   den2 = lc (2 la lb + s_ab) + la s_bc + lb s_ca
-// `risky` is set when max(|det2|, |den2|) < kappa * 2 la lb lc: the pair's
-// atan2 is ill-conditioned there (query near an edge line, or close to the
-// face inside its outline), and the caller re-evaluates it in FP64.
-// For a far face max(|det2|,|den2|) -> 4 * 2 la lb lc.
-__device__ __forceinline__ float pair_rec32(const float4 r0, const float4 r1,
-                                            const float4 r2, const float4 r3,
-                                            float ox, float oy, float oz,
-                                            float kappa, bool& risky) {
-  const float ax = r0.x - ox, ay = r0.y - oy, az = r0.z - oz;
-  const float bx = r0.w - ox, by = r1.x - oy, bz = r1.y - oz;
-  const float cx = r1.z - ox, cy = r1.w - oy, cz = r2.x - oz;
-  const float la2 = __builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax));
-  const float lb2 = __builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx));
-  const float lc2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx));
-  const float la = __builtin_amdgcn_sqrtf(la2);
-  const float lb = __builtin_amdgcn_sqrtf(lb2);
-  const float lc = __builtin_amdgcn_sqrtf(lc2);
-  const float det2 = __builtin_fmaf(r2.w, az, __builtin_fmaf(r2.z, ay, r2.y * ax));
-  const float sab = (la2 + lb2) - r3.x;
-  const float sbc = (lb2 + lc2) - r3.y;
-  const float sca = (lc2 + la2) - r3.z;
-  const float u2 = 2.0f * (la * lb);
-  const float s2 = u2 * lc;
-  const float den2 = __builtin_fmaf(lc, u2 + sab, __builtin_fmaf(la, sbc, lb * sca));
-  float mx;
-  const float ang = atan2_f32(det2, den2, mx);
-  risky = mx < kappa * s2;
-  return ang;
 }
 
 // True on every lane when `pred` holds on every active lane of the wave
@@ -221,17 +184,8 @@ __device__ __forceinline__ bool wave_all(bool pred) {
   return __builtin_amdgcn_ballot_w64(pred) == __builtin_amdgcn_read_exec();
 }
 
-// Far-pair test of every active lane: |det2| < kFarTau * den2 and
-// den2 > kFarKappa * s2 (see pair_geom32). Each compare is balloted on its
-// own so both masks stay in SGPRs (no VALU round trip through a bool).
-__device__ __forceinline__ bool wave_all_far(float det2, float den2, float s2) {
-  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(det2) < kFarTau * den2);
-  const uint64_t m2 = __builtin_amdgcn_ballot_w64(den2 > kFarKappa * s2);
-  return (m1 & m2) == __builtin_amdgcn_read_exec();
-}
-
-// Far test of the strip kernel, for det8 = 8 det, den and dp = den - la lb
-// lc: every active lane has |det8| < den (|det / den| < 1/8, the range of
+// Far test of the strip and face-record kernels, for det8 = 8 det, den and
+// dp = den - la lb lc: every active lane has |det8| < den (|det / den| < 1/8, the range of
 // atan_small_x8_f32) and dp > 0 (den > la lb lc, away from every edge line).
 // Each compare is balloted on its own so both masks stay in SGPRs.
 __device__ __forceinline__ bool wave_all_far_x8(float det8, float den, float dp) {

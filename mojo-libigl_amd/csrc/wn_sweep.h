@@ -91,6 +91,36 @@ __device__ __forceinline__ void block_origin(const double (&Ox)[QPL], const doub
   gz = 0.5 * (lo2 + hi2);
 }
 
+// One face of a tile for QPL queries: its FP32 angle is added to part[k],
+// the FP32 sum of at most two consecutive faces' angles that the caller adds
+// to the FP64 sum. Far pairs on every lane of the wave (|det / den| < 1/8 and
+// den > la lb lc): the small-angle arctangent; otherwise the octant-reduced
+// atan2, and ill-conditioned pairs (max(|det|, |den|) < kappa la lb lc: a
+// query near an edge line, or close to the face inside its outline) are
+// re-evaluated in FP64 from the same record and rounded to FP32 once.
+template <int QPL>
+__device__ __forceinline__ void face_step_f32c(const FaceRec32& rec, const float (&ox)[QPL],
+                                               const float (&oy)[QPL], const float (&oz)[QPL],
+                                               float kappa, float (&part)[QPL]) {
+#pragma clang fp contract(off)  // part += angle rounds the angle first
+  const float4 r0 = rec.r0, r1 = rec.r1, r2 = rec.r2, r3 = rec.r3;
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const PairGeom32 g = pair_geom32(r0, r1, r2, r3, ox[k], oy[k], oz[k]);
+    float ang = atan_small_x8_f32(g.det8 * __builtin_amdgcn_rcpf(g.den));
+    if (__builtin_expect(!wave_all_far_x8(g.det8, g.den, g.dp), 0)) {
+      // An empty volatile asm gives this path a side effect, so the
+      // compiler keeps a real (wave-uniform) branch instead of computing
+      // both paths and selecting.
+      asm volatile("" ::);
+      float mx;
+      ang = atan2_f32(0.125f * g.det8, g.den, mx);
+      if (mx < kappa * g.pql) ang = (float)pair_rec32_f64(r0, r1, r2, ox[k], oy[k], oz[k]);
+    }
+    part[k] += ang;
+  }
+}
+
 template <int QPL>
 __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64_t stride,
                                            int64_t f0, int64_t f1, double gx, double gy,
@@ -109,30 +139,26 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
                                      tri[7 * stride + f], tri[8 * stride + f], gx, gy, gz);
     }
     __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
-      const float4 r0 = sm.face[j].r0, r1 = sm.face[j].r1, r2 = sm.face[j].r2,
-                   r3 = sm.face[j].r3;
+    // the angles of two consecutive faces are summed in FP32 (each |angle|
+    // <= pi; one rounding, at most eps times their magnitudes, the bound of
+    // a compensated sum) and added to the FP64 sum once
+    int j = 0;
+    for (; j + 2 <= cnt; j += 2) {
+      float part[QPL];
 #pragma unroll
-      for (int k = 0; k < QPL; ++k) {
-        const PairGeom32 g = pair_geom32(r0, r1, r2, r3, ox[k], oy[k], oz[k]);
-        float ang;
-        bool risky = false;
-        if (wave_all_far(g.det2, g.den2, g.s2)) {
-          // wave-uniform: every lane's angle is small and well conditioned
-          ang = atan_small3_f32(g.det2 * __builtin_amdgcn_rcpf(g.den2));
-        } else {
-          // An empty volatile asm gives this path a side effect, so the
-          // compiler keeps a real (wave-uniform) branch instead of computing
-          // both paths and selecting.
-          asm volatile("" ::);
-          float mx;
-          ang = atan2_f32(g.det2, g.den2, mx);
-          risky = mx < kappa * g.s2;
-        }
-        double a64 = (double)ang;
-        if (risky) a64 = pair_rec32_f64(r0, r1, r2, ox[k], oy[k], oz[k]);
-        acc[k] += a64;
-      }
+      for (int k = 0; k < QPL; ++k) part[k] = -0.0f;  // -0 + x == x: the first add folds away
+      face_step_f32c<QPL>(sm.face[j], ox, oy, oz, kappa, part);
+      face_step_f32c<QPL>(sm.face[j + 1], ox, oy, oz, kappa, part);
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) acc[k] += (double)part[k];
+    }
+    if (j < cnt) {
+      float part[QPL];
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) part[k] = -0.0f;
+      for (; j < cnt; ++j) face_step_f32c<QPL>(sm.face[j], ox, oy, oz, kappa, part);
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) acc[k] += (double)part[k];
     }
   }
   __syncthreads();
