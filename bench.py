@@ -6,8 +6,10 @@ the FP32-compensated strip kernel (wn_mesh_forward on a resident igl.Mesh) over
 the closed torus R = 1, r = 0.35 on a 120 x 118 grid (28,320 faces) x
 1,048,576 queries uniform in the 20 %-padded bbox, inputs resident in HBM. The
 same steps are timed for the FP32-compensated face-record kernel (wn_forward,
-precision 1, no mesh preprocessing) and for the FP64-plain kernel, the
-comparison point of the metric; outputs are compared with each other and, on a
+precision 1, no mesh preprocessing), for the FP64-plain kernel (precision 2:
+the reference formula with the library's sqrt and atan2 for every pair), the
+comparison point of the metric, and for the FP64 face and strip kernels with
+the far-pair fast path; outputs are compared with each other and, on a
 subsample, with the oracle.
 
 Prints ONE JSON line. `value` = G query-face pairs/s of the FP32-compensated
@@ -30,6 +32,7 @@ Other workloads (one JSON line each, same timing method):
                      1M queries, per accuracy parameter, vs all-pairs
   --workload copy    256 MiB device copy (calibrates the byte counters)
   --workload plain64 the plain FP64 kernel alone on the torus workload
+  --workload face64  the FP64 face kernel (far-pair fast path) alone on it
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
 """
@@ -305,21 +308,34 @@ def run_torus(args, h):
 
     result_fp64, acc = None, {}
     if not args.no_fp64:
-        f64 = Forward(h, trid, Qd, 0)
+        f64 = Forward(h, trid, Qd, 2)
         ms64, _, k64, _ = timed_kernel(h, f64, args.steps, args.warmup)
         log(f"fp64 plain {ms64:.3f} ms/step (device), fwd_f64_kernel {k64:.3f} ms/launch")
         w64 = f64.out.cpu().numpy()
+        f64f = Forward(h, trid, Qd, 0)
+        ms64f, _, k64f, _ = timed_kernel(h, f64f, args.steps, args.warmup)
+        log(f"fp64 face kernel {ms64f:.3f} ms/step (device), fwd_f64f_kernel {k64f:.3f} ms/launch")
+        w64f = f64f.out.cpu().numpy()
         d = np.abs(w_strip - w64)
         acc = {"max_abs_dw_fp32c_vs_fp64": float(d.max()), "mean_abs_dw_fp32c_vs_fp64": float(d.mean()),
                "max_abs_dw_face_kernel_vs_fp64": float(np.abs(w_face - w64).max()),
                "fp64_max_dist_from_integer": float(np.abs(w64 - np.round(w64)).max())}
         result_fp64 = {"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64, "qpl": f64.qpl,
-                       "splits": f64.splits, "kernel_ms": k64,
+                       "splits": f64.splits, "kernel_ms": k64, "precision": 2,
                        "roofline_frac_fp64_peak": roofline(nf * nq, k64, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
+        # the FP64 face kernel of the product (precision 0): far pairs by the
+        # arctangent polynomial, the rest by the plain formula
+        result_fp64["face"] = {"value": nf * nq / (ms64f * 1e-3) / 1e9, "ms_per_step": ms64f,
+                               "kernel_ms": k64f, "qpl": f64f.qpl, "splits": f64f.splits,
+                               "speedup_vs_fp64_plain": ms64 / ms64f,
+                               "roofline_frac_fp64_peak":
+                                   roofline(nf * nq, k64f, FP64_VECTOR_PEAK_TFLOPS)["frac"]}
+        acc["max_abs_dw_fp64_face_vs_fp64"] = float(np.abs(w64f - w64).max())
         sub = np.arange(0, nq, max(1, nq // 512))
         ref = oracle.winding_soa(tri, Q[sub])
         acc["max_abs_dw_fp64_vs_oracle_sub512"] = float(np.abs(w64[sub] - ref).max())
         acc["max_abs_dw_fp32c_vs_oracle_sub512"] = float(np.abs(w_strip[sub] - ref).max())
+        acc["max_abs_dw_fp64_face_vs_oracle_sub512"] = float(np.abs(w64f[sub] - ref).max())
         s64 = MeshForward(h, mesh, Qd, args.splits, fp64=True)
         ms_s64, _, k_s64, _ = timed_kernel(h, s64, args.steps, args.warmup)
         log(f"fp64 strip {ms_s64:.3f} ms/step (device), {k_s64:.3f} ms/launch")
@@ -347,7 +363,7 @@ def run_torus(args, h):
             return min(first), min(again), w
 
         # the reference's own ABI entry (face SoA from the host, FP64): from
-        # 65,536 queries it matches corners, stripifies and runs the FP64
+        # 262,144 queries it matches corners, stripifies and runs the FP64
         # strip kernel, keeping the strip mesh for a repeated call
         from igl._lib import addr
 
@@ -399,8 +415,11 @@ def run_torus(args, h):
         speedup_fp32c_vs_fp64=(result_fp64["ms_per_step"] / ms_strip) if result_fp64 else None,
         # the metric's ratio is FP32c strips over the plain FP64 kernel; the
         # same-algorithm ratios put each FP32c kernel beside its FP64 twin
-        speedup_same_algorithm=({"face_vs_face": result_fp64["ms_per_step"] / ms_face,
-                                 "strip_vs_strip": result_fp64["strip"]["ms_per_step"] / ms_strip}
+        # (both with the far-pair fast path), and the face kernel beside the
+        # plain one
+        speedup_same_algorithm=({"face_vs_face": result_fp64["face"]["ms_per_step"] / ms_face,
+                                 "strip_vs_strip": result_fp64["strip"]["ms_per_step"] / ms_strip,
+                                 "face_vs_plain_face": result_fp64["ms_per_step"] / ms_face}
                                 if result_fp64 else None),
         fp32c_face_kernel={"value": nf * nq / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face,
                            "kernel_ms": k_face, "qpl": face.qpl, "splits": face.splits},
@@ -537,9 +556,11 @@ def run_batch(args, h):
                   "max_abs_dw_batched_vs_separate": float(np.abs(w_face - w_sep).max())})
 
 
-def run_plain64(args, h):
-    """The plain FP64 face kernel alone on the torus workload (wn_forward,
-    precision 0): the comparison point of the metric, for profiling."""
+def run_fp64_face(args, h, precision):
+    """One FP64 face kernel alone on the torus workload (wn_forward): the
+    plain kernel (precision 2), the comparison point of the metric, or the
+    FP64 face kernel with the far-pair fast path (precision 0); for
+    profiling."""
     import torch
 
     import igl
@@ -548,16 +569,28 @@ def run_plain64(args, h):
     V, F, Q = W.config2(args.queries)
     tri = igl.pack_faces(V, F)
     nf, nq = len(F), len(Q)
-    f64 = Forward(h, torch.tensor(tri, device="cuda"), torch.tensor(Q, device="cuda"), 0)
+    f64 = Forward(h, torch.tensor(tri, device="cuda"), torch.tensor(Q, device="cuda"), precision)
     ms, _, k, n = timed_kernel(h, f64, args.steps, args.warmup)
-    log(f"fp64 plain {ms:.3f} ms/step, fwd_f64_kernel {k:.3f} ms/launch")
+    plain = precision == 2
+    kernel = "wn::fwd_f64_kernel" if plain else "wn::fwd_f64f_kernel"
+    log(f"{'fp64 plain' if plain else 'fp64 face'} {ms:.3f} ms/step, {kernel} {k:.3f} ms/launch")
+    what = ("plain FP64 face kernel" if plain else
+            "FP64 face kernel, far pairs by the arctangent polynomial")
     return base_line(
-        args, "G query-face pairs/s, FP64 plain kernel, 1 MI355X", nf * nq / (ms * 1e-3) / 1e9,
-        "G query-face pairs/s", ms, "f64",
-        f"closed torus 120x118 ({nf} faces) x {nq} queries; plain FP64 face kernel (wn_forward "
-        f"precision 0, qpl {f64.qpl}, {f64.splits} splits)",
-        roofline=roofline(nf * nq, k, FP64_VECTOR_PEAK_TFLOPS, kernel="wn::fwd_f64_kernel<1>",
+        args, f"G query-face pairs/s, {'FP64 plain' if plain else 'FP64 face'} kernel, 1 MI355X",
+        nf * nq / (ms * 1e-3) / 1e9, "G query-face pairs/s", ms, "f64",
+        f"closed torus 120x118 ({nf} faces) x {nq} queries; {what} (wn_forward precision "
+        f"{precision}, qpl {f64.qpl}, {f64.splits} splits)",
+        roofline=roofline(nf * nq, k, FP64_VECTOR_PEAK_TFLOPS, kernel=f"{kernel}<{f64.qpl}>",
                           launches=n, torus_traffic=False))
+
+
+def run_plain64(args, h):
+    return run_fp64_face(args, h, 2)
+
+
+def run_face64(args, h):
+    return run_fp64_face(args, h, 0)
 
 
 def run_stress(args, h):
@@ -584,9 +617,12 @@ def run_stress(args, h):
     face = Forward(h, trid, Qd, 1)
     ms_face, _ = timed(face, max(1, args.steps // 2), min(args.warmup, 1))
     w_face = face.out.cpu().numpy()
-    f64 = Forward(h, trid, Qd, 0)
+    f64 = Forward(h, trid, Qd, 2)
     ms64, _ = timed(f64, max(1, args.steps // 2), min(args.warmup, 1))
     w64 = f64.out.cpu().numpy()
+    f64f = Forward(h, trid, Qd, 0)
+    ms64f, _ = timed(f64f, max(1, args.steps // 2), min(args.warmup, 1))
+    w64f = f64f.out.cpu().numpy()
     mesh.close()
     uni = np.isnan(dist)
     mag = np.abs(dist)
@@ -610,7 +646,9 @@ def run_stress(args, h):
         "|d| in [1e-7, 1e-2] along the normal); FP32-compensated strip kernel",
         roofline=roofline(nf * nq, k32, kernel="wn::fwd_strip_kernel", launches=n32,
                           torus_traffic=False),
-        fp64_plain={"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64},
+        fp64_plain={"value": nf * nq / (ms64 * 1e-3) / 1e9, "ms_per_step": ms64,
+                    "face": {"value": nf * nq / (ms64f * 1e-3) / 1e9, "ms_per_step": ms64f,
+                             "max_abs_dw_vs_fp64_plain": float(np.abs(w64f - w64).max())}},
         fp32c_face_kernel={"value": nf * nq / (ms_face * 1e-3) / 1e9, "ms_per_step": ms_face},
         speedup_fp32c_vs_fp64=ms64 / ms32, mesh_build_ms=create_ms,
         accuracy={"max_abs_dw_fp32c_vs_fp64": float(np.abs(w32 - w64).max()),
@@ -893,7 +931,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="torus",
                     choices=("torus", "batch", "stress", "grad", "readme", "aabb", "fast", "copy",
-                             "plain64"))
+                             "plain64", "face64"))
     ap.add_argument("--queries", type=int, default=1_048_576)
     ap.add_argument("--qpl", type=int, default=0, help="queries per lane (0 = library default)")
     ap.add_argument("--splits", type=int, default=0, help="face splits (0 = library default)")
@@ -916,7 +954,7 @@ def main() -> int:
         return 1
     runner = {"torus": run_torus, "batch": run_batch, "stress": run_stress, "grad": run_grad,
               "aabb": run_aabb, "fast": run_fast, "readme": run_readme,
-              "copy": run_copy, "plain64": run_plain64}[args.workload]
+              "copy": run_copy, "plain64": run_plain64, "face64": run_face64}[args.workload]
     print(json.dumps(runner(args, h)), flush=True)
     return 0
 

@@ -14,9 +14,17 @@
  * (reference python/igl/__init__.py:248-250). Queries: nq x 3 float64,
  * row-major. Output: nq float64 winding numbers.
  *
- * precision: 0 = FP64 plain (the reference formula in FP64 for every pair);
+ * precision: 0 = FP64: a wave whose pairs are all far (|det| < den / 8,
+ *                den > |a||b||c|) takes atan(det / den) from a polynomial,
+ *                with square roots and the quotient refined to FP64
+ *                rounding; other pairs take the reference formula (within
+ *                ~1e-15 of precision 2, 2x its speed);
  *            1 = FP32-compensated (FP32 pairs, FP64 accumulation, FP64
- *                re-evaluation of ill-conditioned pairs).
+ *                re-evaluation of ill-conditioned pairs);
+ *            2 = FP64 plain: the reference formula in FP64 for every pair,
+ *                with the library's sqrt and atan2 (face-SoA entry points
+ *                only: wn_winding_gpu_ex, wn_forward, wn_resolve_config,
+ *                wn_forward_workspace_bytes).
  * Tuning arguments (0 or -1 = library default): qpl = queries per lane
  * (1, 2, 4), splits = face splits (split-K, 1..65535), sort = Morton-order
  * the queries (FP32-compensated only; -1 auto), kappa = conditioning
@@ -40,11 +48,11 @@ void mli_winding(const double *tri, const double *q, double *dst, int64_t nf,
                  int64_t nq, int64_t q0, int64_t q1);
 
 /* Replaces mli_winding_gpu, src/capi.mojo:945-947 (python/igl/_lib.py:23,
- * 5 x c_int64 -> c_int64). GPU, FP64. (host) tri, q, dst. Below 65,536
- * queries the plain FP64 face kernel; from there the FP64 strip kernel (the
- * faces' shared corners found by coordinate equality, the strip mesh kept
- * for a repeated call as wn_winding_gpu_mesh keeps it), which agrees with
- * the plain kernel to ~1e-15. Returns 1 on success, 0 on any failure (reason
+ * 5 x c_int64 -> c_int64). GPU, FP64. (host) tri, q, dst. Below 262,144
+ * queries the FP64 face kernel (precision 0); from there the FP64 strip
+ * kernel (the faces' shared corners found by coordinate equality, the strip
+ * mesh kept for a repeated call as wn_winding_gpu_mesh keeps it). Both agree
+ * with the plain formula (precision 2) to ~1e-15. Returns 1 on success, 0 on any failure (reason
  * in wn_last_error()), as the reference's try_winding_gpu does
  * (src/capi.mojo:366-398). Thread-safe. */
 int64_t mli_winding_gpu(const double *tri, const double *q, double *dst,

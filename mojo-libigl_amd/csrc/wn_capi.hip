@@ -93,9 +93,9 @@ std::string hip_msg(const char* what, hipError_t e) {
 // Resident blocks per (precision, qpl) on this device, cached; 0 = unknown.
 int resident_blocks(int precision, int qpl) {
   static std::mutex mu;
-  static int cache[2][5] = {};
-  static bool asked[2][5] = {};
-  if (precision < 0 || precision > 1 || qpl < 1 || qpl > 4) return 0;
+  static int cache[3][5] = {};
+  static bool asked[3][5] = {};
+  if (precision < 0 || precision > 2 || qpl < 1 || qpl > 4) return 0;
   std::lock_guard<std::mutex> lock(mu);
   if (!asked[precision][qpl]) {
     asked[precision][qpl] = true;
@@ -128,7 +128,8 @@ wn::FwdConfig make_config(int64_t nf, int64_t nq, int64_t precision, int64_t qpl
   c.splits = (int)splits;
   c.sort = (int)sort;
   c.kappa = (float)kappa;
-  const bool known = precision == wn::kFP64 || precision == wn::kFP32C;
+  const bool known =
+      precision == wn::kFP64 || precision == wn::kFP32C || precision == wn::kFP64Plain;
   auto qblocks_of = [nq](int lanes) { return std::max<int64_t>(1, (nq + 256 * lanes - 1) / (256 * lanes)); };
   auto resident_of = [&](int lanes) {
     const int r = known ? resident_blocks((int)precision, lanes) : 0;
@@ -594,10 +595,10 @@ int64_t wn_device_free_bytes(void) {
 // mli_winding_gpu calls with at least this many queries take the FP64 strip
 // kernel (the corners' vertex identity recovered from the face SoA by
 // coordinate equality, then the host strip route of wn_winding_gpu_mesh,
-// which keeps the mesh for the next call): the strip kernel saves ~2.3 ps
-// per query-face pair over the plain FP64 kernel and the host side costs
-// ~0.3 us per face, the break-even of igl.winding_number's strip route.
-constexpr int64_t kTriStripMinQueries = 65536;
+// which keeps the mesh for the next call): the strip kernel saves ~0.93 ps
+// per query-face pair over the FP64 face kernel and the host side costs
+// ~0.3 us per face, so a first call breaks even at ~320k queries.
+constexpr int64_t kTriStripMinQueries = 262144;
 
 int64_t mli_winding_gpu(const double* tri, const double* q, double* dst, int64_t nf,
                         int64_t nq) {

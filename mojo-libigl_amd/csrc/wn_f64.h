@@ -1,0 +1,77 @@
+// FP64 building blocks of the far-pair fast path shared by the FP64 face
+// and strip kernels: square root and quotient refined from the hardware
+// estimates to FP64 rounding, the arctangent polynomial on |det / den| < 1/8,
+// and the wave-uniform far test.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wn {
+
+// sqrt(x) for x > 0 finite: one Goldschmidt step from v_rsq_f64 (relative
+// error d of the estimate -> 1.5 d^2), then the root corrected by its exact
+// remainder x - g^2 times the unrefined half reciprocal (-> 1.5 d^3 before
+// the final rounding). x == 0 gives NaN (0 * inf), which fails the far test
+// and sends the wave to pair_f64.
+__device__ __forceinline__ double sqrt_gs_f64(double x) {
+  const double r = __builtin_amdgcn_rsq(x);
+  const double g = x * r, h = 0.5 * r;
+  const double e = __builtin_fma(-g, h, 0.5);
+  const double g1 = __builtin_fma(g, e, g);
+  return __builtin_fma(__builtin_fma(-g1, g1, x), h, g1);
+}
+
+// n / d for d > 0 finite: v_rcp_f64, one Newton step, then the quotient
+// corrected by its remainder.
+__device__ __forceinline__ double div_f64(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+  const double q = n * r;
+  return __builtin_fma(__builtin_fma(-d, q, n), r, q);
+}
+
+// atan(x / 8) for |x| <= 1: x * P(x^2) with the n = 7 relative-error
+// minimax coefficients of atan on (0, 1/8], each divided by 8^(2k+1)
+// (powers of two: exact). The Horner steps are one asm block of VOP3
+// v_fma_f64 with the coefficients in scalar registers: compiled from C, the
+// coefficients stay in VGPRs and each step copies one into a v_fmac_f64
+// destination (extra FP64 moves per pair), and one asm block per step costs
+// a wait state after each.
+__device__ __forceinline__ double atan_small_x8_f64(double x) {
+  const double t = x * x;
+  double p;
+  asm("v_fma_f64 %0, %2, %1, %3\n\t"
+      "v_fma_f64 %0, %0, %1, %4\n\t"
+      "v_fma_f64 %0, %0, %1, %5\n\t"
+      "v_fma_f64 %0, %0, %1, %6\n\t"
+      "v_fma_f64 %0, %0, %1, %7\n\t"
+      "v_fma_f64 %0, %0, %1, %8"
+      : "=&v"(p)
+      : "v"(t), "v"(0.073382651424641993073 / 549755813888.0),  // x^13: 8^13
+        "s"(-0.090833776841241786793 / 8589934592.0),             // x^11: 8^11
+        "s"(0.11111031276417376124 / 134217728.0),                // x^9: 8^9
+        "s"(-0.14285713850448952581 / 2097152.0),                 // x^7: 8^7
+        "s"(0.19999999998868866968 / 32768.0),                    // x^5: 8^5
+        "s"(-0.33333333333332230442 / 512.0),                     // x^3: 8^3
+        "s"(0.99999999999999999824 / 8.0));                       // x^1: 8
+  return x * p;
+}
+
+// h + e for h = (u + v) / 2, e in a scalar register: one VOP3 v_fma_f64
+// (compiled from C, the addend is copied to VGPRs for a v_fmac_f64).
+__device__ __forceinline__ double half_sum_plus(double u, double v, double e) {
+  double r;
+  asm("v_fma_f64 %0, %1, 0.5, %2" : "=v"(r) : "v"(u + v), "s"(e));
+  return r;
+}
+
+// |det8| < den is |det / den| < 1/8 (det8 = 8 det); dp > 0 is
+// den > la lb lc.
+__device__ __forceinline__ bool wave_all_far_x8_f64(double det8, double den, double dp) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(__builtin_fabs(det8) < den);
+  const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp > 0.0);
+  return (m1 & m2) == __builtin_amdgcn_read_exec();
+}
+
+}  // namespace wn

@@ -1,10 +1,14 @@
 // Forward winding-number kernels for gfx950.
 //
-// Both kernels are "one query per lane, faces broadcast from LDS": a block of
-// 256 threads owns 256*QPL queries, streams the face list through LDS tiles of
-// 256 faces (one face staged per thread, coalesced loads from the 9 x nf SoA),
-// and every lane sweeps each tile with all its queries. Accumulation stays in
-// registers in fixed face order, so results are bitwise reproducible.
+// All three kernels run one query per lane (QPL per lane with the template
+// argument) over a face range. The plain FP64 and the FP32-compensated
+// kernels broadcast faces from LDS: a block of 256 threads owns 256*QPL
+// queries, streams the face list through LDS tiles of 256 faces (one face
+// staged per thread, coalesced loads from the 9 x nf SoA), and every lane
+// sweeps each tile with all its queries. The FP64 kernel with the far-pair
+// fast path reads prepared 128-byte face records through scalar loads
+// instead. Accumulation stays in registers in fixed face order, so results
+// are bitwise reproducible.
 //
 // Split-K: grid.y splits the face range. Each (block, split) writes unscaled
 // partials to partial[s * nq + slot] (slot = the query's position in launch
@@ -19,6 +23,7 @@
 // is materialised here.
 
 #include "wn_device.h"
+#include "wn_f64.h"
 #include "wn_internal.h"
 #include "wn_sweep.h"
 
@@ -58,6 +63,122 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f64_kernel(
         out[qi] = acc[k] * kInv2Pi;
       else
         out[(int64_t)blockIdx.y * nq + qi] = acc[k];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// FP64 with the far-pair fast path: the FP64 strip kernel's per-pair
+// arithmetic on face records. Per pair: three offsets, three squared lengths
+// and three refined square roots, the law-of-cosines dot products with the
+// record's halved squared edge lengths, and det = N . a with the record's
+// normal (exact: N . (b - a) = N . (c - a) = 0). A wave whose pairs are all
+// far (|det| < den / 8 and den > la lb lc) takes atan(det / den) from the
+// 7-term polynomial; any other wave evaluates every lane's pair with the
+// plain formula (pair_f64 on the same offsets, the plain kernel's own
+// arithmetic). Records are read at wave-uniform addresses, so they arrive in
+// scalar registers through scalar loads: no LDS, no tile staging, no
+// barriers.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlockThreads) void face_rec64_kernel(
+    const double* __restrict__ tri, int64_t nf, FaceRec64* __restrict__ out) {
+  const int64_t f = (int64_t)blockIdx.x * kBlockThreads + threadIdx.x;
+  if (f >= nf) return;
+  FaceRec64 r;
+  r.ax = tri[f];
+  r.ay = tri[nf + f];
+  r.az = tri[2 * nf + f];
+  r.bx = tri[3 * nf + f];
+  r.by = tri[4 * nf + f];
+  r.bz = tri[5 * nf + f];
+  r.cx = tri[6 * nf + f];
+  r.cy = tri[7 * nf + f];
+  r.cz = tri[8 * nf + f];
+  const double e1x = r.bx - r.ax, e1y = r.by - r.ay, e1z = r.bz - r.az;  // B - A
+  const double e2x = r.cx - r.ax, e2y = r.cy - r.ay, e2z = r.cz - r.az;  // C - A
+  const double e3x = r.cx - r.bx, e3y = r.cy - r.by, e3z = r.cz - r.bz;  // C - B
+  r.nx = 8.0 * (e1y * e2z - e1z * e2y);
+  r.ny = 8.0 * (e1z * e2x - e1x * e2z);
+  r.nz = 8.0 * (e1x * e2y - e1y * e2x);
+  r.eab = -0.5 * (e1x * e1x + e1y * e1y + e1z * e1z);
+  r.ebc = -0.5 * (e3x * e3x + e3y * e3y + e3z * e3z);
+  r.eca = -0.5 * (e2x * e2x + e2y * e2y + e2z * e2z);
+  r.pad = 0.0;
+  out[f] = r;
+}
+
+// The plain FP64 pair, out of line so that its registers are not reserved
+// for the whole sweep.
+__device__ __noinline__ double face64_pair(double ax, double ay, double az, double bx,
+                                           double by, double bz, double cx, double cy,
+                                           double cz) {
+  return pair_f64(ax, ay, az, bx, by, bz, cx, cy, cz);
+}
+
+// One face for the lane's QPL queries: far pairs into acc, pairs of the
+// plain formula into acc_slow (two sums, each only ever updated in place).
+template <int QPL>
+__device__ __forceinline__ void face64_step(const FaceRec64& r, const double (&ox)[QPL],
+                                            const double (&oy)[QPL], const double (&oz)[QPL],
+                                            double (&acc)[QPL], double (&acc_slow)[QPL]) {
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const double ax = r.ax - ox[k], ay = r.ay - oy[k], az = r.az - oz[k];
+    const double bx = r.bx - ox[k], by = r.by - oy[k], bz = r.bz - oz[k];
+    const double cx = r.cx - ox[k], cy = r.cy - oy[k], cz = r.cz - oz[k];
+    const double a2 = __builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax));
+    const double b2 = __builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx));
+    const double c2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
+    const double la = sqrt_gs_f64(a2), lb = sqrt_gs_f64(b2), lc = sqrt_gs_f64(c2);
+    const double s_ab = half_sum_plus(a2, b2, r.eab);  // a.b
+    const double s_bc = half_sum_plus(b2, c2, r.ebc);  // b.c
+    const double s_ca = half_sum_plus(c2, a2, r.eca);  // c.a
+    const double det8 = __builtin_fma(r.nz, az, __builtin_fma(r.ny, ay, r.nx * ax));
+    const double dp = __builtin_fma(lc, s_ab, __builtin_fma(la, s_bc, lb * s_ca));
+    const double den = __builtin_fma(la * lb, lc, dp);
+    if (__builtin_expect(wave_all_far_x8_f64(det8, den, dp), 1)) {
+      acc[k] += atan_small_x8_f64(div_f64(det8, den));
+    } else {
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      acc_slow[k] += face64_pair(ax, ay, az, bx, by, bz, cx, cy, cz);
+    }
+  }
+}
+
+template <int QPL>
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(QPL == 1 ? 6 : 4))) void fwd_f64f_kernel(
+    const FaceRec64* __restrict__ rec, const double* __restrict__ q, double* __restrict__ out,
+    int64_t nf, int64_t nq, int64_t faces_per_split, int splits) {
+  const int tid = threadIdx.x;
+  const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
+  double ox[QPL], oy[QPL], oz[QPL], acc[QPL], acc_slow[QPL];
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    const bool ok = qi < nq;
+    ox[k] = ok ? q[3 * qi] : 0.0;
+    oy[k] = ok ? q[3 * qi + 1] : 0.0;
+    oz[k] = ok ? q[3 * qi + 2] : 0.0;
+    acc[k] = 0.0;
+    acc_slow[k] = 0.0;
+  }
+  const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
+  const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
+  // 32-bit inner counter: the loop test stays a scalar compare
+  for (int64_t c0 = f0; c0 < f1; c0 += (int64_t)1 << 30) {
+    const FaceRec64* r = rec + c0;
+    const int cnt = (int)(f1 - c0 < ((int64_t)1 << 30) ? f1 - c0 : ((int64_t)1 << 30));
+    for (int j = 0; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
+  }
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int64_t qi = qbase + k * kBlockThreads + tid;
+    if (qi < nq) {
+      const double a = acc[k] + acc_slow[k];
+      if (splits == 1)
+        out[qi] = a * kInv2Pi;
+      else
+        out[(int64_t)blockIdx.y * nq + qi] = a;
     }
   }
 }
@@ -225,8 +346,10 @@ hipError_t reduce_splits_launch(const double* partial, const int32_t* perm, doub
 
 int fwd_resident_blocks(int precision, int qpl) {
   const void* fn = nullptr;
-  if (precision == kFP64)
+  if (precision == kFP64Plain)
     fn = qpl == 1 ? (const void*)fwd_f64_kernel<1> : (const void*)fwd_f64_kernel<2>;
+  else if (precision == kFP64)
+    fn = qpl == 1 ? (const void*)fwd_f64f_kernel<1> : (const void*)fwd_f64f_kernel<2>;
   else
     fn = qpl == 1 ? (const void*)fwd_f32c_kernel<1>
                   : (qpl == 2 ? (const void*)fwd_f32c_kernel<2> : (const void*)fwd_f32c_kernel<4>);
@@ -248,9 +371,10 @@ static int64_t faces_per_split_of(int64_t nf, int splits) {
 
 const char* fwd_check(int64_t nf, int64_t nq, const FwdConfig& cfg) {
   if (nf < 0 || nq < 0) return "negative size";
-  if (cfg.precision != kFP64 && cfg.precision != kFP32C) return "unknown precision";
+  if (cfg.precision != kFP64 && cfg.precision != kFP32C && cfg.precision != kFP64Plain)
+    return "unknown precision";
   if (cfg.qpl != 1 && cfg.qpl != 2 && cfg.qpl != 4) return "queries_per_lane must be 1, 2 or 4";
-  if (cfg.precision == kFP64 && cfg.qpl == 4) return "fp64 supports queries_per_lane 1 or 2";
+  if (cfg.precision != kFP32C && cfg.qpl == 4) return "fp64 supports queries_per_lane 1 or 2";
   if (cfg.splits < 1 || cfg.splits > 65535) return "splits must be in [1, 65535]";
   if (nq > (int64_t)0x7fffffff) return "more than 2^31-1 queries per call";
   const int64_t qblock = (int64_t)kBlockThreads * cfg.qpl;
@@ -262,6 +386,7 @@ const char* fwd_check(int64_t nf, int64_t nq, const FwdConfig& cfg) {
 size_t fwd_workspace_bytes(int64_t nf, int64_t nq, const FwdConfig& cfg) {
   size_t bytes = 0;
   if (cfg.splits > 1) bytes += align_up((size_t)nq * (size_t)cfg.splits * sizeof(double));
+  if (cfg.precision == kFP64 && nf > 0) bytes += align_up((size_t)nf * sizeof(FaceRec64));
   if (cfg.precision == kFP32C && cfg.sort) {
     bytes += align_up((size_t)nq * sizeof(int32_t));
     bytes += align_up((size_t)nq * 3 * sizeof(double));
@@ -309,12 +434,26 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
                       kBlockThreads, 0, stream>>>(dst, nq, 0.0);
     return hipGetLastError();
   }
-  timer_begin(stream);
+  FaceRec64* rec = nullptr;
   if (cfg.precision == kFP64) {
+    rec = (FaceRec64*)cursor;
+    cursor += align_up((size_t)nf * sizeof(FaceRec64));
+    face_rec64_kernel<<<dim3((unsigned)((nf + kBlockThreads - 1) / kBlockThreads)), kBlockThreads,
+                        0, stream>>>(tri, nf, rec);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  timer_begin(stream);
+  if (cfg.precision == kFP64Plain) {
     if (cfg.qpl == 1)
       fwd_f64_kernel<1><<<grid, kBlockThreads, 0, stream>>>(tri, q, out, nf, nq, fps, cfg.splits);
     else
       fwd_f64_kernel<2><<<grid, kBlockThreads, 0, stream>>>(tri, q, out, nf, nq, fps, cfg.splits);
+  } else if (cfg.precision == kFP64) {
+    if (cfg.qpl == 1)
+      fwd_f64f_kernel<1><<<grid, kBlockThreads, 0, stream>>>(rec, q, out, nf, nq, fps, cfg.splits);
+    else
+      fwd_f64f_kernel<2><<<grid, kBlockThreads, 0, stream>>>(rec, q, out, nf, nq, fps, cfg.splits);
   } else {
     if (cfg.qpl == 1)
       fwd_f32c_kernel<1><<<grid, kBlockThreads, 0, stream>>>(tri, q, perm, out, nf, nq, fps,

@@ -12,7 +12,11 @@
 
 namespace wn {
 
-enum Precision : int { kFP64 = 0, kFP32C = 1 };
+// kFP64: FP64 with the far-pair fast path (refined square roots, an
+// arctangent polynomial for far pairs, the reference formula for the rest);
+// kFP32C: FP32 pairs, FP64 accumulation and re-evaluation; kFP64Plain: the
+// reference formula in FP64 for every pair (face kernels only).
+enum Precision : int { kFP64 = 0, kFP32C = 1, kFP64Plain = 2 };
 
 constexpr int kBlockThreads = 256;  // 4 waves
 constexpr int kFaceTile = 256;      // faces staged in LDS per tile (one per thread)
@@ -26,8 +30,15 @@ struct alignas(64) StripRec64 {
   double x, y, z, e1, e2, nx, ny, nz;
 };
 
+// FP64 face record of the FP64 face kernel (wn_forward.hip), built on the
+// device from the face SoA: its corners, 8 x its normal (B - A) x (C - A),
+// and minus half its squared edge lengths |B-A|^2, |C-B|^2, |A-C|^2.
+struct alignas(128) FaceRec64 {
+  double ax, ay, az, bx, by, bz, cx, cy, cz, nx, ny, nz, eab, ebc, eca, pad;
+};
+
 struct FwdConfig {
-  int precision = kFP64;  // kFP64: plain FP64; kFP32C: FP32 + FP64 accumulation/fallback
+  int precision = kFP64;  // a Precision
   int qpl = 1;            // queries per lane (1, 2 or 4)
   int splits = 1;         // face splits (split-K over faces), >= 1
   int sort = 0;           // kFP32C: process queries in Morton order

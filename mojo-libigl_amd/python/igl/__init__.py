@@ -169,11 +169,11 @@ def pack_faces(vertices: np.ndarray, faces: np.ndarray) -> np.ndarray:
 # GPU calls with at least this many queries (by precision mode: 0 = FP64,
 # 1 = FP32-compensated) go through the strip kernels (wn_winding_gpu_mesh
 # here, a kept WindingField in igl.autograd.winding_number): stripifying on
-# the host costs ~0.14 us per face and the strip kernels save ~2.3 ps (FP64)
-# / ~0.42 ps (FP32c) per query-face pair over the face kernels, so a first
-# call breaks even at ~60k / ~330k queries and calls that repeat a kept mesh
-# win well below that.
-STRIP_MIN_QUERIES = {0: 65536, 1: 262144}
+# the host costs ~0.1-0.14 us per face and the strip kernels save ~0.93 ps
+# (FP64) / ~0.36 ps (FP32c) per query-face pair over the face kernels, so a
+# first call breaks even at ~100-150k / ~330k queries and calls that repeat a
+# kept mesh win well below that.
+STRIP_MIN_QUERIES = {0: 131072, 1: 262144}
 
 
 def winding_number(V, F, O, device="cpu", *, precision="fp64"):

@@ -244,6 +244,14 @@ def test_config_validation_through_the_abi():
     assert h.wn_resolve_config(28320, 1024, 0, 0, 0, -1, -1.0, cfg.ctypes.data) == 0
     assert cfg[0] == 1 and cfg[1] > 1  # split-K for few queries
     assert h.wn_resolve_config(10, 10, 7, 0, 0, 0, 1.0, cfg.ctypes.data) == -1
+    assert h.wn_resolve_config(10, 10, 3, 0, 0, 0, 1.0, cfg.ctypes.data) == -1
+    # precision 2 (FP64 plain, the metric's comparison kernel): no sort, qpl 1 or 2
+    assert h.wn_resolve_config(28320, 1024, 2, 0, 0, -1, -1.0, cfg.ctypes.data) == 0
+    assert cfg[0] == 1 and cfg[1] > 1 and cfg[2] == 0 and cfg[3] == 2
+    assert h.wn_resolve_config(10, 10, 2, 4, 0, 0, 1.0, cfg.ctypes.data) == -1
+    # the FP64 face kernel's 128-byte records: workspace grows with nf, plain needs none
+    assert h.wn_forward_workspace_bytes(1000, 256, 2, 1, 1, 0) == 0
+    assert h.wn_forward_workspace_bytes(1000, 256, 0, 1, 1, 0) >= 128 * 1000
     assert h.wn_resolve_config(10, 10, 0, 3, 0, 0, 1.0, cfg.ctypes.data) == -1
     assert h.wn_resolve_config(10, 10, 0, 1, 70000, 0, 1.0, cfg.ctypes.data) == -1
     assert h.wn_resolve_config(10, 2**31, 0, 1, 1, 0, 1.0, cfg.ctypes.data) == -1

@@ -1,8 +1,11 @@
 """GPU parity of the forward kernels against the FP64 oracle (restatement of
 the reference CPU path, src/capi.mojo:205-294).
 
-Bars: FP64 plain within 1e-10 of the oracle (the contract of the reference's
-FP64 mode); FP32-compensated within 1e-6 of the oracle on every query of the
+Precision modes: 0 = FP64 (far pairs by the arctangent polynomial, the rest
+by the reference formula), 1 = FP32-compensated, 2 = FP64 plain (the
+reference formula for every pair). Bars: both FP64 modes within 1e-10 of the
+oracle (the contract of the reference's FP64 mode) and FP64 within 1e-13 of
+FP64 plain; FP32-compensated within 1e-6 of the oracle on every query of the
 sizes run here. All calls go through the C ABI (include/wn.h).
 """
 
@@ -55,20 +58,102 @@ def test_tetrahedron_gpu_matches_cpu(require_gpu):
     assert igl.winding_number(V, F, Q[0], device="gpu") == pytest.approx(1.0, abs=1e-14)
 
 
+FAST_VS_PLAIN_TOL = 1e-13
+
+
+@pytest.mark.parametrize("precision", [0, 2])
 @pytest.mark.parametrize("qpl,splits", [(1, 1), (2, 1), (1, 7), (2, 20), (0, 0)])
-def test_fp64_config1(require_gpu, c1, qpl, splits):
+def test_fp64_config1(require_gpu, c1, precision, qpl, splits):
     V, F, Q, tri, ref = c1
-    out = host_call(tri, Q, 0, qpl, splits)
-    print(f"fp64 c1 qpl={qpl} splits={splits} max|dw|={np.abs(out - ref).max():.3e}", flush=True)
+    out = host_call(tri, Q, precision, qpl, splits)
+    print(f"fp64 mode {precision} c1 qpl={qpl} splits={splits} "
+          f"max|dw|={np.abs(out - ref).max():.3e}", flush=True)
     assert np.abs(out - ref).max() <= FP64_TOL
 
 
 def test_fp64_config2_1024(require_gpu, c2):
     V, F, Q, tri, ref = c2
-    out = host_call(tri, Q, 0)
-    assert np.abs(out - ref).max() <= FP64_TOL
-    # closed torus: integer winding numbers
-    assert np.abs(out - np.round(out)).max() <= FP64_TOL
+    for precision in (0, 2):
+        out = host_call(tri, Q, precision)
+        assert np.abs(out - ref).max() <= FP64_TOL
+        # closed torus: integer winding numbers
+        assert np.abs(out - np.round(out)).max() <= FP64_TOL
+
+
+def special_case_mesh():
+    """An 8 x 8 curved grid (open) plus degenerate faces (a repeated corner,
+    three collinear corners, a zero-area sliver 1e-12 wide), with queries
+    1e-10 from the grid's vertices and 1e-5 from edge midpoints (random
+    directions),
+    on edge lines beyond the edge, 1e-9 and 1e-5 above face centroids, and
+    uniform in a box around it. Points ON the surface are left out: there
+    the reference formula's det and den are both zero for the faces through
+    the point, and atan2 of the signed zeros that rounding leaves gives 0 or
+    +-pi (w off by 1/2 or 1) in any implementation. Next to an edge, at
+    distance d, det and den of its two faces both cancel to O(d): the
+    formula's own error is ~1e-16 x (edge length / d) whatever the order of
+    its operations (6e-8 at d = 1e-9, 1e-10 at 1e-6), hence 1e-5."""
+    V, Fg = W.grid(8)
+    V = np.vstack([V, [[0.3, 0.3, 0.0], [0.5, 0.5, 0.0], [0.7, 0.7, 0.0],
+                       [0.2, 0.8, 0.1], [0.2 + 1e-12, 0.8, 0.1], [0.6, 0.1, 0.3]]])
+    n = len(V) - 6
+    F = np.vstack([Fg, [[0, 0, 5], [n, n + 1, n + 2], [n + 3, n + 4, n + 5]]])
+    rng = np.random.default_rng(44)
+    T = V[Fg]  # edge and face queries from the grid's own faces
+    cen = T.mean(axis=1)
+    nrm = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+    ln = np.linalg.norm(nrm, axis=1, keepdims=True)
+    nrm = np.where(ln > 0, nrm / np.where(ln > 0, ln, 1.0), 0.0)
+    def directions(k):
+        d = rng.normal(size=(k, 3))
+        return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+    Q = np.vstack([V[:n] + 1e-10 * directions(n),
+                   0.5 * (T[:, 0] + T[:, 1]) + 1e-5 * directions(len(Fg)),
+                   T[:, 0] + 0.5 * (T[:, 0] - T[:, 1]),
+                   cen + 1e-9 * nrm, cen - 1e-5 * nrm,
+                   rng.uniform(V.min(0) - 0.5, V.max(0) + 0.5, size=(2000, 3))])
+    near = np.zeros(len(Q), bool)
+    near[:n] = True  # 1e-10 from the grid's vertices
+    return V, F, np.ascontiguousarray(Q), near
+
+
+def test_fp64_fast_path_against_plain(require_gpu, c1, c2):
+    """Mode 0 (far pairs by the polynomial, refined square roots and
+    quotient) agrees with mode 2 (the reference formula for every pair) to
+    1e-13 and both meet the oracle bar: on the reference-size meshes, on
+    degenerate faces and queries next to vertices and edges, on the same mesh
+    moved 1e6 from the origin (queries off the surface by more than the
+    coordinates' rounding), and with queries that are NaN or infinite (NaN
+    either way)."""
+    cases = {"c1": (c1[3], c1[2]), "c2": (c2[3], c2[2])}
+    V, F, Q, near = special_case_mesh()
+    cases["special"] = (oracle.pack_soa(V, F), Q)
+    # at 1e6 the coordinates' rounding (1.2e-10) would put the queries next to
+    # vertices onto the surface: those are left out there
+    shift = np.array([1e6, -3e5, 2e6])
+    cases["special at 1e6"] = (oracle.pack_soa(V + shift, F), np.ascontiguousarray(Q[~near] + shift))
+    for name, (tri, Qc) in cases.items():
+        fast = host_call(tri, Qc, 0)
+        plain = host_call(tri, Qc, 2)
+        ref = oracle.winding_soa(tri, Qc)
+        d = np.abs(fast - plain)
+        print(f"{name}: fast vs plain {d.max():.3e}, fast vs oracle "
+              f"{np.abs(fast - ref).max():.3e}, plain vs oracle {np.abs(plain - ref).max():.3e}",
+              flush=True)
+        assert d.max() <= FAST_VS_PLAIN_TOL, name
+        assert np.abs(fast - ref).max() <= FP64_TOL, name
+        assert np.abs(plain - ref).max() <= FP64_TOL, name
+    tri, Qc = cases["c1"]
+    Qn = Qc[:300].copy()
+    Qn[7] = np.nan
+    Qn[8, 1] = np.inf
+    for precision in (0, 2):
+        out = host_call(tri, Qn, precision)
+        assert np.isnan(out[7]) and np.isnan(out[8]), precision
+        keep = np.ones(len(Qn), bool)
+        keep[[7, 8]] = False
+        assert np.abs(out[keep] - c1[4][:300][keep]).max() <= FP64_TOL
 
 
 @pytest.mark.parametrize("qpl,splits,sort", [(1, 1, 0), (2, 1, 0), (4, 1, 0), (2, 9, 1), (0, 0, -1)])
@@ -132,7 +217,7 @@ def test_ragged_sizes(require_gpu, c1, nf_take, nq):
     t = oracle.pack_soa(V, Fs)
     r = oracle.winding_soa(t, Qs)
     tol32 = FP32C_TOL
-    for precision, tol in ((0, FP64_TOL), (1, tol32)):
+    for precision, tol in ((0, FP64_TOL), (1, tol32), (2, FP64_TOL)):
         for splits in (1, 3):
             out = host_call(t, Qs, precision, 1, splits, 0)
             assert np.abs(out - r).max() <= tol, (precision, splits)
@@ -141,7 +226,7 @@ def test_ragged_sizes(require_gpu, c1, nf_take, nq):
 def test_empty_mesh_and_queries(require_gpu):
     tri = np.zeros((9, 0))
     Q = np.random.default_rng(0).normal(size=(10, 3))
-    for precision in (0, 1):
+    for precision in (0, 1, 2):
         assert np.array_equal(host_call(tri, Q, precision), np.zeros(10))
     V, F = W.tetrahedron()
     assert igl.winding_number(V, F, np.zeros((0, 3)), device="gpu").shape == (0,)
@@ -149,7 +234,7 @@ def test_empty_mesh_and_queries(require_gpu):
 
 def test_deterministic_run_to_run(require_gpu, c2):
     V, F, Q, tri, ref = c2
-    for precision in (0, 1):
+    for precision in (0, 1, 2):
         a = host_call(tri, Q, precision, 0, 16)
         b = host_call(tri, Q, precision, 0, 16)
         assert np.array_equal(a, b)
@@ -174,7 +259,7 @@ def test_device_api_matches_host_api(require_gpu, c1):
     torch.cuda.synchronize()
     assert int(bad.item()) == 2**31 - 1
     assert np.array_equal(trid.cpu().numpy(), tri)
-    for precision in (0, 1):
+    for precision in (0, 1, 2):
         nbytes = h.wn_forward_workspace_bytes(len(F), len(Q), precision, 0, 0, -1)
         assert nbytes >= 0
         ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
@@ -213,16 +298,20 @@ def test_injected_fault_raises(require_gpu, monkeypatch):
 
 
 def test_config2_full_size_properties(require_gpu):
-    """28,320 faces x 1,048,576 queries: FP32-compensated against FP64 plain
-    on the GPU (size-independent properties: agreement <= 1e-6 and, on the
-    closed torus, integer values), plus the oracle on a strided subsample."""
+    """28,320 faces x 1,048,576 queries: FP32-compensated and FP64 against
+    FP64 plain on the GPU (size-independent properties: agreement <= 1e-6 and
+    1e-13 and, on the closed torus, integer values), plus the oracle on a
+    strided subsample."""
     V, F, Q = W.config2(1_048_576)
     tri = oracle.pack_soa(V, F)
+    plain = host_call(tri, Q, 2)
     w64 = host_call(tri, Q, 0)
     w32 = host_call(tri, Q, 1)
-    d = np.abs(w32 - w64)
-    print(f"c2@1M fp32c vs fp64: max {d.max():.3e} mean {d.mean():.3e}", flush=True)
+    d = np.abs(w32 - plain)
+    print(f"c2@1M fp32c vs fp64 plain: max {d.max():.3e} mean {d.mean():.3e}; fp64 vs fp64 "
+          f"plain {np.abs(w64 - plain).max():.3e}", flush=True)
     assert d.max() <= FP32C_TOL
+    assert np.abs(w64 - plain).max() <= FAST_VS_PLAIN_TOL
     assert np.abs(w64 - np.round(w64)).max() <= 1e-9
     sub = np.arange(0, len(Q), 4099)
     ref = oracle.winding_soa(tri, Q[sub])
@@ -297,9 +386,10 @@ def test_large_calls_use_strip_kernels(require_gpu, monkeypatch):
 
 def test_reference_entry_large_calls_use_fp64_strips(require_gpu):
     """mli_winding_gpu (the reference's ABI entry, face SoA only) with >=
-    65,536 queries finds the faces' shared corners by coordinate equality and
-    runs the FP64 strip kernel: within 1e-12 of the plain FP64 face kernel
-    (wn_winding_gpu_ex, precision 0) and within the FP64 bar of the oracle; a
+    262,144 queries finds the faces' shared corners by coordinate equality and
+    runs the FP64 strip kernel: within 1e-12 of the FP64 face kernels
+    (wn_winding_gpu_ex, precisions 0 and 2) and within the FP64 bar of the
+    oracle; a
     repeated call (kept strip mesh) gives the same bits; a non-manifold soup
     (repeated and reversed faces, coordinate-duplicated vertices) too."""
     from igl import pack_faces
@@ -307,17 +397,20 @@ def test_reference_entry_large_calls_use_fp64_strips(require_gpu):
 
     h = lib()
     V, F, Q = W.config2(1 << 20)
-    Q = np.ascontiguousarray(Q[:70_000])
+    Q = np.ascontiguousarray(Q[:270_000])
     soup_V, soup_F = W.icosphere(2)
     soup_V = np.vstack([soup_V, soup_V[:40]])  # same coordinates, other ids
     soup_F = np.vstack([soup_F, soup_F[:30, ::-1], soup_F[5:9]])
     soup_F[::7, 0] = np.where(soup_F[::7, 0] < 40, soup_F[::7, 0] + len(soup_V) - 40,
                               soup_F[::7, 0])
-    soup_Q = np.ascontiguousarray(np.random.default_rng(9).uniform(-1.5, 1.5, size=(65_536, 3)))
+    soup_Q = np.ascontiguousarray(np.random.default_rng(9).uniform(-1.5, 1.5, size=(262_144, 3)))
     for name, (tri, Qc) in {"torus": (pack_faces(V, F), Q),
                             "soup": (pack_faces(soup_V, soup_F), soup_Q)}.items():
         face = np.full(len(Qc), np.nan)
         assert h.wn_winding_gpu_ex(addr(tri), addr(Qc), addr(face), tri.shape[1], len(Qc), 0, 0,
+                                   0, 0, -1.0) == 0, igl._lib.last_error()
+        plain = np.full(len(Qc), np.nan)
+        assert h.wn_winding_gpu_ex(addr(tri), addr(Qc), addr(plain), tri.shape[1], len(Qc), 2, 0,
                                    0, 0, -1.0) == 0, igl._lib.last_error()
         out = np.full(len(Qc), np.nan)
         assert h.mli_winding_gpu(addr(tri), addr(Qc), addr(out), tri.shape[1], len(Qc)) == 1
@@ -330,6 +423,7 @@ def test_reference_entry_large_calls_use_fp64_strips(require_gpu):
               flush=True)
         assert np.array_equal(out, again)
         assert np.abs(out - face).max() <= 1e-12
+        assert np.abs(out - plain).max() <= 1e-12
         assert np.abs(out[sub] - ref).max() <= FP64_TOL
 
 
