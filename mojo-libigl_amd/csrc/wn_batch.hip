@@ -27,7 +27,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <type_traits>
 #include <vector>
 
 #include "wn_internal.h"
@@ -41,13 +40,15 @@ __global__ void counter_reset_kernel(int32_t* counter) {
   if (threadIdx.x == 0) counter[0] = 0;
 }
 
-template <int PREC, int QPL>
+// FP32-compensated face plans (FP64 face plans run batch_f64f_kernel of
+// wn_forward.hip over FP64 face records).
+template <int QPL>
 __global__ __launch_bounds__(kBlockThreads) void batch_kernel(
     const double* __restrict__ tri, int64_t ftot, const double* __restrict__ q,
     const int32_t* __restrict__ perm, double* __restrict__ dst, double* __restrict__ partial,
     const BatchMesh* __restrict__ meshes, const BatchItem* __restrict__ items, int64_t n_items,
     int32_t* __restrict__ counter, int64_t chunk_faces, float kappa) {
-  __shared__ typename std::conditional<PREC == kFP64, SweepF64Smem, SweepF32Smem>::type sm;
+  __shared__ SweepF32Smem sm;
   __shared__ int32_t s_item;
   const int tid = threadIdx.x;
   constexpr int QB = kBlockThreads * QPL;
@@ -79,20 +80,16 @@ __global__ __launch_bounds__(kBlockThreads) void batch_kernel(
       Oz[k] = valid[k] ? q[3 * slot + 2] : 0.0;
       acc[k] = 0.0;
     }
-    if constexpr (PREC == kFP64) {
-      sweep_f64<QPL>(tri, ftot, f0, f1, Ox, Oy, Oz, acc, sm);
-    } else {
-      double gx, gy, gz;
-      block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz);
-      float ox[QPL], oy[QPL], oz[QPL];
+    double gx, gy, gz;
+    block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz);
+    float ox[QPL], oy[QPL], oz[QPL];
 #pragma unroll
-      for (int k = 0; k < QPL; ++k) {
-        ox[k] = valid[k] ? (float)(Ox[k] - gx) : 0.0f;
-        oy[k] = valid[k] ? (float)(Oy[k] - gy) : 0.0f;
-        oz[k] = valid[k] ? (float)(Oz[k] - gz) : 0.0f;
-      }
-      sweep_f32c<QPL>(tri, ftot, f0, f1, gx, gy, gz, ox, oy, oz, kappa, acc, sm);
+    for (int k = 0; k < QPL; ++k) {
+      ox[k] = valid[k] ? (float)(Ox[k] - gx) : 0.0f;
+      oy[k] = valid[k] ? (float)(Oy[k] - gy) : 0.0f;
+      oz[k] = valid[k] ? (float)(Oz[k] - gz) : 0.0f;
     }
+    sweep_f32c<QPL>(tri, ftot, f0, f1, gx, gy, gz, ox, oy, oz, kappa, acc, sm);
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
       if (valid[k]) {
@@ -157,6 +154,7 @@ struct BatchPlan {
   void* d_sort_ws = nullptr;
   size_t sort_bytes = 0;
   StripRec64* d_rec64 = nullptr;  // FP64 strip plans: sweep records, rebuilt per run
+  FaceRec64* d_frec64 = nullptr;  // FP64 face plans: face records, rebuilt per run
   // Boundary correction of FP32-compensated plans (wn_ribbon.hip): one block
   // per (mesh, query tile), each mesh's edge range. Face plans find their
   // boundary edges from the SoA on every run (the pass before the graph);
@@ -183,9 +181,8 @@ struct BatchPlan {
 };
 
 static const void* batch_kernel_ptr(int precision, int qpl) {
-  if (precision == kFP64)
-    return qpl == 1 ? (const void*)batch_kernel<kFP64, 1> : (const void*)batch_kernel<kFP64, 2>;
-  return qpl == 1 ? (const void*)batch_kernel<kFP32C, 1> : (const void*)batch_kernel<kFP32C, 2>;
+  if (precision == kFP64) return batch_face64_kernel(qpl);
+  return qpl == 1 ? (const void*)batch_kernel<1> : (const void*)batch_kernel<2>;
 }
 
 const char* batch_check(const int64_t* face_off, const int64_t* query_off, int64_t n_meshes,
@@ -360,9 +357,12 @@ hipError_t batch_create(const int64_t* face_off, const int64_t* query_off, int64
     e = plan_boundary(p, face_off, QB);
     p->boundary = e == hipSuccess;
   }
+  if (e == hipSuccess && precision == kFP64 && p->ftot > 0)
+    e = hipMalloc(&p->d_frec64, (size_t)p->ftot * sizeof(FaceRec64));
   if (e != hipSuccess) {
     if (p->ws) (void)hipFree(p->ws);
     if (p->ws2) (void)hipFree(p->ws2);
+    if (p->d_frec64) (void)hipFree(p->d_frec64);
     delete p;
     return e;
   }
@@ -423,21 +423,26 @@ hipError_t batch_create_strips(const int64_t* rec_off, const int64_t* query_off,
   return hipSuccess;
 }
 
-static void face_batch_launch(const BatchPlan* p, const double* tri, const double* q,
-                              const int32_t* perm, double* dst, hipStream_t s) {
-  const dim3 grid((unsigned)p->grid);
-#define WN_BATCH_LAUNCH(PREC, QPL)                                                             \
-  batch_kernel<PREC, QPL><<<grid, kBlockThreads, 0, s>>>(tri, p->ftot, q, perm, dst,          \
-                                                          p->d_partial, p->d_meshes,          \
-                                                          p->d_items, p->n_items,             \
-                                                          p->d_counter, p->chunk_faces,       \
-                                                          p->kappa)
+static hipError_t face_batch_launch(const BatchPlan* p, const double* tri, const double* q,
+                                    const int32_t* perm, double* dst, hipStream_t s) {
   if (p->precision == kFP64) {
-    if (p->qpl == 1) WN_BATCH_LAUNCH(kFP64, 1); else WN_BATCH_LAUNCH(kFP64, 2);
-  } else {
-    if (p->qpl == 1) WN_BATCH_LAUNCH(kFP32C, 1); else WN_BATCH_LAUNCH(kFP32C, 2);
+    // (no faces at all: no records, and every item writes w = 0)
+    hipError_t e = face_rec64_launch(tri, p->ftot, p->d_frec64, s);
+    if (e == hipSuccess)
+      e = batch_face64_enqueue(p->d_frec64, q, perm, dst, p->d_partial, p->d_meshes, p->d_items,
+                               p->n_items, p->d_counter, p->chunk_faces, p->qpl, p->grid, s);
+    return e;
   }
-#undef WN_BATCH_LAUNCH
+  const dim3 grid((unsigned)p->grid);
+  if (p->qpl == 1)
+    batch_kernel<1><<<grid, kBlockThreads, 0, s>>>(tri, p->ftot, q, perm, dst, p->d_partial,
+                                                   p->d_meshes, p->d_items, p->n_items,
+                                                   p->d_counter, p->chunk_faces, p->kappa);
+  else
+    batch_kernel<2><<<grid, kBlockThreads, 0, s>>>(tri, p->ftot, q, perm, dst, p->d_partial,
+                                                   p->d_meshes, p->d_items, p->n_items,
+                                                   p->d_counter, p->chunk_faces, p->kappa);
+  return hipGetLastError();
 }
 
 // tri: the 9 x F_total face SoA, or the strip records of a strip plan.
@@ -459,7 +464,8 @@ static hipError_t batch_enqueue(BatchPlan* p, const double* tri, const double* q
                                              p->chunk_faces, p->kappa, p->grid, s);
     if (e != hipSuccess) return e;
   } else {
-    face_batch_launch(p, tri, q, perm, dst, s);
+    const hipError_t e = face_batch_launch(p, tri, q, perm, dst, s);
+    if (e != hipSuccess) return e;
   }
   if (p->n_partials > 0 && p->qtot > 0)
     batch_reduce_kernel<<<dim3((unsigned)((p->qtot + kBlockThreads - 1) / kBlockThreads)),
@@ -533,9 +539,10 @@ void batch_destroy(BatchPlan* p) {
     (void)hipStreamSynchronize(p->cap_stream);
     (void)hipStreamDestroy(p->cap_stream);
   }
-  if (p->ws || p->d_rec64 || p->ws2) (void)hipDeviceSynchronize();
+  if (p->ws || p->d_rec64 || p->d_frec64 || p->ws2) (void)hipDeviceSynchronize();
   if (p->ws) (void)hipFree(p->ws);
   if (p->d_rec64) (void)hipFree(p->d_rec64);
+  if (p->d_frec64) (void)hipFree(p->d_frec64);
   if (p->ws2) (void)hipFree(p->ws2);
   delete p;
 }

@@ -183,6 +183,65 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
   }
 }
 
+// Persistent FP64 face kernel over a batch plan's work list (mesh, query
+// tile of 256*QPL queries, face chunk), with the batch kernels' item
+// protocol (atomic counter, block-uniform exit) and output rule (single-chunk
+// meshes write w, others unscaled chunk-major partials summed in chunk
+// order). rec holds the records of the batch's 9 x F_total SoA.
+template <int QPL>
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(QPL == 1 ? 6 : 4))) void batch_f64f_kernel(
+    const FaceRec64* __restrict__ rec, const double* __restrict__ q,
+    const int32_t* __restrict__ perm, double* __restrict__ dst, double* __restrict__ partial,
+    const BatchMesh* __restrict__ meshes, const BatchItem* __restrict__ items, int64_t n_items,
+    int32_t* __restrict__ counter, int64_t chunk_faces) {
+  __shared__ int32_t s_item;
+  const int tid = threadIdx.x;
+  constexpr int QB = kBlockThreads * QPL;
+  for (;;) {
+    __syncthreads();  // every thread has read the previous s_item
+    if (tid == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= n_items) break;  // block-uniform exit
+    const BatchItem item = items[it];
+    const BatchMesh m = meshes[item.mesh];
+    const int64_t slot0 = m.query_off + (int64_t)item.qtile * QB;
+    const int64_t slot_end = m.query_off + m.nq;
+    const int64_t f0 = m.face_off + (int64_t)item.chunk * chunk_faces;
+    int64_t f1 = f0 + chunk_faces;
+    if (f1 > m.face_off + m.nf) f1 = m.face_off + m.nf;
+    bool valid[QPL];
+    int64_t qidx[QPL];
+    double ox[QPL], oy[QPL], oz[QPL], acc[QPL], acc_slow[QPL];
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      const int64_t slot = slot0 + k * kBlockThreads + tid;
+      valid[k] = slot < slot_end;
+      qidx[k] = valid[k] ? (perm ? (int64_t)perm[slot] : slot) : 0;
+      // q is in launch order (gathered through perm beforehand when sorted)
+      ox[k] = valid[k] ? q[3 * slot] : 0.0;
+      oy[k] = valid[k] ? q[3 * slot + 1] : 0.0;
+      oz[k] = valid[k] ? q[3 * slot + 2] : 0.0;
+      acc[k] = 0.0;
+      acc_slow[k] = 0.0;
+    }
+    const FaceRec64* r = rec + f0;
+    const int cnt = (int)(f1 - f0);  // a mesh has at most 2^31-1 faces
+    for (int j = 0; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      if (valid[k]) {
+        const double a = acc[k] + acc_slow[k];
+        const int64_t slot = slot0 + k * kBlockThreads + tid;
+        if (m.chunks == 1)
+          dst[qidx[k]] = a * kInv2Pi;
+        else  // chunk-major, slot order: a block's writes are contiguous
+          partial[m.pbase + (int64_t)item.chunk * m.nq + (slot - m.query_off)] = a;
+      }
+    }
+  }
+}
+
 // --------------------------------------------------------------------------
 // FP32-compensated.
 //  * local origin: the block's query bbox centre g (FP64). Query offsets
@@ -330,6 +389,30 @@ hipError_t fill_launch(double* p, int64_t n, double v, hipStream_t stream) {
   return hipGetLastError();
 }
 
+hipError_t face_rec64_launch(const double* tri, int64_t nf, FaceRec64* out, hipStream_t stream) {
+  if (nf <= 0) return hipSuccess;
+  face_rec64_kernel<<<dim3((unsigned)((nf + kBlockThreads - 1) / kBlockThreads)), kBlockThreads, 0,
+                      stream>>>(tri, nf, out);
+  return hipGetLastError();
+}
+
+const void* batch_face64_kernel(int qpl) {
+  return qpl == 1 ? (const void*)batch_f64f_kernel<1> : (const void*)batch_f64f_kernel<2>;
+}
+
+hipError_t batch_face64_enqueue(const FaceRec64* rec, const double* q, const int32_t* perm,
+                                double* dst, double* partial, const BatchMesh* meshes,
+                                const BatchItem* items, int64_t n_items, int32_t* counter,
+                                int64_t chunk_faces, int qpl, int grid, hipStream_t stream) {
+  if (qpl == 1)
+    batch_f64f_kernel<1><<<dim3((unsigned)grid), kBlockThreads, 0, stream>>>(
+        rec, q, perm, dst, partial, meshes, items, n_items, counter, chunk_faces);
+  else
+    batch_f64f_kernel<2><<<dim3((unsigned)grid), kBlockThreads, 0, stream>>>(
+        rec, q, perm, dst, partial, meshes, items, n_items, counter, chunk_faces);
+  return hipGetLastError();
+}
+
 hipError_t reduce_splits_launch(const double* partial, const int32_t* perm, double* dst,
                                 int64_t nq, int splits, hipStream_t stream) {
   if (nq <= 0) return hipSuccess;
@@ -438,9 +521,7 @@ hipError_t fwd_launch(const double* tri, const double* q, double* dst, int64_t n
   if (cfg.precision == kFP64) {
     rec = (FaceRec64*)cursor;
     cursor += align_up((size_t)nf * sizeof(FaceRec64));
-    face_rec64_kernel<<<dim3((unsigned)((nf + kBlockThreads - 1) / kBlockThreads)), kBlockThreads,
-                        0, stream>>>(tri, nf, rec);
-    const hipError_t e = hipGetLastError();
+    const hipError_t e = face_rec64_launch(tri, nf, rec, stream);
     if (e != hipSuccess) return e;
   }
   timer_begin(stream);

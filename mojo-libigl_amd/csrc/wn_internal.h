@@ -231,6 +231,15 @@ struct BatchItem {
   int32_t pad;
 };
 
+// FP64 face records of a 9 x nf SoA (wn_forward.hip), and the persistent
+// FP64 face kernel of batch face plans over them (qpl 1 or 2).
+hipError_t face_rec64_launch(const double* tri, int64_t nf, FaceRec64* out, hipStream_t stream);
+const void* batch_face64_kernel(int qpl);
+hipError_t batch_face64_enqueue(const FaceRec64* rec, const double* q, const int32_t* perm,
+                                double* dst, double* partial, const BatchMesh* meshes,
+                                const BatchItem* items, int64_t n_items, int32_t* counter,
+                                int64_t chunk_faces, int qpl, int grid, hipStream_t stream);
+
 // Persistent strip kernel over a work list (wn_strip.hip): one query per
 // lane, blocks of kBlockThreads queries, chunks of chunk_tiles tiles.
 int batch_strip_resident_blocks();
