@@ -143,6 +143,60 @@ __device__ __forceinline__ float atan_small_x8_f32(float x) {
   return x * p;
 }
 
+// atan(x / 8) for |x| <= 3.2 (|x / 8| <= 0.4): r * Q(r^2), Q of degree 4 in
+// r^2, relative-error minimax on (0, 0.4] (max 1.30e-8, gen_atan_coeffs.py,
+// n = 5), with its coefficients scaled by powers of two so that it takes
+// x = 8 r. The angle of a product of several far faces' (den + i det).
+__device__ __forceinline__ float atan_sum_x8_f32(float x) {
+  const float t = x * x;
+  float p = 0.08156783959325319f * (1.0f / 134217728.0f);  // x^9: 8^9
+  p = __builtin_fmaf(p, t, -0.13901368582940957f * (1.0f / 2097152.0f));  // x^7: 8^7
+  p = __builtin_fmaf(p, t, 0.19978842639359343f * (1.0f / 32768.0f));    // x^5: 8^5
+  p = __builtin_fmaf(p, t, -0.33332919866326131f * (1.0f / 512.0f));     // x^3: 8^3
+  p = __builtin_fmaf(p, t, 0.99999998698746807f * 0.125f);               // x^1: 8
+  return x * p;
+}
+
+// The angles of three faces, summed as the argument of the product of
+// their (den + i det): from det8 = 8 det and den of each,
+//   z12 = w1 w2:  re12 = d1 d2 - det8_1 det8_2 / 64,  8 im12 = d1 det8_2 + det8_1 d2
+//   z = z12 w3:   re = re12 d3 - 8 im12 det8_3 / 64,   8 im = re12 det8_3 + 8 im12 d3
+// and x8 = 8 im / re = 8 tan(angle_1 + angle_2 + angle_3). Valid (no wrap,
+// x8 in the polynomial's range, no overflow or underflow) when every face
+// has den > la lb lc (each angle within +-pi/2, so the sum within +-3pi/2:
+// the product's argument is the sum itself once it lies within +-0.39),
+// re is a positive normal number and |x8| < 3.2 (tan of the sum < 0.4);
+// wave_all_sum3 tests that. The absolute error of the argument is that of
+// adding the three FP32 angles: a few ulps of |angle_1| + |angle_2| +
+// |angle_3|.
+struct Sum3 {
+  float re, x8;
+};
+
+__device__ __forceinline__ Sum3 sum3_x8(float e1, float d1, float e2, float d2, float e3,
+                                        float d3) {
+  const float re12 = __builtin_fmaf(e1 * e2, -1.0f / 64.0f, d1 * d2);
+  const float j12 = __builtin_fmaf(d1, e2, e1 * d2);
+  Sum3 s;
+  s.re = __builtin_fmaf(j12 * e3, -1.0f / 64.0f, re12 * d3);
+  s.x8 = __builtin_fmaf(re12, e3, j12 * d3) * __builtin_amdgcn_rcpf(s.re);
+  return s;
+}
+
+// Every active lane meets sum3_x8's conditions (dp_k = den_k - la lb lc).
+__device__ __forceinline__ bool wave_all_sum3(float dp1, float dp2, float dp3, const Sum3& s) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(dp1 > 0.0f);
+  const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp2 > 0.0f);
+  const uint64_t m3 = __builtin_amdgcn_ballot_w64(dp3 > 0.0f);
+  // re positive normal (not zero, denormal, infinite or NaN): one
+  // v_cmp_class_f32 writing the lane mask (the class builtin goes through a
+  // VGPR select and a second compare before its ballot)
+  uint64_t m4;
+  asm volatile("v_cmp_class_f32_e64 %0, %1, %2" : "=s"(m4) : "v"(s.re), "s"(1 << 8));
+  const uint64_t m5 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x8) < 3.2f);
+  return (m1 & m2 & m3 & m4 & m5) == __builtin_amdgcn_read_exec();
+}
+
 // Geometry of one pair in FP32 from a record. Algebra (exact identities,
 // cheaper and better conditioned than the direct form):
 //   det = N . a              with N = (B-A) x (C-A)  (det(a,b,c) = det(a,b-a,c-a))

@@ -24,9 +24,13 @@
 // the doubled forms, scaled exactly).
 // The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
 // orientation vs strip order) is folded into N8; a record with flag 0 is
-// staged with N8 = 0 and R1.x = +inf, which makes its angle 0 without a
+// staged with N8 = 0 and R1.x = +2^30, which makes its angle 0 without a
 // branch. The flag itself is read (from the records, at a wave-uniform
 // address) only on the rare slow path.
+// Fast path: for each group of three records, the sum of their angles is the
+// argument of the product of the faces' (den + i det) (sum3_x8): one
+// reciprocal and one 5-term polynomial per three faces instead of three of
+// each.
 // Squared lengths and N2 are computed in FP64 from the rounded FP32 corners.
 
 #include "wn_device.h"
@@ -64,12 +68,13 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
   // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag; stored
   // as 8 N so that the lane's dot product is 8 det
-  // A record that closes no face (flag 0) gets N = 0 and +inf in place of
-  // -|w_k - w_{k-2}|^2 / 2: c.a and with it dp and den are then +inf, and
-  // its fast-path angle is 8 det / den = 0 / inf = 0 exactly (see strip_step).
+  // A record that closes no face (flag 0) gets N = 0 and +2^30 in place of
+  // -|w_k - w_{k-2}|^2 / 2: det8 is then 0 exactly and c.a ~ 2^30 makes dp
+  // and den positive and finite (see strip_record), so both fast paths give
+  // it angle 0 (0 / den, and a real factor den + 0i of the triple product).
   const double f8 = 8.0 * c.w;
   sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
-  sm.r1[j] = make_float4(c.w == 0.0 ? __builtin_inff()
+  sm.r1[j] = make_float4(c.w == 0.0 ? 1073741824.0f
                                     : (float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
                          (float)(f8 * (ey * d2z - ez * d2y)), (float)(f8 * (ez * d2x - ex * d2z)),
                          (float)(f8 * (ex * d2y - ey * d2x)));
@@ -118,26 +123,33 @@ __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& 
 }
 
 // Record j of the tile closes face (P, Q, R) (P and Q from the two previous
-// records, R computed here); its angle is added to part, the FP32 sum of at
-// most three angles that the caller adds to the FP64 sum. Every record takes
-// the same instructions: one that closes no face (flag 0: the first two of a
-// strip, tile padding) was staged (stage_strip_tile) with N8 = 0 and c.a =
-// +inf, so dp = den = +inf and its fast-path angle is 0 / inf = 0 exactly.
-// Its flag (raw bits of the record's w; read from global memory at a
-// wave-uniform address) is looked at only on the slow path, the one place
-// where such a record could come out nonzero (its Q.l = 0 makes dp a NaN).
+// records, R computed here): its FP32 geometry. Every record takes the same
+// instructions: one that closes no face (flag 0: the first two of a strip,
+// tile padding) was staged (stage_strip_tile) with N8 = 0 and c.a = +2^30,
+// so det8 = 0 exactly and, with Q's length above ~1e-9 of the mesh's scale,
+// dp and den are positive: its angle is 0 on both fast paths.
 // boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
-__device__ __forceinline__ void strip_step(const StripSmem& sm,
-                                           const long long* __restrict__ flags, int j, int boff,
-                                           VSlot& P, VSlot& Q, VSlot& R, float& s_carry,
-                                           float& part, float ox, float oy, float oz, float kappa) {
-#pragma clang fp contract(off)  // part += angle rounds the angle first
+__device__ __forceinline__ StripGeom strip_record(const StripSmem& sm, int boff, const VSlot& P,
+                                                  const VSlot& Q, VSlot& R, float& s_carry,
+                                                  float ox, float oy, float oz) {
   const float4 r0 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r0) + boff);
   strip_vertex(r0, ox, oy, oz, R);
   const float4 r1 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(sm.r1) + boff);
   const StripGeom g = strip_geom(r0, r1, P, Q, R, s_carry);
-  // far pairs (|det / den| < 1/8 and den > la lb lc on every lane): the
-  // small-angle arctangent
+  s_carry = g.s_qr;
+  return g;
+}
+
+// The FP32 angle of face j on its own (geometry g, rl = the length of its
+// newest vertex): the small-angle arctangent when every lane is far
+// (|det / den| < 1/8 and den > la lb lc), else atan2 in FP32 and, for an
+// ill-conditioned pair, the FP64 angle rounded once. The record flag (raw
+// bits of the record's w; read from global memory at a wave-uniform
+// address) is looked at only here, off the fast path.
+__device__ __forceinline__ float strip_angle(const StripSmem& sm,
+                                             const long long* __restrict__ flags, int j,
+                                             const StripGeom& g, float rl, float ox, float oy,
+                                             float oz, float kappa) {
   float ang = atan_small_x8_f32(g.det8 * __builtin_amdgcn_rcpf(g.den));
   if (__builtin_expect(!wave_all_far_x8(g.det8, g.den, g.dp), 0)) {
     asm volatile("" ::);  // keep a real wave-uniform branch
@@ -150,12 +162,36 @@ __device__ __forceinline__ void strip_step(const StripSmem& sm,
       // ill-conditioned (mx < kappa x la lb lc): the FP64 angle, rounded
       // once, replaces the FP32 one. Flag -1.0 has the sign bit set: the
       // face runs against the strip order.
-      if (mx < kappa * (g.pq * R.l))
+      if (mx < kappa * (g.pq * rl))
         ang = (float)strip_pair_f64(sm, j, flag_bits < 0, ox, oy, oz);
     }
   }
-  part += ang;
-  s_carry = g.s_qr;
+  return ang;
+}
+
+// Records j, j+1, j+2 (slots A, B, C rotating as P, Q, R): the FP32 sum of
+// their three angles. When every lane meets sum3_x8's conditions the sum is
+// the argument of the product of the faces' (den + i det), one reciprocal
+// and one polynomial for the three faces; otherwise each face takes
+// strip_angle and the three FP32 angles are added (-0 + x == x: the first
+// add folds away).
+__device__ __forceinline__ float strip_triple(const StripSmem& sm,
+                                              const long long* __restrict__ flags, int j,
+                                              int boff, VSlot& A, VSlot& B, VSlot& C,
+                                              float& s_carry, float ox, float oy, float oz,
+                                              float kappa) {
+#pragma clang fp contract(off)  // part += angle rounds the angle first
+  const StripGeom g1 = strip_record(sm, boff, A, B, C, s_carry, ox, oy, oz);
+  const StripGeom g2 = strip_record(sm, boff + 16, B, C, A, s_carry, ox, oy, oz);
+  const StripGeom g3 = strip_record(sm, boff + 32, C, A, B, s_carry, ox, oy, oz);
+  const Sum3 s = sum3_x8(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
+  if (__builtin_expect(wave_all_sum3(g1.dp, g2.dp, g3.dp, s), 1)) return atan_sum_x8_f32(s.x8);
+  asm volatile("" ::);  // keep a real wave-uniform branch
+  float part = -0.0f;
+  part += strip_angle(sm, flags, j, g1, C.l, ox, oy, oz, kappa);
+  part += strip_angle(sm, flags, j + 1, g2, A.l, ox, oy, oz, kappa);
+  part += strip_angle(sm, flags, j + 2, g3, B.l, ox, oy, oz, kappa);
+  return part;
 }
 
 // First thread of the calling wave, in a scalar register (all lanes active).
@@ -224,13 +260,9 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     int boff = 0;
     for (int j = 0; j < kStripTileRecords; j += 3) {
       asm volatile("" : "+v"(boff));
-      // the three records' FP32 angles are summed in FP32 (at most three
-      // terms, each |angle| <= pi) and added to the FP64 sum once
-      float part = -0.0f;  // -0 + x == x for every x: the first add folds away
-      strip_step(sm, flags, j, boff, A, B, C, s_carry, part, ox, oy, oz, kappa);
-      strip_step(sm, flags, j + 1, boff + 16, B, C, A, s_carry, part, ox, oy, oz, kappa);
-      strip_step(sm, flags, j + 2, boff + 32, C, A, B, s_carry, part, ox, oy, oz, kappa);
-      acc += (double)part;
+      // the three records' angles are summed in FP32 (at most three terms,
+      // each |angle| <= pi) and added to the FP64 sum once
+      acc += (double)strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
       boff += 3 * (int)sizeof(float4);
     }
   }

@@ -4,8 +4,11 @@ is a variant of, through the C ABI.
 
 Bars: with the same splits and Morton order the two agree bit for bit; with
 the default configuration (whose split count the static entry lowers until its
-scratch fits) within 1e-12; both within 1e-6 of the oracle on a subsample (a
-closed torus). Each case runs two calls in a row, the second with fewer
+scratch fits) within 1e-7: the split count decides which groups of three
+records start a split, and such a group sums its three FP32 angles face by
+face where elsewhere it takes them as one product (FP32-level rounding,
+~5e-9 per group); both within 1e-6 of the oracle on a subsample (a closed
+torus). Each case runs two calls in a row, the second with fewer
 queries, so the second reads a scratch the first left behind. A configuration
 beyond the 64 MiB scratch fails with -4 before anything is launched, and a
 call leaves the device's free memory unchanged.
@@ -80,7 +83,7 @@ def test_noalloc_default_config_and_oracle(require_gpu, torus_mesh):
         Qd = torch.tensor(Q[:n], device="cuda")
         ref = _workspace_entry(h, mesh, Qd, 0, -1).cpu().numpy()
         got = _noalloc_entry(h, mesh, Qd, 0, -1).cpu().numpy()
-        assert np.abs(got - ref).max() <= 1e-12
+        assert np.abs(got - ref).max() <= 1e-7
         s = sub[sub < n]
         assert np.abs(got[s] - expect[: len(s)]).max() <= 1e-6
 
