@@ -92,8 +92,8 @@ __device__ __forceinline__ void block_origin(const double (&Ox)[QPL], const doub
 }
 
 // One face of a tile for QPL queries: its FP32 angle is added to part[k],
-// the FP32 sum of at most two consecutive faces' angles that the caller adds
-// to the FP64 sum. Far pairs on every lane of the wave (|det / den| < 1/8 and
+// the FP32 sum of at most three consecutive faces' angles that the caller
+// adds to the FP64 sum. Far pairs on every lane of the wave (|det / den| < 1/8 and
 // den > la lb lc): the small-angle arctangent; otherwise the octant-reduced
 // atan2, and ill-conditioned pairs (max(|det|, |den|) < kappa la lb lc: a
 // query near an edge line, or close to the face inside its outline) are
@@ -121,6 +121,37 @@ __device__ __forceinline__ void face_step_f32c(const FaceRec32& rec, const float
   }
 }
 
+// Three faces of a tile for QPL queries: the FP32 sum of their angles, as
+// the argument of the product of their (den + i det) when every lane meets
+// sum3_x8's conditions (one reciprocal and one polynomial for the three);
+// otherwise face by face as face_step_f32c.
+template <int QPL>
+__device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const FaceRec32& rb,
+                                                 const FaceRec32& rc, const float (&ox)[QPL],
+                                                 const float (&oy)[QPL], const float (&oz)[QPL],
+                                                 float kappa, double (&acc)[QPL]) {
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const PairGeom32 g1 = pair_geom32(ra.r0, ra.r1, ra.r2, ra.r3, ox[k], oy[k], oz[k]);
+    const PairGeom32 g2 = pair_geom32(rb.r0, rb.r1, rb.r2, rb.r3, ox[k], oy[k], oz[k]);
+    const PairGeom32 g3 = pair_geom32(rc.r0, rc.r1, rc.r2, rc.r3, ox[k], oy[k], oz[k]);
+    const Sum3 s = sum3_x8(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
+    float part;
+    if (__builtin_expect(wave_all_sum3(g1.dp, g2.dp, g3.dp, s), 1)) {
+      part = atan_sum_x8_f32(s.x8);
+    } else {
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      float one[1] = {-0.0f};
+      const float qx[1] = {ox[k]}, qy[1] = {oy[k]}, qz[1] = {oz[k]};
+      face_step_f32c<1>(ra, qx, qy, qz, kappa, one);
+      face_step_f32c<1>(rb, qx, qy, qz, kappa, one);
+      face_step_f32c<1>(rc, qx, qy, qz, kappa, one);
+      part = one[0];
+    }
+    acc[k] += (double)part;
+  }
+}
+
 template <int QPL>
 __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64_t stride,
                                            int64_t f0, int64_t f1, double gx, double gy,
@@ -139,19 +170,13 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
                                      tri[7 * stride + f], tri[8 * stride + f], gx, gy, gz);
     }
     __syncthreads();
-    // the angles of two consecutive faces are summed in FP32 (each |angle|
-    // <= pi; one rounding, at most eps times their magnitudes, the bound of
-    // a compensated sum) and added to the FP64 sum once
+    // the angles of three consecutive faces are summed in FP32 (the
+    // argument of their product, or three FP32 angles added: at most a few
+    // ulps of their magnitudes, the bound of a compensated sum) and added to
+    // the FP64 sum once
     int j = 0;
-    for (; j + 2 <= cnt; j += 2) {
-      float part[QPL];
-#pragma unroll
-      for (int k = 0; k < QPL; ++k) part[k] = -0.0f;  // -0 + x == x: the first add folds away
-      face_step_f32c<QPL>(sm.face[j], ox, oy, oz, kappa, part);
-      face_step_f32c<QPL>(sm.face[j + 1], ox, oy, oz, kappa, part);
-#pragma unroll
-      for (int k = 0; k < QPL; ++k) acc[k] += (double)part[k];
-    }
+    for (; j + 3 <= cnt; j += 3)
+      face_triple_f32c<QPL>(sm.face[j], sm.face[j + 1], sm.face[j + 2], ox, oy, oz, kappa, acc);
     if (j < cnt) {
       float part[QPL];
 #pragma unroll

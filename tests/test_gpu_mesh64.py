@@ -17,15 +17,18 @@ torch = pytest.importorskip("torch")
 
 
 def _plain_fp64(V, F, Q):
-    """The plain FP64 face kernel through the reference's host entry
-    (igl.winding_number routes calls this large to the strip kernels)."""
+    """The plain FP64 face kernel (wn_winding_gpu_ex, precision 2: the
+    reference formula for every pair; mli_winding_gpu and igl.winding_number
+    route calls this large to the strip kernels themselves)."""
+    import igl
     from igl import pack_faces
     from igl._lib import addr, lib
 
     tri = pack_faces(np.asarray(V, dtype=np.float64), F)
     Qc = np.ascontiguousarray(Q, dtype=np.float64)
     out = np.empty(len(Qc))
-    assert lib().mli_winding_gpu(addr(tri), addr(Qc), addr(out), tri.shape[1], len(Qc)) == 1
+    assert lib().wn_winding_gpu_ex(addr(tri), addr(Qc), addr(out), tri.shape[1], len(Qc), 2, 0, 0,
+                                   0, -1.0) == 0, igl._lib.last_error()
     return out
 
 
