@@ -107,40 +107,84 @@ __global__ __launch_bounds__(kBlockThreads) void face_rec64_kernel(
   out[f] = r;
 }
 
-// The plain FP64 pair, out of line so that its registers are not reserved
-// for the whole sweep.
-__device__ __noinline__ double face64_pair(double ax, double ay, double az, double bx,
-                                           double by, double bz, double cx, double cy,
-                                           double cz) {
-  return pair_f64(ax, ay, az, bx, by, bz, cx, cy, cz);
+// The plain FP64 pair of face r and query o, out of line (offsets formed
+// inside: the sweep keeps no registers for them).
+__device__ __noinline__ double face64_pair(const FaceRec64* r, double ox, double oy, double oz) {
+  return pair_f64(r->ax - ox, r->ay - oy, r->az - oz, r->bx - ox, r->by - oy, r->bz - oz,
+                  r->cx - ox, r->cy - oy, r->cz - oz);
 }
 
-// One face for the lane's QPL queries: far pairs into acc, pairs of the
-// plain formula into acc_slow (two sums, each only ever updated in place).
+struct FaceGeom64 {
+  double det8, den, dp;
+};
+
+// det8 = 8 det, den and dp = den - la lb lc of face r for query o.
+__device__ __forceinline__ FaceGeom64 face64_geom(const FaceRec64& r, double ox, double oy,
+                                                  double oz) {
+  const double ax = r.ax - ox, ay = r.ay - oy, az = r.az - oz;
+  const double bx = r.bx - ox, by = r.by - oy, bz = r.bz - oz;
+  const double cx = r.cx - ox, cy = r.cy - oy, cz = r.cz - oz;
+  const double a2 = __builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax));
+  const double b2 = __builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx));
+  const double c2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
+  const double la = sqrt_gs_f64(a2), lb = sqrt_gs_f64(b2), lc = sqrt_gs_f64(c2);
+  const double s_ab = half_sum_plus(a2, b2, r.eab);  // a.b
+  const double s_bc = half_sum_plus(b2, c2, r.ebc);  // b.c
+  const double s_ca = half_sum_plus(c2, a2, r.eca);  // c.a
+  FaceGeom64 g;
+  g.det8 = __builtin_fma(r.nz, az, __builtin_fma(r.ny, ay, r.nx * ax));
+  g.dp = __builtin_fma(lc, s_ab, __builtin_fma(la, s_bc, lb * s_ca));
+  g.den = __builtin_fma(la * lb, lc, g.dp);
+  return g;
+}
+
+// One face on its own for one query: a far pair on every lane into acc, a
+// pair of the plain formula (offsets formed again from the record: the same
+// values) into acc_slow (two sums, each only ever updated in place).
+__device__ __forceinline__ void face64_single(const FaceRec64* r, const FaceGeom64& g, double ox,
+                                              double oy, double oz, double& acc,
+                                              double& acc_slow) {
+  if (__builtin_expect(wave_all_far_x8_f64(g.det8, g.den, g.dp), 1)) {
+    acc += atan_small_x8_f64(div_f64(g.det8, g.den));
+  } else {
+    asm volatile("" ::);  // keep a real wave-uniform branch
+    acc_slow += face64_pair(r, ox, oy, oz);
+  }
+}
+
+// One face for the lane's QPL queries.
 template <int QPL>
 __device__ __forceinline__ void face64_step(const FaceRec64& r, const double (&ox)[QPL],
                                             const double (&oy)[QPL], const double (&oz)[QPL],
                                             double (&acc)[QPL], double (&acc_slow)[QPL]) {
 #pragma unroll
+  for (int k = 0; k < QPL; ++k)
+    face64_single(&r, face64_geom(r, ox[k], oy[k], oz[k]), ox[k], oy[k], oz[k], acc[k],
+                  acc_slow[k]);
+}
+
+// Three faces for the lane's QPL queries: when every lane meets
+// sum3_x8_f64's conditions their angles are taken together as the argument
+// of the product of their (den + i det), one quotient and one polynomial for
+// the three; otherwise face by face.
+template <int QPL>
+__device__ __forceinline__ void face64_triple(const FaceRec64* __restrict__ r,
+                                              const double (&ox)[QPL], const double (&oy)[QPL],
+                                              const double (&oz)[QPL], double (&acc)[QPL],
+                                              double (&acc_slow)[QPL]) {
+#pragma unroll
   for (int k = 0; k < QPL; ++k) {
-    const double ax = r.ax - ox[k], ay = r.ay - oy[k], az = r.az - oz[k];
-    const double bx = r.bx - ox[k], by = r.by - oy[k], bz = r.bz - oz[k];
-    const double cx = r.cx - ox[k], cy = r.cy - oy[k], cz = r.cz - oz[k];
-    const double a2 = __builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax));
-    const double b2 = __builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx));
-    const double c2 = __builtin_fma(cz, cz, __builtin_fma(cy, cy, cx * cx));
-    const double la = sqrt_gs_f64(a2), lb = sqrt_gs_f64(b2), lc = sqrt_gs_f64(c2);
-    const double s_ab = half_sum_plus(a2, b2, r.eab);  // a.b
-    const double s_bc = half_sum_plus(b2, c2, r.ebc);  // b.c
-    const double s_ca = half_sum_plus(c2, a2, r.eca);  // c.a
-    const double det8 = __builtin_fma(r.nz, az, __builtin_fma(r.ny, ay, r.nx * ax));
-    const double dp = __builtin_fma(lc, s_ab, __builtin_fma(la, s_bc, lb * s_ca));
-    const double den = __builtin_fma(la * lb, lc, dp);
-    if (__builtin_expect(wave_all_far_x8_f64(det8, den, dp), 1)) {
-      acc[k] += atan_small_x8_f64(div_f64(det8, den));
+    const FaceGeom64 g1 = face64_geom(r[0], ox[k], oy[k], oz[k]);
+    const FaceGeom64 g2 = face64_geom(r[1], ox[k], oy[k], oz[k]);
+    const FaceGeom64 g3 = face64_geom(r[2], ox[k], oy[k], oz[k]);
+    const Sum3F64 s = sum3_x8_f64(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
+    if (__builtin_expect(wave_all_sum3_f64(g1.dp, g2.dp, g3.dp, s), 1)) {
+      acc[k] += atan_sum_x8_f64(s.x8);
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
-      acc_slow[k] += face64_pair(ax, ay, az, bx, by, bz, cx, cy, cz);
+      face64_single(r, g1, ox[k], oy[k], oz[k], acc[k], acc_slow[k]);
+      face64_single(r + 1, g2, ox[k], oy[k], oz[k], acc[k], acc_slow[k]);
+      face64_single(r + 2, g3, ox[k], oy[k], oz[k], acc[k], acc_slow[k]);
     }
   }
 }
@@ -168,7 +212,9 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
   for (int64_t c0 = f0; c0 < f1; c0 += (int64_t)1 << 30) {
     const FaceRec64* r = rec + c0;
     const int cnt = (int)(f1 - c0 < ((int64_t)1 << 30) ? f1 - c0 : ((int64_t)1 << 30));
-    for (int j = 0; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
+    int j = 0;
+    for (; j + 3 <= cnt; j += 3) face64_triple<QPL>(r + j, ox, oy, oz, acc, acc_slow);
+    for (; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
   }
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
@@ -227,7 +273,9 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
     }
     const FaceRec64* r = rec + f0;
     const int cnt = (int)(f1 - f0);  // a mesh has at most 2^31-1 faces
-    for (int j = 0; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
+    int j = 0;
+    for (; j + 3 <= cnt; j += 3) face64_triple<QPL>(r + j, ox, oy, oz, acc, acc_slow);
+    for (; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
       if (valid[k]) {

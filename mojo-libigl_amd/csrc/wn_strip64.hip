@@ -69,42 +69,61 @@ __device__ __noinline__ double strip64_pair(double ax, double ay, double az, dou
   return flip ? -v : v;
 }
 
-// One record: R takes vertex j; when the record closes face (P, Q, R) its
-// angle joins acc (far pairs) or acc_slow (pairs of the plain formula; two
-// sums, so that each is only ever updated in place). s_carry is the dot
-// product of the previous record's two newest vertices (Q, R of the last
-// step = P, Q here). Dot products come from the law of cosines,
-// b.c = (|b|^2 + |c|^2) / 2 - |b - c|^2 / 2, with -|b - c|^2 / 2 stored in the
-// record: one fma, and den = pq lc + dp needs no doubling.
-__device__ __forceinline__ void strip64_step(const StripRec64& r, long long flag_bits,
-                                             const Slot64& P, const Slot64& Q, Slot64& R,
-                                             double& s_carry, double& acc, double& acc_slow,
-                                             double ox, double oy, double oz) {
+// Geometry of record j: R takes its vertex, and for face (P, Q, R) det8 =
+// 8 det, den and dp = den - la lb lc (meaningful where the record closes a
+// face; a record that closes none, flag 0, has N = 0 and is never summed).
+// s_carry is the dot product of the previous record's two newest vertices
+// (Q, R of the last record = P, Q here). Dot products come from the law of
+// cosines, b.c = (|b|^2 + |c|^2) / 2 - |b - c|^2 / 2, with -|b - c|^2 / 2
+// stored in the record: one fma, and den = pq lc + dp needs no doubling.
+struct Geom64 {
+  double det8, den, dp;
+};
+
+__device__ __forceinline__ Geom64 strip64_geom(const StripRec64& r, const Slot64& P,
+                                               const Slot64& Q, Slot64& R, double& s_carry,
+                                               double ox, double oy, double oz) {
   R.ax = r.x - ox;
   R.ay = r.y - oy;
   R.az = r.z - oz;
   R.l2 = __builtin_fma(R.az, R.az, __builtin_fma(R.ay, R.ay, R.ax * R.ax));
   R.l = sqrt_gs_f64(R.l2);
   const double s_qr = half_sum_plus(Q.l2, R.l2, r.e1);  // b.c
-  if (flag_bits != 0) {
-    const double s_pr = half_sum_plus(P.l2, R.l2, r.e2);  // c.a
-    const double det8 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
-    const double pq = P.l * Q.l;
-    const double dp = __builtin_fma(R.l, s_carry, __builtin_fma(P.l, s_qr, Q.l * s_pr));
-    const double den = __builtin_fma(pq, R.l, dp);
-    if (__builtin_expect(wave_all_far_x8_f64(det8, den, dp), 1)) {
-      acc += atan_small_x8_f64(div_f64(det8, den));
-    } else {
-      asm volatile("" ::);  // keep a real wave-uniform branch
-      // flag -1.0 has the sign bit set: the face runs against the strip order
-      acc_slow +=
-          strip64_pair(P.ax, P.ay, P.az, Q.ax, Q.ay, Q.az, R.ax, R.ay, R.az, flag_bits < 0);
-    }
-  }
+  const double s_pr = half_sum_plus(P.l2, R.l2, r.e2);  // c.a
+  Geom64 g;
+  g.det8 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
+  const double pq = P.l * Q.l;
+  g.dp = __builtin_fma(R.l, s_carry, __builtin_fma(P.l, s_qr, Q.l * s_pr));
+  g.den = __builtin_fma(pq, R.l, g.dp);
   s_carry = s_qr;
+  return g;
+}
+
+// Face j (closed by a record with flag_bits != 0) on its own: far on every
+// lane (|det / den| < 1/8, den > la lb lc), the 7-term polynomial into acc;
+// otherwise the plain formula into acc_slow (two sums, so that each is only
+// ever updated in place), from the corners' offsets rebuilt from records
+// j-2, j-1, j of the tile (the values the register slots held).
+__device__ __forceinline__ void strip64_face(const StripRec64* __restrict__ r, int j,
+                                             long long flag_bits, const Geom64& g, double& acc,
+                                             double& acc_slow, double ox, double oy, double oz) {
+  if (__builtin_expect(wave_all_far_x8_f64(g.det8, g.den, g.dp), 1)) {
+    acc += atan_small_x8_f64(div_f64(g.det8, g.den));
+  } else {
+    asm volatile("" ::);  // keep a real wave-uniform branch
+    const StripRec64 &a = r[j - 2], &b = r[j - 1], &c = r[j];
+    // flag -1.0 has the sign bit set: the face runs against the strip order
+    acc_slow += strip64_pair(a.x - ox, a.y - oy, a.z - oz, b.x - ox, b.y - oy, b.z - oz,
+                             c.x - ox, c.y - oy, c.z - oz, flag_bits < 0);
+  }
 }
 
 // Unscaled FP64 angle sum of one query over strip tiles [t_begin, t_end).
+// Records go in groups of three (a tile's 255 = 85 x 3, so the register
+// slots rotate statically). When all three close a face (flags: scalar) and
+// every lane meets sum3_x8_f64's conditions, the group's angle is the
+// argument of the product of its faces' (den + i det): one quotient and one
+// 11-term polynomial for the three; otherwise each face on its own.
 __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ rec64,
                                                 const double4* __restrict__ rec, int64_t t_begin,
                                                 int64_t t_end, double ox, double oy, double oz) {
@@ -114,9 +133,21 @@ __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ r
     const StripRec64* r = rec64 + t * kStripTileRecords;
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     for (int j = 0; j < kStripTileRecords; j += 3) {
-      strip64_step(r[j], flags[4 * j], A, B, C, s_carry, acc, acc_slow, ox, oy, oz);
-      strip64_step(r[j + 1], flags[4 * (j + 1)], B, C, A, s_carry, acc, acc_slow, ox, oy, oz);
-      strip64_step(r[j + 2], flags[4 * (j + 2)], C, A, B, s_carry, acc, acc_slow, ox, oy, oz);
+      const long long f1 = flags[4 * j], f2 = flags[4 * (j + 1)], f3 = flags[4 * (j + 2)];
+      const Geom64 g1 = strip64_geom(r[j], A, B, C, s_carry, ox, oy, oz);
+      const Geom64 g2 = strip64_geom(r[j + 1], B, C, A, s_carry, ox, oy, oz);
+      const Geom64 g3 = strip64_geom(r[j + 2], C, A, B, s_carry, ox, oy, oz);
+      if (__builtin_expect(f1 != 0 && f2 != 0 && f3 != 0, 1)) {
+        const Sum3F64 s = sum3_x8_f64(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
+        if (__builtin_expect(wave_all_sum3_f64(g1.dp, g2.dp, g3.dp, s), 1)) {
+          acc += atan_sum_x8_f64(s.x8);
+          continue;
+        }
+      }
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      if (f1 != 0) strip64_face(r, j, f1, g1, acc, acc_slow, ox, oy, oz);
+      if (f2 != 0) strip64_face(r, j + 1, f2, g2, acc, acc_slow, ox, oy, oz);
+      if (f3 != 0) strip64_face(r, j + 2, f3, g3, acc, acc_slow, ox, oy, oz);
     }
   }
   return acc + acc_slow;
