@@ -71,14 +71,13 @@ __device__ __forceinline__ double pair_f64(double ax, double ay, double az,
 // ---------------------------------------------------------------------------
 // FP32-compensated face record, 64 bytes, staged in LDS per block.
 // Coordinates are relative to the block's local origin (its query bbox
-// centre), rounded once to FP32; N8 = 8 (B'-A') x (C'-A') and the halved
+// centre), rounded once to FP32; N = (B'-A') x (C'-A') and the halved
 // squared edge lengths are computed in FP64 from those same rounded corners,
 // so every quantity the pair uses describes one consistent (rounded)
-// triangle. The scalings are powers of two, exact: the lane's dot product
-// with N8 is 8 det, and a law-of-cosines dot product is one add and one fma
-// with 0.5.
+// triangle. The lane's dot product with N is det, and a law-of-cosines dot
+// product is one add and one fma with 0.5 (the halving is exact).
 //   r0 = (A.x, A.y, A.z, B.x)   r1 = (B.y, B.z, C.x, C.y)
-//   r2 = (C.z, N8.x, N8.y, N8.z) r3 = (-|B-A|^2/2, -|C-B|^2/2, -|A-C|^2/2, 0)
+//   r2 = (C.z, N.x, N.y, N.z)   r3 = (-|B-A|^2/2, -|C-B|^2/2, -|A-C|^2/2, 0)
 // ---------------------------------------------------------------------------
 struct FaceRec32 {
   float4 r0, r1, r2, r3;
@@ -98,9 +97,9 @@ __device__ __forceinline__ FaceRec32 make_face_rec32(double Ax, double Ay, doubl
                e2z = (double)cz - (double)az;
   const double e3x = (double)cx - (double)bx, e3y = (double)cy - (double)by,
                e3z = (double)cz - (double)bz;
-  const double nx = 8.0 * (e1y * e2z - e1z * e2y);
-  const double ny = 8.0 * (e1z * e2x - e1x * e2z);
-  const double nz = 8.0 * (e1x * e2y - e1y * e2x);
+  const double nx = e1y * e2z - e1z * e2y;
+  const double ny = e1z * e2x - e1x * e2z;
+  const double nz = e1x * e2y - e1y * e2x;
   FaceRec32 r;
   r.r0 = make_float4(ax, ay, az, bx);
   r.r1 = make_float4(by, bz, cx, cy);
@@ -143,58 +142,58 @@ __device__ __forceinline__ float atan_small_x8_f32(float x) {
   return x * p;
 }
 
-// atan(x / 8) for |x| <= 3.2 (|x / 8| <= 0.4): r * Q(r^2), Q of degree 4 in
-// r^2, relative-error minimax on (0, 0.4] (max 1.30e-8, gen_atan_coeffs.py,
-// n = 5), with its coefficients scaled by powers of two so that it takes
-// x = 8 r. The angle of a product of several far faces' (den + i det).
-__device__ __forceinline__ float atan_sum_x8_f32(float x) {
+// atan(x) for |x| <= 0.4: x * Q(x^2), Q of degree 4 in x^2, relative-error
+// minimax on (0, 0.4] (max 1.30e-8, gen_atan_coeffs.py, n = 5). The angle of
+// a product of three far faces' (den + i det) (sum3_f32).
+__device__ __forceinline__ float atan_sum_f32(float x) {
   const float t = x * x;
-  float p = 0.08156783959325319f * (1.0f / 134217728.0f);  // x^9: 8^9
-  p = __builtin_fmaf(p, t, -0.13901368582940957f * (1.0f / 2097152.0f));  // x^7: 8^7
-  p = __builtin_fmaf(p, t, 0.19978842639359343f * (1.0f / 32768.0f));    // x^5: 8^5
-  p = __builtin_fmaf(p, t, -0.33332919866326131f * (1.0f / 512.0f));     // x^3: 8^3
-  p = __builtin_fmaf(p, t, 0.99999998698746807f * 0.125f);               // x^1: 8
+  float p = 0.08156783959325319f;
+  p = __builtin_fmaf(p, t, -0.13901368582940957f);
+  p = __builtin_fmaf(p, t, 0.19978842639359343f);
+  p = __builtin_fmaf(p, t, -0.33332919866326131f);
+  p = __builtin_fmaf(p, t, 0.99999998698746807f);
   return x * p;
 }
 
 // The angles of three faces, summed as the argument of the product of
-// their (den + i det): from det8 = 8 det and den of each,
-//   z12 = w1 w2:  re12 = d1 d2 - det8_1 det8_2 / 64,  8 im12 = d1 det8_2 + det8_1 d2
-//   z = z12 w3:   re = re12 d3 - 8 im12 det8_3 / 64,   8 im = re12 det8_3 + 8 im12 d3
-// and x8 = 8 im / re = 8 tan(angle_1 + angle_2 + angle_3). Valid (no wrap,
-// x8 in the polynomial's range, no overflow or underflow) when every face
-// has den > la lb lc (each angle within +-pi/2, so the sum within +-3pi/2:
-// the product's argument is the sum itself once it lies within +-0.39),
-// re is a positive normal number and |x8| < 3.2 (tan of the sum < 0.4);
-// wave_all_sum3 tests that. The absolute error of the argument is that of
-// adding the three FP32 angles: a few ulps of |angle_1| + |angle_2| +
-// |angle_3|.
+// their w_k = den_k + i det_k:
+//   w1 w2 = re12 + i im12:  re12 = d1 d2 - e1 e2,     im12 = d1 e2 + e1 d2
+//   w1 w2 w3 = re + i im:   re = re12 d3 - im12 e3,   im = re12 e3 + im12 d3
+// and x = im / re = tan(angle_1 + angle_2 + angle_3). Valid (no wrap, x in
+// the polynomial's range, no overflow or underflow) when every face has
+// den > la lb lc (each angle within +-pi/2, so the sum within +-3pi/2: the
+// product's argument is the sum itself once it lies within +-0.39), re is
+// a positive normal number and |x| < 0.4; wave_all_sum3 tests that. The
+// absolute error of the argument is that of adding the three FP32 angles:
+// a few ulps of |angle_1| + |angle_2| + |angle_3|.
 struct Sum3 {
-  float re, x8;
+  float re, x;
 };
 
-__device__ __forceinline__ Sum3 sum3_x8(float e1, float d1, float e2, float d2, float e3,
-                                        float d3) {
-  const float re12 = __builtin_fmaf(e1 * e2, -1.0f / 64.0f, d1 * d2);
-  const float j12 = __builtin_fmaf(d1, e2, e1 * d2);
+__device__ __forceinline__ Sum3 sum3_f32(float e1, float d1, float e2, float d2, float e3,
+                                         float d3) {
+  const float re12 = __builtin_fmaf(-e1, e2, d1 * d2);
+  const float im12 = __builtin_fmaf(d1, e2, e1 * d2);
   Sum3 s;
-  s.re = __builtin_fmaf(j12 * e3, -1.0f / 64.0f, re12 * d3);
-  s.x8 = __builtin_fmaf(re12, e3, j12 * d3) * __builtin_amdgcn_rcpf(s.re);
+  s.re = __builtin_fmaf(-im12, e3, re12 * d3);
+  s.x = __builtin_fmaf(re12, e3, im12 * d3) * __builtin_amdgcn_rcpf(s.re);
   return s;
 }
 
-// Every active lane meets sum3_x8's conditions (dp_k = den_k - la lb lc).
+// Every active lane meets sum3_f32's conditions (dp_k = den_k - la lb lc).
+// The three dp tests are one min3 and one compare: a NaN dp_k, which the
+// min would skip, comes with a NaN den_k and so a NaN re, which the class
+// test rejects.
 __device__ __forceinline__ bool wave_all_sum3(float dp1, float dp2, float dp3, const Sum3& s) {
-  const uint64_t m1 = __builtin_amdgcn_ballot_w64(dp1 > 0.0f);
-  const uint64_t m2 = __builtin_amdgcn_ballot_w64(dp2 > 0.0f);
-  const uint64_t m3 = __builtin_amdgcn_ballot_w64(dp3 > 0.0f);
+  const uint64_t m1 =
+      __builtin_amdgcn_ballot_w64(__builtin_fminf(__builtin_fminf(dp1, dp2), dp3) > 0.0f);
   // re positive normal (not zero, denormal, infinite or NaN): one
   // v_cmp_class_f32 writing the lane mask (the class builtin goes through a
   // VGPR select and a second compare before its ballot)
-  uint64_t m4;
-  asm volatile("v_cmp_class_f32_e64 %0, %1, %2" : "=s"(m4) : "v"(s.re), "s"(1 << 8));
-  const uint64_t m5 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x8) < 3.2f);
-  return (m1 & m2 & m3 & m4 & m5) == __builtin_amdgcn_read_exec();
+  uint64_t m2;
+  asm volatile("v_cmp_class_f32_e64 %0, %1, %2" : "=s"(m2) : "v"(s.re), "s"(1 << 8));
+  const uint64_t m3 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
+  return (m1 & m2 & m3) == __builtin_amdgcn_read_exec();
 }
 
 // Geometry of one pair in FP32 from a record. Algebra (exact identities,
@@ -202,10 +201,10 @@ __device__ __forceinline__ bool wave_all_sum3(float dp1, float dp2, float dp3, c
 //   det = N . a              with N = (B-A) x (C-A)  (det(a,b,c) = det(a,b-a,c-a))
 //   a.b = (la^2 + lb^2)/2 - |B-A|^2/2   (law of cosines), likewise b.c, c.a
 //   den = la lb lc + dp,  dp = lc a.b + la b.c + lb c.a
-// det8 = 8 det (the record's normal is 8 N); pql = la lb lc, the scale of the
-// conditioning test. A distant face has den -> 4 la lb lc.
+// pql = la lb lc, the scale of the conditioning test. A distant face has
+// den -> 4 la lb lc.
 struct PairGeom32 {
-  float det8, den, dp, pql;
+  float det, den, dp, pql;
 };
 
 __device__ __forceinline__ PairGeom32 pair_geom32(const float4 r0, const float4 r1,
@@ -221,7 +220,7 @@ __device__ __forceinline__ PairGeom32 pair_geom32(const float4 r0, const float4 
   const float lb = __builtin_amdgcn_sqrtf(lb2);
   const float lc = __builtin_amdgcn_sqrtf(lc2);
   PairGeom32 g;
-  g.det8 = __builtin_fmaf(r2.w, az, __builtin_fmaf(r2.z, ay, r2.y * ax));
+  g.det = __builtin_fmaf(r2.w, az, __builtin_fmaf(r2.z, ay, r2.y * ax));
   const float ab = __builtin_fmaf(la2 + lb2, 0.5f, r3.x);
   const float bc = __builtin_fmaf(lb2 + lc2, 0.5f, r3.y);
   const float ca = __builtin_fmaf(lc2 + la2, 0.5f, r3.z);

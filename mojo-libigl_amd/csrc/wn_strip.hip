@@ -17,21 +17,19 @@
 // LDS tile (255 records, a multiple of 3 so the three register slots rotate
 // statically through a 3-way unrolled loop):
 //   R0[j] = (w.x, w.y, w.z, -|w_j - w_{j-1}|^2 / 2)  (block-local FP32 coords)
-//   R1[j] = (-|w_j - w_{j-2}|^2 / 2, N8.x, N8.y, N8.z)  (N8 = 8 x face normal)
-// so that a dot product is one add and one fma with 0.5 (law of cosines),
-// den is formed undoubled, and det8 = 8 det needs no multiply for the far
-// test |det| < den / 8 (all scalings powers of two: the values are those of
-// the doubled forms, scaled exactly).
+//   R1[j] = (-|w_j - w_{j-2}|^2 / 2, N.x, N.y, N.z)  (N = face normal)
+// so that a dot product is one add and one fma with 0.5 (law of cosines;
+// the halving is exact) and den is formed undoubled.
 // The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
-// orientation vs strip order) is folded into N8; a record with flag 0 is
-// staged with N8 = 0 and R1.x = +2^30, which makes its angle 0 without a
+// orientation vs strip order) is folded into N; a record with flag 0 is
+// staged with N = 0 and R1.x = +2^30, which makes its angle 0 without a
 // branch. The flag itself is read (from the records, at a wave-uniform
 // address) only on the rare slow path.
 // Fast path: for each group of three records, the sum of their angles is the
-// argument of the product of the faces' (den + i det) (sum3_x8): one
+// argument of the product of the faces' (den + i det) (sum3_f32): one
 // reciprocal and one 5-term polynomial per three faces instead of three of
 // each.
-// Squared lengths and N2 are computed in FP64 from the rounded FP32 corners.
+// Squared lengths and N are computed in FP64 from the rounded FP32 corners.
 
 #include "wn_device.h"
 #include "wn_internal.h"
@@ -66,18 +64,18 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   const double d1x = (double)cx - px, d1y = (double)cy - py, d1z = (double)cz - pz;
   const double d2x = (double)cx - qx, d2y = (double)cy - qy, d2z = (double)cz - qz;
   const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
-  // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag; stored
-  // as 8 N so that the lane's dot product is 8 det
+  // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag: the
+  // lane's dot product with it is det
   // A record that closes no face (flag 0) gets N = 0 and +2^30 in place of
-  // -|w_k - w_{k-2}|^2 / 2: det8 is then 0 exactly and c.a ~ 2^30 makes dp
+  // -|w_k - w_{k-2}|^2 / 2: det is then 0 exactly and c.a ~ 2^30 makes dp
   // and den positive and finite (see strip_record), so both fast paths give
   // it angle 0 (0 / den, and a real factor den + 0i of the triple product).
-  const double f8 = 8.0 * c.w;
+  const double f = c.w;
   sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
   sm.r1[j] = make_float4(c.w == 0.0 ? 1073741824.0f
                                     : (float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
-                         (float)(f8 * (ey * d2z - ez * d2y)), (float)(f8 * (ez * d2x - ex * d2z)),
-                         (float)(f8 * (ex * d2y - ey * d2x)));
+                         (float)(f * (ey * d2z - ez * d2y)), (float)(f * (ez * d2x - ex * d2z)),
+                         (float)(f * (ex * d2y - ey * d2x)));
 }
 
 // FP64 re-evaluation of face (j-2, j-1, j) from the LDS corners, oriented.
@@ -104,10 +102,10 @@ __device__ __forceinline__ void strip_vertex(const float4& r0, float ox, float o
 
 // Face (a, b, c) = (P, Q, R) closed by a record with tile data r0, r1; a.b
 // = s_carry (carried from the previous record), b.c = s_qr (carried to the
-// next), c.a from r1.x (law of cosines). det8 = 8 det (the normal is stored
-// scaled by 8); den = la lb lc + dp with dp = lc a.b + la b.c + lb c.a.
+// next), c.a from r1.x (law of cosines); den = la lb lc + dp with
+// dp = lc a.b + la b.c + lb c.a.
 struct StripGeom {
-  float det8, den, dp, pq, s_qr;
+  float det, den, dp, pq, s_qr;
 };
 
 __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& r1, const VSlot& P,
@@ -115,7 +113,7 @@ __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& 
   StripGeom g;
   g.s_qr = __builtin_fmaf(Q.l2 + R.l2, 0.5f, r0.w);
   const float s_pr = __builtin_fmaf(P.l2 + R.l2, 0.5f, r1.x);
-  g.det8 = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
+  g.det = __builtin_fmaf(r1.w, R.az, __builtin_fmaf(r1.z, R.ay, r1.y * R.ax));
   g.pq = P.l * Q.l;
   g.dp = __builtin_fmaf(R.l, s_carry, __builtin_fmaf(P.l, g.s_qr, Q.l * s_pr));
   g.den = __builtin_fmaf(g.pq, R.l, g.dp);
@@ -125,8 +123,8 @@ __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& 
 // Record j of the tile closes face (P, Q, R) (P and Q from the two previous
 // records, R computed here): its FP32 geometry. Every record takes the same
 // instructions: one that closes no face (flag 0: the first two of a strip,
-// tile padding) was staged (stage_strip_tile) with N8 = 0 and c.a = +2^30,
-// so det8 = 0 exactly and, with Q's length above ~1e-9 of the mesh's scale,
+// tile padding) was staged (stage_strip_tile) with N = 0 and c.a = +2^30,
+// so det = 0 exactly and, with Q's length above ~1e-9 of the mesh's scale,
 // dp and den are positive: its angle is 0 on both fast paths.
 // boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
 __device__ __forceinline__ StripGeom strip_record(const StripSmem& sm, int boff, const VSlot& P,
@@ -150,15 +148,16 @@ __device__ __forceinline__ float strip_angle(const StripSmem& sm,
                                              const long long* __restrict__ flags, int j,
                                              const StripGeom& g, float rl, float ox, float oy,
                                              float oz, float kappa) {
-  float ang = atan_small_x8_f32(g.det8 * __builtin_amdgcn_rcpf(g.den));
-  if (__builtin_expect(!wave_all_far_x8(g.det8, g.den, g.dp), 0)) {
+  const float det8 = 8.0f * g.det;  // exact
+  float ang = atan_small_x8_f32(det8 * __builtin_amdgcn_rcpf(g.den));
+  if (__builtin_expect(!wave_all_far_x8(det8, g.den, g.dp), 0)) {
     asm volatile("" ::);  // keep a real wave-uniform branch
     const long long flag_bits = flags[4 * j];
     if (flag_bits == 0) {
       ang = 0.0f;
     } else {
       float mx;
-      ang = atan2_f32(0.125f * g.det8, g.den, mx);
+      ang = atan2_f32(g.det, g.den, mx);
       // ill-conditioned (mx < kappa x la lb lc): the FP64 angle, rounded
       // once, replaces the FP32 one. Flag -1.0 has the sign bit set: the
       // face runs against the strip order.
@@ -170,7 +169,7 @@ __device__ __forceinline__ float strip_angle(const StripSmem& sm,
 }
 
 // Records j, j+1, j+2 (slots A, B, C rotating as P, Q, R): the FP32 sum of
-// their three angles. When every lane meets sum3_x8's conditions the sum is
+// their three angles. When every lane meets sum3_f32's conditions the sum is
 // the argument of the product of the faces' (den + i det), one reciprocal
 // and one polynomial for the three faces; otherwise each face takes
 // strip_angle and the three FP32 angles are added (-0 + x == x: the first
@@ -184,8 +183,8 @@ __device__ __forceinline__ float strip_triple(const StripSmem& sm,
   const StripGeom g1 = strip_record(sm, boff, A, B, C, s_carry, ox, oy, oz);
   const StripGeom g2 = strip_record(sm, boff + 16, B, C, A, s_carry, ox, oy, oz);
   const StripGeom g3 = strip_record(sm, boff + 32, C, A, B, s_carry, ox, oy, oz);
-  const Sum3 s = sum3_x8(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
-  if (__builtin_expect(wave_all_sum3(g1.dp, g2.dp, g3.dp, s), 1)) return atan_sum_x8_f32(s.x8);
+  const Sum3 s = sum3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+  if (__builtin_expect(wave_all_sum3(g1.dp, g2.dp, g3.dp, s), 1)) return atan_sum_f32(s.x);
   asm volatile("" ::);  // keep a real wave-uniform branch
   float part = -0.0f;
   part += strip_angle(sm, flags, j, g1, C.l, ox, oy, oz, kappa);

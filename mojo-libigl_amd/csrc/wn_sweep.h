@@ -107,14 +107,15 @@ __device__ __forceinline__ void face_step_f32c(const FaceRec32& rec, const float
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
     const PairGeom32 g = pair_geom32(r0, r1, r2, r3, ox[k], oy[k], oz[k]);
-    float ang = atan_small_x8_f32(g.det8 * __builtin_amdgcn_rcpf(g.den));
-    if (__builtin_expect(!wave_all_far_x8(g.det8, g.den, g.dp), 0)) {
+    const float det8 = 8.0f * g.det;  // exact
+    float ang = atan_small_x8_f32(det8 * __builtin_amdgcn_rcpf(g.den));
+    if (__builtin_expect(!wave_all_far_x8(det8, g.den, g.dp), 0)) {
       // An empty volatile asm gives this path a side effect, so the
       // compiler keeps a real (wave-uniform) branch instead of computing
       // both paths and selecting.
       asm volatile("" ::);
       float mx;
-      ang = atan2_f32(0.125f * g.det8, g.den, mx);
+      ang = atan2_f32(g.det, g.den, mx);
       if (mx < kappa * g.pql) ang = (float)pair_rec32_f64(r0, r1, r2, ox[k], oy[k], oz[k]);
     }
     part[k] += ang;
@@ -123,7 +124,7 @@ __device__ __forceinline__ void face_step_f32c(const FaceRec32& rec, const float
 
 // Three faces of a tile for QPL queries: the FP32 sum of their angles, as
 // the argument of the product of their (den + i det) when every lane meets
-// sum3_x8's conditions (one reciprocal and one polynomial for the three);
+// sum3_f32's conditions (one reciprocal and one polynomial for the three);
 // otherwise face by face as face_step_f32c.
 template <int QPL>
 __device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const FaceRec32& rb,
@@ -135,10 +136,10 @@ __device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const Face
     const PairGeom32 g1 = pair_geom32(ra.r0, ra.r1, ra.r2, ra.r3, ox[k], oy[k], oz[k]);
     const PairGeom32 g2 = pair_geom32(rb.r0, rb.r1, rb.r2, rb.r3, ox[k], oy[k], oz[k]);
     const PairGeom32 g3 = pair_geom32(rc.r0, rc.r1, rc.r2, rc.r3, ox[k], oy[k], oz[k]);
-    const Sum3 s = sum3_x8(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
+    const Sum3 s = sum3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
     float part;
     if (__builtin_expect(wave_all_sum3(g1.dp, g2.dp, g3.dp, s), 1)) {
-      part = atan_sum_x8_f32(s.x8);
+      part = atan_sum_f32(s.x);
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       float one[1] = {-0.0f};

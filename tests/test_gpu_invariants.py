@@ -4,7 +4,9 @@ against the oracle evaluated on the same transformed inputs:
 * translation far from the origin (1e4 .. 1e6): the FP32-compensated paths
   keep their bar only because each block works in FP32 offsets from its own
   origin; plain FP32 coordinates at 1e6 would be off by ~0.06;
-* uniform scale (1e-3, 1e3), a rotation, a face permutation, a cyclic
+* uniform scale (1e-5, 1e-3, 1e3, 1e6: the FP32 product of three faces'
+  den underflows or overflows at the ends, so the fast paths' per-face
+  fallback is exercised), a rotation, a face permutation, a cyclic
   permutation of every face's corners (w unchanged), and a swap of two
   corners of every face (w negated).
 
@@ -62,6 +64,11 @@ def transforms():
         "translate_1e6": (V + 1e6, F, Q + 1e6, 1.0),
         "scale_1e-3": (V * 1e-3, F, Q * 1e-3, 1.0),
         "scale_1e3": (V * 1e3, F, Q * 1e3, 1.0),
+        # the FP32c fast paths multiply three faces' den (~ size^3 each):
+        # the product underflows at 1e-5 and overflows at 1e6, and those
+        # groups must fall back to one face at a time
+        "scale_1e-5": (V * 1e-5, F, Q * 1e-5, 1.0),
+        "scale_1e6": (V * 1e6, F, Q * 1e6, 1.0),
         "rotate": (V @ R.T, F, Q @ R.T, 1.0),
         "permute_faces": (V, F[perm], Q, 1.0),
         "cycle_corners": (V, F[:, [1, 2, 0]], Q, 1.0),
