@@ -74,11 +74,10 @@ __device__ __forceinline__ bool wave_all_far_x8_f64(double det8, double den, dou
   return (m1 & m2) == __builtin_amdgcn_read_exec();
 }
 
-// atan(x / 8) for |x| <= 3.2 (|x / 8| <= 0.4): x * P(x^2) with the n = 11
-// relative-error minimax coefficients of atan on (0, 0.4] (max 1.6e-17,
-// gen_atan_coeffs.py), each divided by 8^(2k+1) (powers of two: exact). The
-// angle of the product of three far faces' (den + i det) (sum3_x8_f64).
-__device__ __forceinline__ double atan_sum_x8_f64(double x) {
+// atan(x) for |x| <= 0.4: x * P(x^2) with the n = 11 relative-error minimax
+// coefficients of atan on (0, 0.4] (max 1.6e-17, gen_atan_coeffs.py). The
+// angle of the product of three far faces' (den + i det) (sum3_f64).
+__device__ __forceinline__ double atan_sum_f64(double x) {
   // one asm block of VOP3 v_fma_f64 with the coefficients in scalar
   // registers, as atan_small_x8_f64 (from C they are kept in VGPRs)
   const double t = x * x;
@@ -94,37 +93,37 @@ __device__ __forceinline__ double atan_sum_x8_f64(double x) {
       "v_fma_f64 %0, %0, %1, %11\n\t"
       "v_fma_f64 %0, %0, %1, %12"
       : "=&v"(p)
-      : "v"(t), "v"(0.02229460232451987210308 / 0x1p63),  // x^21: 8^21
-        "s"(-0.04439518713060166475412 / 0x1p57),           // x^19
-        "s"(0.0571895757608583000099 / 0x1p51),             // x^17
-        "s"(-0.06645891376788731639366 / 0x1p45),           // x^15
-        "s"(0.07690583722896371814793 / 0x1p39),            // x^13
-        "s"(-0.09090816394011522252152 / 0x1p33),           // x^11
-        "s"(0.1111110798684710014473 / 0x1p27),             // x^9
-        "s"(-0.1428571422406846551985 / 0x1p21),            // x^7
-        "s"(0.1999999999937472279326 / 0x1p15),             // x^5
-        "s"(-0.333333333333308557907 / 0x1p9),              // x^3
-        "s"(0.999999999999999983756 / 0x1p3));              // x^1: 8
+      : "v"(t), "v"(0.02229460232451987210308),  // x^21
+        "s"(-0.04439518713060166475412),          // x^19
+        "s"(0.0571895757608583000099),            // x^17
+        "s"(-0.06645891376788731639366),          // x^15
+        "s"(0.07690583722896371814793),           // x^13
+        "s"(-0.09090816394011522252152),          // x^11
+        "s"(0.1111110798684710014473),            // x^9
+        "s"(-0.1428571422406846551985),           // x^7
+        "s"(0.1999999999937472279326),            // x^5
+        "s"(-0.333333333333308557907),            // x^3
+        "s"(0.999999999999999983756));            // x^1
   return x * p;
 }
 
-// The FP64 counterpart of sum3_x8 (wn_device.h): from det8 = 8 det and den
-// of three faces, re = Re(w1 w2 w3) and x8 = 8 Im / Re = 8 tan(angle_1 +
-// angle_2 + angle_3) for w_k = den_k + i det_k; valid when every face has
-// dp_k = den_k - la lb lc > 0, re is a positive normal number and |x8| < 3.2
+// The FP64 counterpart of sum3_f32 (wn_device.h): from det and den of three
+// faces, re = Re(w1 w2 w3) and x = Im / Re = tan(angle_1 + angle_2 +
+// angle_3) for w_k = den_k + i det_k; valid when every face has dp_k =
+// den_k - la lb lc > 0, re is a positive normal number and |x| < 0.4
 // (wave_all_sum3_f64).
 struct Sum3F64 {
-  double re, x8;
+  double re, x;
 };
 
-__device__ __forceinline__ Sum3F64 sum3_x8_f64(double e1, double d1, double e2, double d2,
-                                              double e3, double d3) {
-  const double re12 = __builtin_fma(e1 * e2, -1.0 / 64.0, d1 * d2);
-  const double j12 = __builtin_fma(d1, e2, e1 * d2);
+__device__ __forceinline__ Sum3F64 sum3_f64(double e1, double d1, double e2, double d2,
+                                           double e3, double d3) {
+  const double re12 = __builtin_fma(-e1, e2, d1 * d2);
+  const double im12 = __builtin_fma(d1, e2, e1 * d2);
   Sum3F64 s;
-  s.re = __builtin_fma(j12 * e3, -1.0 / 64.0, re12 * d3);
+  s.re = __builtin_fma(-im12, e3, re12 * d3);
   // (the quotient is used only where re is a positive normal number)
-  s.x8 = div_f64(__builtin_fma(re12, e3, j12 * d3), s.re);
+  s.x = div_f64(__builtin_fma(re12, e3, im12 * d3), s.re);
   return s;
 }
 
@@ -135,7 +134,7 @@ __device__ __forceinline__ bool wave_all_sum3_f64(double dp1, double dp2, double
   const uint64_t m3 = __builtin_amdgcn_ballot_w64(dp3 > 0.0);
   uint64_t m4;  // re positive normal: one v_cmp_class_f64 writing the lane mask
   asm volatile("v_cmp_class_f64_e64 %0, %1, %2" : "=s"(m4) : "v"(s.re), "s"(1 << 8));
-  const uint64_t m5 = __builtin_amdgcn_ballot_w64(__builtin_fabs(s.x8) < 3.2);
+  const uint64_t m5 = __builtin_amdgcn_ballot_w64(__builtin_fabs(s.x) < 0.4);
   return (m1 & m2 & m3 & m4 & m5) == __builtin_amdgcn_read_exec();
 }
 

@@ -97,9 +97,9 @@ __global__ __launch_bounds__(kBlockThreads) void face_rec64_kernel(
   const double e1x = r.bx - r.ax, e1y = r.by - r.ay, e1z = r.bz - r.az;  // B - A
   const double e2x = r.cx - r.ax, e2y = r.cy - r.ay, e2z = r.cz - r.az;  // C - A
   const double e3x = r.cx - r.bx, e3y = r.cy - r.by, e3z = r.cz - r.bz;  // C - B
-  r.nx = 8.0 * (e1y * e2z - e1z * e2y);
-  r.ny = 8.0 * (e1z * e2x - e1x * e2z);
-  r.nz = 8.0 * (e1x * e2y - e1y * e2x);
+  r.nx = e1y * e2z - e1z * e2y;
+  r.ny = e1z * e2x - e1x * e2z;
+  r.nz = e1x * e2y - e1y * e2x;
   r.eab = -0.5 * (e1x * e1x + e1y * e1y + e1z * e1z);
   r.ebc = -0.5 * (e3x * e3x + e3y * e3y + e3z * e3z);
   r.eca = -0.5 * (e2x * e2x + e2y * e2y + e2z * e2z);
@@ -115,10 +115,10 @@ __device__ __noinline__ double face64_pair(const FaceRec64* r, double ox, double
 }
 
 struct FaceGeom64 {
-  double det8, den, dp;
+  double det, den, dp;
 };
 
-// det8 = 8 det, den and dp = den - la lb lc of face r for query o.
+// det, den and dp = den - la lb lc of face r for query o.
 __device__ __forceinline__ FaceGeom64 face64_geom(const FaceRec64& r, double ox, double oy,
                                                   double oz) {
   const double ax = r.ax - ox, ay = r.ay - oy, az = r.az - oz;
@@ -132,7 +132,7 @@ __device__ __forceinline__ FaceGeom64 face64_geom(const FaceRec64& r, double ox,
   const double s_bc = half_sum_plus(b2, c2, r.ebc);  // b.c
   const double s_ca = half_sum_plus(c2, a2, r.eca);  // c.a
   FaceGeom64 g;
-  g.det8 = __builtin_fma(r.nz, az, __builtin_fma(r.ny, ay, r.nx * ax));
+  g.det = __builtin_fma(r.nz, az, __builtin_fma(r.ny, ay, r.nx * ax));
   g.dp = __builtin_fma(lc, s_ab, __builtin_fma(la, s_bc, lb * s_ca));
   g.den = __builtin_fma(la * lb, lc, g.dp);
   return g;
@@ -144,8 +144,9 @@ __device__ __forceinline__ FaceGeom64 face64_geom(const FaceRec64& r, double ox,
 __device__ __forceinline__ void face64_single(const FaceRec64* r, const FaceGeom64& g, double ox,
                                               double oy, double oz, double& acc,
                                               double& acc_slow) {
-  if (__builtin_expect(wave_all_far_x8_f64(g.det8, g.den, g.dp), 1)) {
-    acc += atan_small_x8_f64(div_f64(g.det8, g.den));
+  const double det8 = 8.0 * g.det;  // exact
+  if (__builtin_expect(wave_all_far_x8_f64(det8, g.den, g.dp), 1)) {
+    acc += atan_small_x8_f64(div_f64(det8, g.den));
   } else {
     asm volatile("" ::);  // keep a real wave-uniform branch
     acc_slow += face64_pair(r, ox, oy, oz);
@@ -164,7 +165,7 @@ __device__ __forceinline__ void face64_step(const FaceRec64& r, const double (&o
 }
 
 // Three faces for the lane's QPL queries: when every lane meets
-// sum3_x8_f64's conditions their angles are taken together as the argument
+// sum3_f64's conditions their angles are taken together as the argument
 // of the product of their (den + i det), one quotient and one polynomial for
 // the three; otherwise face by face.
 template <int QPL>
@@ -177,9 +178,9 @@ __device__ __forceinline__ void face64_triple(const FaceRec64* __restrict__ r,
     const FaceGeom64 g1 = face64_geom(r[0], ox[k], oy[k], oz[k]);
     const FaceGeom64 g2 = face64_geom(r[1], ox[k], oy[k], oz[k]);
     const FaceGeom64 g3 = face64_geom(r[2], ox[k], oy[k], oz[k]);
-    const Sum3F64 s = sum3_x8_f64(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
+    const Sum3F64 s = sum3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
     if (__builtin_expect(wave_all_sum3_f64(g1.dp, g2.dp, g3.dp, s), 1)) {
-      acc[k] += atan_sum_x8_f64(s.x8);
+      acc[k] += atan_sum_f64(s.x);
     } else {
       asm volatile("" ::);  // keep a real wave-uniform branch
       face64_single(r, g1, ox[k], oy[k], oz[k], acc[k], acc_slow[k]);

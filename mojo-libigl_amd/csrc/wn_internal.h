@@ -24,14 +24,14 @@ constexpr int kStripTileRecords = 255;  // strip records per tile (strips cut at
 
 // FP64 sweep record of strip record k (wn_strip64.hip), built on the device
 // from the records: its vertex, minus half the squared lengths to the
-// previous two vertices of its tile, and 8 x the normal of the face it
-// closes, oriented by its flag (zero when it closes none).
+// previous two vertices of its tile, and the normal of the face it closes,
+// oriented by its flag (zero when it closes none).
 struct alignas(64) StripRec64 {
   double x, y, z, e1, e2, nx, ny, nz;
 };
 
 // FP64 face record of the FP64 face kernel (wn_forward.hip), built on the
-// device from the face SoA: its corners, 8 x its normal (B - A) x (C - A),
+// device from the face SoA: its corners, its normal (B - A) x (C - A),
 // and minus half its squared edge lengths |B-A|^2, |C-B|^2, |A-C|^2.
 struct alignas(128) FaceRec64 {
   double ax, ay, az, bx, by, bz, cx, cy, cz, nx, ny, nz, eab, ebc, eca, pad;

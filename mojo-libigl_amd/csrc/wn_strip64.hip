@@ -5,12 +5,16 @@
 // square root (the other two vertices are kept from the previous records),
 // the dot products from the law of cosines with the halved squared edge
 // lengths of the sweep record, and det = N . c with the face normal of the
-// sweep record (exact: N . (a - c) = 0). A wave whose 64 pairs are all far
+// sweep record (exact: N . (a - c) = 0). Records go in groups of three
+// whose angles are taken together as the argument of the product of their
+// faces' (den + i det) (sum3_f64; an 11-term Remez polynomial of the
+// product's tan, relative error 1.6e-17 on |x| <= 0.4) when every lane
+// meets its conditions; otherwise a wave whose 64 pairs are all far
 // (|det| < den / 8 and den > la lb lc, the FP32 strip kernel's test) takes
-// the angle from a 7-term Remez polynomial of det / den (relative error
-// 1.8e-18 on |x| <= 1/8, below FP64 rounding; gen_atan_coeffs.py, n = 7);
-// any other wave evaluates every lane's pair with the plain FP64 formula
-// (pair_f64, the plain kernel's arithmetic). Square roots and the quotient
+// each angle from a 7-term polynomial of det / den (1.8e-18 on |x| <= 1/8,
+// below FP64 rounding; gen_atan_coeffs.py), and any other wave evaluates
+// every lane's pair with the plain FP64 formula (pair_f64, the plain
+// kernel's arithmetic). Square roots and the quotient
 // are refined from the hardware estimates to FP64 rounding (Goldschmidt for
 // the root, one Newton step and a remainder correction for the quotient).
 //
@@ -43,16 +47,16 @@ __global__ __launch_bounds__(kBlockThreads) void strip64_prep_kernel(
   const double d1x = c.x - p.x, d1y = c.y - p.y, d1z = c.z - p.z;
   const double d2x = c.x - pp.x, d2y = c.y - pp.y, d2z = c.z - pp.z;
   const double ex = p.x - pp.x, ey = p.y - pp.y, ez = p.z - pp.z;
-  const double f8 = 8.0 * c.w;  // orientation flag (0 = no face), times 8
+  const double f = c.w;  // orientation flag (0 = no face)
   StripRec64 r;
   r.x = c.x;
   r.y = c.y;
   r.z = c.z;
   r.e1 = -0.5 * (d1x * d1x + d1y * d1y + d1z * d1z);
   r.e2 = -0.5 * (d2x * d2x + d2y * d2y + d2z * d2z);
-  r.nx = f8 * (ey * d2z - ez * d2y);
-  r.ny = f8 * (ez * d2x - ex * d2z);
-  r.nz = f8 * (ex * d2y - ey * d2x);
+  r.nx = f * (ey * d2z - ez * d2y);
+  r.ny = f * (ez * d2x - ex * d2z);
+  r.nz = f * (ex * d2y - ey * d2x);
   out[k] = r;
 }
 
@@ -69,15 +73,15 @@ __device__ __noinline__ double strip64_pair(double ax, double ay, double az, dou
   return flip ? -v : v;
 }
 
-// Geometry of record j: R takes its vertex, and for face (P, Q, R) det8 =
-// 8 det, den and dp = den - la lb lc (meaningful where the record closes a
+// Geometry of record j: R takes its vertex, and for face (P, Q, R) det, den
+// and dp = den - la lb lc (meaningful where the record closes a
 // face; a record that closes none, flag 0, has N = 0 and is never summed).
 // s_carry is the dot product of the previous record's two newest vertices
 // (Q, R of the last record = P, Q here). Dot products come from the law of
 // cosines, b.c = (|b|^2 + |c|^2) / 2 - |b - c|^2 / 2, with -|b - c|^2 / 2
 // stored in the record: one fma, and den = pq lc + dp needs no doubling.
 struct Geom64 {
-  double det8, den, dp;
+  double det, den, dp;
 };
 
 __device__ __forceinline__ Geom64 strip64_geom(const StripRec64& r, const Slot64& P,
@@ -91,7 +95,7 @@ __device__ __forceinline__ Geom64 strip64_geom(const StripRec64& r, const Slot64
   const double s_qr = half_sum_plus(Q.l2, R.l2, r.e1);  // b.c
   const double s_pr = half_sum_plus(P.l2, R.l2, r.e2);  // c.a
   Geom64 g;
-  g.det8 = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
+  g.det = __builtin_fma(r.nz, R.az, __builtin_fma(r.ny, R.ay, r.nx * R.ax));
   const double pq = P.l * Q.l;
   g.dp = __builtin_fma(R.l, s_carry, __builtin_fma(P.l, s_qr, Q.l * s_pr));
   g.den = __builtin_fma(pq, R.l, g.dp);
@@ -107,8 +111,9 @@ __device__ __forceinline__ Geom64 strip64_geom(const StripRec64& r, const Slot64
 __device__ __forceinline__ void strip64_face(const StripRec64* __restrict__ r, int j,
                                              long long flag_bits, const Geom64& g, double& acc,
                                              double& acc_slow, double ox, double oy, double oz) {
-  if (__builtin_expect(wave_all_far_x8_f64(g.det8, g.den, g.dp), 1)) {
-    acc += atan_small_x8_f64(div_f64(g.det8, g.den));
+  const double det8 = 8.0 * g.det;  // exact
+  if (__builtin_expect(wave_all_far_x8_f64(det8, g.den, g.dp), 1)) {
+    acc += atan_small_x8_f64(div_f64(det8, g.den));
   } else {
     asm volatile("" ::);  // keep a real wave-uniform branch
     const StripRec64 &a = r[j - 2], &b = r[j - 1], &c = r[j];
@@ -121,7 +126,7 @@ __device__ __forceinline__ void strip64_face(const StripRec64* __restrict__ r, i
 // Unscaled FP64 angle sum of one query over strip tiles [t_begin, t_end).
 // Records go in groups of three (a tile's 255 = 85 x 3, so the register
 // slots rotate statically). When all three close a face (flags: scalar) and
-// every lane meets sum3_x8_f64's conditions, the group's angle is the
+// every lane meets sum3_f64's conditions, the group's angle is the
 // argument of the product of its faces' (den + i det): one quotient and one
 // 11-term polynomial for the three; otherwise each face on its own.
 __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ rec64,
@@ -138,9 +143,9 @@ __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ r
       const Geom64 g2 = strip64_geom(r[j + 1], B, C, A, s_carry, ox, oy, oz);
       const Geom64 g3 = strip64_geom(r[j + 2], C, A, B, s_carry, ox, oy, oz);
       if (__builtin_expect(f1 != 0 && f2 != 0 && f3 != 0, 1)) {
-        const Sum3F64 s = sum3_x8_f64(g1.det8, g1.den, g2.det8, g2.den, g3.det8, g3.den);
+        const Sum3F64 s = sum3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
         if (__builtin_expect(wave_all_sum3_f64(g1.dp, g2.dp, g3.dp, s), 1)) {
-          acc += atan_sum_x8_f64(s.x8);
+          acc += atan_sum_f64(s.x);
           continue;
         }
       }
