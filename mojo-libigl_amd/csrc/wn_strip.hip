@@ -257,13 +257,19 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     // from a VGPR: a VGPR byte offset advanced once per three records and
     // read with immediate offsets.
     int boff = 0;
-    for (int j = 0; j < kStripTileRecords; j += 3) {
+    // two groups of three records: their angle sums are added in FP32 (one
+    // rounding, a few ulps of the six angles' magnitudes) and joined to the
+    // FP64 sum once; 255 = 42 x 6 + 3, the last group on its own
+    int j = 0;
+    for (; j + 6 <= kStripTileRecords; j += 6) {
       asm volatile("" : "+v"(boff));
-      // the three records' angles are summed in FP32 (at most three terms,
-      // each |angle| <= pi) and added to the FP64 sum once
-      acc += (double)strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
-      boff += 3 * (int)sizeof(float4);
+      const float p1 = strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
+      const float p2 =
+          strip_triple(sm, flags, j + 3, boff + 48, A, B, C, s_carry, ox, oy, oz, kappa);
+      acc += (double)(p1 + p2);
+      boff += 6 * (int)sizeof(float4);
     }
+    acc += (double)strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
   }
   __syncthreads();
   return acc;

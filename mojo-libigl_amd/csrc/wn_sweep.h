@@ -122,15 +122,15 @@ __device__ __forceinline__ void face_step_f32c(const FaceRec32& rec, const float
   }
 }
 
-// Three faces of a tile for QPL queries: the FP32 sum of their angles, as
-// the argument of the product of their (den + i det) when every lane meets
-// sum3_f32's conditions (one reciprocal and one polynomial for the three);
-// otherwise face by face as face_step_f32c.
+// Three faces of a tile for QPL queries: the FP32 sum of their angles into
+// sum[k], as the argument of the product of their (den + i det) when every
+// lane meets sum3_f32's conditions (one reciprocal and one polynomial for
+// the three); otherwise face by face as face_step_f32c.
 template <int QPL>
 __device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const FaceRec32& rb,
                                                  const FaceRec32& rc, const float (&ox)[QPL],
                                                  const float (&oy)[QPL], const float (&oz)[QPL],
-                                                 float kappa, double (&acc)[QPL]) {
+                                                 float kappa, float (&sum)[QPL]) {
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
     const PairGeom32 g1 = pair_geom32(ra.r0, ra.r1, ra.r2, ra.r3, ox[k], oy[k], oz[k]);
@@ -149,7 +149,7 @@ __device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const Face
       face_step_f32c<1>(rc, qx, qy, qz, kappa, one);
       part = one[0];
     }
-    acc[k] += (double)part;
+    sum[k] = part;
   }
 }
 
@@ -173,11 +173,24 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
     __syncthreads();
     // the angles of three consecutive faces are summed in FP32 (the
     // argument of their product, or three FP32 angles added: at most a few
-    // ulps of their magnitudes, the bound of a compensated sum) and added to
-    // the FP64 sum once
+    // ulps of their magnitudes, the bound of a compensated sum), two such
+    // sums are added in FP32 (one more rounding) and joined to the FP64 sum
+    // once
     int j = 0;
-    for (; j + 3 <= cnt; j += 3)
-      face_triple_f32c<QPL>(sm.face[j], sm.face[j + 1], sm.face[j + 2], ox, oy, oz, kappa, acc);
+    for (; j + 6 <= cnt; j += 6) {
+      float s1[QPL], s2[QPL];
+      face_triple_f32c<QPL>(sm.face[j], sm.face[j + 1], sm.face[j + 2], ox, oy, oz, kappa, s1);
+      face_triple_f32c<QPL>(sm.face[j + 3], sm.face[j + 4], sm.face[j + 5], ox, oy, oz, kappa,
+                            s2);
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) acc[k] += (double)(s1[k] + s2[k]);
+    }
+    for (; j + 3 <= cnt; j += 3) {
+      float s1[QPL];
+      face_triple_f32c<QPL>(sm.face[j], sm.face[j + 1], sm.face[j + 2], ox, oy, oz, kappa, s1);
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) acc[k] += (double)s1[k];
+    }
     if (j < cnt) {
       float part[QPL];
 #pragma unroll
