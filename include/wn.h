@@ -54,7 +54,9 @@ void mli_winding(const double *tri, const double *q, double *dst, int64_t nf,
  * mesh kept for a repeated call as wn_winding_gpu_mesh keeps it). Both agree
  * with the plain formula (precision 2) to ~1e-15. Returns 1 on success, 0 on any failure (reason
  * in wn_last_error()), as the reference's try_winding_gpu does
- * (src/capi.mojo:366-398). Thread-safe. */
+ * (src/capi.mojo:366-398). Thread-safe: the host-buffer entries run on a
+ * pool of four contexts (stream, device buffer, kept strip mesh), so calls
+ * from several threads overlap. */
 int64_t mli_winding_gpu(const double *tri, const double *q, double *dst,
                         int64_t nf, int64_t nq);
 

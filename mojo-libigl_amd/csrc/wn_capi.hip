@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -81,10 +82,53 @@ struct HostPathContext {
   size_t cap = 0;
 };
 
-HostPathContext& host_ctx() {
-  static HostPathContext ctx;
+// The host-buffer entries (mli_winding_gpu, wn_winding_gpu_ex / _vf /
+// _mesh) run on one of kHostContexts contexts, each with its own stream,
+// grow-only device buffer and kept strip mesh, so that callers on several
+// threads overlap their copies and kernels instead of queueing on one lock
+// (as many as the device's default hardware queues).
+constexpr int kHostContexts = 4;
+
+HostPathContext* host_contexts() {
+  static HostPathContext ctx[kHostContexts];
   return ctx;
 }
+
+// A context for the calling thread, held for the call: the one the thread
+// used last (its kept strip mesh and buffers) when free, else any free one,
+// else the thread's own once it is released. A thread's first context is
+// dealt round robin, so concurrent first calls spread out.
+class HostLease {
+ public:
+  HostLease() {
+    static std::atomic<int> deal{0};
+    thread_local int last = -1;
+    if (last < 0) last = deal.fetch_add(1) % kHostContexts;
+    HostPathContext* all = host_contexts();
+    int pick = last;
+    std::unique_lock<std::mutex> l(all[pick].mu, std::try_to_lock);
+    for (int i = 1; i < kHostContexts && !l.owns_lock(); ++i) {
+      const int k = (last + i) % kHostContexts;
+      std::unique_lock<std::mutex> t(all[k].mu, std::try_to_lock);
+      if (t.owns_lock()) {
+        l = std::move(t);
+        pick = k;
+      }
+    }
+    if (!l.owns_lock()) {
+      l = std::unique_lock<std::mutex>(all[last].mu);
+      pick = last;
+    }
+    last = pick;
+    ctx_ = &all[pick];
+    lock_ = std::move(l);
+  }
+  HostPathContext& ctx() { return *ctx_; }
+
+ private:
+  HostPathContext* ctx_ = nullptr;
+  std::unique_lock<std::mutex> lock_;
+};
 
 std::string hip_msg(const char* what, hipError_t e) {
   return std::string(what) + ": " + hipGetErrorString(e);
@@ -187,8 +231,8 @@ int64_t host_path(const HostMesh& mesh, const double* q, double* dst, int64_t nf
   }
   if (!q || !dst) return fail(kErrArg, "null query or output buffer");
   if (const char* why = wn::fwd_check(nf, nq, cfg)) return fail(kErrArg, why);
-  HostPathContext& ctx = host_ctx();
-  std::lock_guard<std::mutex> lock(ctx.mu);
+  HostLease lease;
+  HostPathContext& ctx = lease.ctx();
   hipError_t e;
   if (!ctx.ready) {
     if (device_count() < 1) return fail(kErrNoDevice, "no HIP device available");
@@ -1086,8 +1130,8 @@ int64_t wn_winding_gpu_mesh(const double* V, const void* F, int64_t nv, int64_t 
   if (!q || !dst) return fail(kErrArg, "null query or output buffer");
   if (fdtype != 0 && fdtype != 1) return fail(kErrArg, "fdtype must be 0 (int64) or 1 (int32)");
   if (nf > 0 && (!V || !F)) return fail(kErrArg, "null host buffer");
-  HostPathContext& ctx = host_ctx();
-  std::lock_guard<std::mutex> lock(ctx.mu);
+  HostLease lease;
+  HostPathContext& ctx = lease.ctx();
   HostStripCache& cache = ctx.strips;
   const size_t f_bytes = (size_t)nf * 3 * (fdtype == 0 ? 8 : 4);
   const size_t v_bytes = (size_t)nv * 3 * sizeof(double);
@@ -1180,15 +1224,18 @@ int64_t wn_winding_gpu_mesh(const double* V, const void* F, int64_t nv, int64_t 
 }
 
 int64_t wn_release_host_buffers(void) {
-  HostPathContext& ctx = host_ctx();
-  std::lock_guard<std::mutex> lock(ctx.mu);
-  if (ctx.strips.mesh != 0) (void)wn_mesh_destroy(ctx.strips.mesh);
-  ctx.strips = HostStripCache();
-  if (ctx.buf) {
-    (void)hipStreamSynchronize(ctx.stream);
-    (void)hipFree(ctx.buf);
-    ctx.buf = nullptr;
-    ctx.cap = 0;
+  HostPathContext* all = host_contexts();
+  for (int k = 0; k < kHostContexts; ++k) {
+    HostPathContext& ctx = all[k];
+    std::lock_guard<std::mutex> lock(ctx.mu);
+    if (ctx.strips.mesh != 0) (void)wn_mesh_destroy(ctx.strips.mesh);
+    ctx.strips = HostStripCache();
+    if (ctx.buf) {
+      (void)hipStreamSynchronize(ctx.stream);
+      (void)hipFree(ctx.buf);
+      ctx.buf = nullptr;
+      ctx.cap = 0;
+    }
   }
   return kOk;
 }

@@ -357,6 +357,43 @@ def test_host_entry_is_safe_under_concurrent_callers(require_gpu):
     assert not errors and all(got)
 
 
+def test_strip_route_under_concurrent_callers(require_gpu, monkeypatch):
+    """wn_winding_gpu_mesh (the strip route of igl.winding_number) from 6
+    threads at once, each on its own mesh and alternating FP64 and FP32c:
+    the host entries run on a pool of contexts (own stream, buffers and kept
+    strip mesh each), and every call returns the single-threaded result, bit
+    for bit, whichever context it lands on."""
+    import threading
+
+    monkeypatch.setattr(igl, "STRIP_MIN_QUERIES", {0: 1000, 1: 1000})
+    V0, F0, Q = W.config2(4096)
+    meshes = [(V0 * (1.0 + 0.1 * i) + [0.05 * i, 0.0, -0.02 * i], F0) for i in range(6)]
+    expect = {}
+    for i, (V, F) in enumerate(meshes):
+        for precision in ("fp64", "fp32c"):
+            assert igl.lib().wn_release_host_buffers() == 0
+            expect[i, precision] = igl.winding_number(V, F, Q, device="gpu", precision=precision)
+    errors = []
+
+    def run(i):
+        V, F = meshes[i]
+        try:
+            for r in range(6):
+                precision = ("fp64", "fp32c")[(i + r) % 2]
+                w = igl.winding_number(V, F, Q, device="gpu", precision=precision)
+                if not np.array_equal(w, expect[i, precision]):
+                    errors.append((i, r, precision))
+        except Exception as e:  # pragma: no cover - reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(6)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors
+
+
 def test_large_calls_use_strip_kernels(require_gpu, monkeypatch):
     """igl.winding_number sends calls with many queries through the strip
     kernels (wn_winding_gpu_mesh): FP64 agrees with the plain FP64 kernel to
