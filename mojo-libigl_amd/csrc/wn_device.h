@@ -144,7 +144,7 @@ __device__ __forceinline__ float atan_small_x8_f32(float x) {
 
 // atan(x) for |x| <= 0.4: x * Q(x^2), Q of degree 4 in x^2, relative-error
 // minimax on (0, 0.4] (max 1.30e-8, gen_atan_coeffs.py, n = 5). The angle of
-// a product of three far faces' (den + i det) (sum3_f32).
+// a product of far faces' (den + i det) (sum3_f32, sum6_f32).
 __device__ __forceinline__ float atan_sum_f32(float x) {
   const float t = x * x;
   float p = 0.08156783959325319f;
@@ -155,45 +155,99 @@ __device__ __forceinline__ float atan_sum_f32(float x) {
   return x * p;
 }
 
-// The angles of three faces, summed as the argument of the product of
-// their w_k = den_k + i det_k:
+// The angles of a group of faces, summed as the argument of the product of
+// their w_k = den_k + i det_k. For three faces (prod3_f32):
 //   w1 w2 = re12 + i im12:  re12 = d1 d2 - e1 e2,     im12 = d1 e2 + e1 d2
 //   w1 w2 w3 = re + i im:   re = re12 d3 - im12 e3,   im = re12 e3 + im12 d3
-// and x = im / re = tan(angle_1 + angle_2 + angle_3). Valid (no wrap, x in
-// the polynomial's range, no overflow or underflow) when every face has
-// den > la lb lc (each angle within +-pi/2, so the sum within +-3pi/2: the
-// product's argument is the sum itself once it lies within +-0.39), re is
-// a positive normal number and |x| < 0.4; wave_all_sum3 tests that. The
-// absolute error of the argument is that of adding the three FP32 angles:
-// a few ulps of |angle_1| + |angle_2| + |angle_3|.
-struct Sum3 {
-  float re, x;
+// and x = im / re = tan(angle_1 + angle_2 + angle_3) (sum3_f32); for six
+// faces the product of two such groups (sum6_f32). A face with
+// den > la lb lc has |det| <= la lb lc < den (|a . (b x c)| <= |a||b||c|),
+// so its angle lies within +-pi/4: three angles sum to within +-3pi/4, six
+// to within +-3pi/2, and in both cases a product argument within +-0.39 is
+// the sum itself (the other candidates lie 2pi away, beyond 5.8). The sum
+// is then atan(x) when, on top of that, 1/re is a positive normal number
+// and |x| < 0.4 (the polynomial's range; wave_all_group tests both). 1/re
+// positive normal bounds re to [2^-128, 2^126] (and rejects re zero,
+// negative, infinite or NaN): above 2^126 the reciprocal would leave the
+// normal range, which the transcendental unit returns as 0, and x would
+// read 0 whatever the angles. The absolute error of the argument is that of
+// adding the FP32 angles: a few ulps of the sum of their magnitudes (each
+// product term carries one relative rounding; re just below 2^-126, whose
+// reciprocal still passes, keeps 21 of its 24 bits).
+struct Prod3 {
+  float re, im;
 };
 
-__device__ __forceinline__ Sum3 sum3_f32(float e1, float d1, float e2, float d2, float e3,
-                                         float d3) {
+__device__ __forceinline__ Prod3 prod3_f32(float e1, float d1, float e2, float d2, float e3,
+                                           float d3) {
   const float re12 = __builtin_fmaf(-e1, e2, d1 * d2);
   const float im12 = __builtin_fmaf(d1, e2, e1 * d2);
-  Sum3 s;
-  s.re = __builtin_fmaf(-im12, e3, re12 * d3);
-  s.x = __builtin_fmaf(re12, e3, im12 * d3) * __builtin_amdgcn_rcpf(s.re);
+  Prod3 t;
+  t.re = __builtin_fmaf(-im12, e3, re12 * d3);
+  t.im = __builtin_fmaf(re12, e3, im12 * d3);
+  return t;
+}
+
+struct AngleSum {
+  float rc, x;  // 1 / re and x = im / re
+};
+
+__device__ __forceinline__ AngleSum angle_sum_of(float re, float im) {
+  AngleSum s;
+  s.rc = __builtin_amdgcn_rcpf(re);
+  s.x = im * s.rc;
   return s;
 }
 
-// Every active lane meets sum3_f32's conditions (dp_k = den_k - la lb lc).
-// The three dp tests are one min3 and one compare: a NaN dp_k, which the
-// min would skip, comes with a NaN den_k and so a NaN re, which the class
-// test rejects.
-__device__ __forceinline__ bool wave_all_sum3(float dp1, float dp2, float dp3, const Sum3& s) {
-  const uint64_t m1 =
-      __builtin_amdgcn_ballot_w64(__builtin_fminf(__builtin_fminf(dp1, dp2), dp3) > 0.0f);
-  // re positive normal (not zero, denormal, infinite or NaN): one
-  // v_cmp_class_f32 writing the lane mask (the class builtin goes through a
-  // VGPR select and a second compare before its ballot)
-  uint64_t m2;
-  asm volatile("v_cmp_class_f32_e64 %0, %1, %2" : "=s"(m2) : "v"(s.re), "s"(1 << 8));
+__device__ __forceinline__ AngleSum sum3_f32(float e1, float d1, float e2, float d2, float e3,
+                                             float d3) {
+  const Prod3 t = prod3_f32(e1, d1, e2, d2, e3, d3);
+  return angle_sum_of(t.re, t.im);
+}
+
+// Six faces as two groups of three: (t1.re + i t1.im)(t2.re + i t2.im).
+__device__ __forceinline__ AngleSum sum6_f32(const Prod3& t1, const Prod3& t2) {
+  return angle_sum_of(__builtin_fmaf(-t1.im, t2.im, t1.re * t2.re),
+                      __builtin_fmaf(t1.re, t2.im, t1.im * t2.re));
+}
+
+// 1/re positive normal (see above): one v_cmp_class_f32 writing the lane
+// mask (the class builtin goes through a VGPR select and a second compare
+// before its ballot).
+__device__ __forceinline__ uint64_t lanes_positive_normal(float v) {
+  uint64_t m;
+  asm volatile("v_cmp_class_f32_e64 %0, %1, %2" : "=s"(m) : "v"(v), "s"(1 << 8));
+  return m;
+}
+
+// Every active lane meets the group's conditions: dp_min, the smallest
+// dp_k = den_k - la lb lc of the group, positive; 1/re positive normal;
+// |x| < 0.4. A NaN dp_k, which a min skips, comes with a NaN den_k and so a
+// NaN re and reciprocal, which the class test rejects.
+__device__ __forceinline__ uint64_t group_lanes(float dp_min, const AngleSum& s) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(dp_min > 0.0f);
+  const uint64_t m2 = lanes_positive_normal(s.rc);
   const uint64_t m3 = __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
-  return (m1 & m2 & m3) == __builtin_amdgcn_read_exec();
+  return m1 & m2 & m3;
+}
+
+__device__ __forceinline__ bool wave_all_group(float dp_min, const AngleSum& s) {
+  return group_lanes(dp_min, s) == __builtin_amdgcn_read_exec();
+}
+
+// Three faces: the three dp tests are one min3 and the group's compare.
+__device__ __forceinline__ bool wave_all_sum3(float dp1, float dp2, float dp3,
+                                              const AngleSum& s) {
+  return wave_all_group(__builtin_fminf(__builtin_fminf(dp1, dp2), dp3), s);
+}
+
+// Six faces (sum6_f32 of t1, t2; m1, m2 the groups' smallest dp): also
+// t1.re and t2.re positive normal, so that neither group's product lost
+// precision to underflow before the last multiplication.
+__device__ __forceinline__ bool wave_all_sum6(float m1, float m2, const Prod3& t1,
+                                              const Prod3& t2, const AngleSum& s) {
+  const uint64_t m = lanes_positive_normal(__builtin_fminf(t1.re, t2.re));
+  return (group_lanes(__builtin_fminf(m1, m2), s) & m) == __builtin_amdgcn_read_exec();
 }
 
 // Geometry of one pair in FP32 from a record. Algebra (exact identities,

@@ -4,11 +4,11 @@
 // Same arithmetic as fwd_f32c_kernel (block-local origin, FP32 pairs, FP64
 // accumulation, far-pair fast path, FP64 re-evaluation of ill-conditioned
 // pairs, the fast path on |det / den| < 1/8 with a 3-term arctangent),
-// except that the angles of three consecutive records are summed in
-// FP32 before each FP64 add (error at most 2 eps times the three angles'
-// magnitudes, the bound of a compensated sum, with a third of the FP64
-// instructions), and the re-evaluated angle is rounded to FP32 once before
-// that sum. Faces arrive as a strip: record k brings vertex w_k and closes
+// except that the angles of six consecutive records are summed in FP32
+// before each FP64 add (error a few ulps of the six angles' magnitudes, the
+// bound of a compensated sum, with a sixth of the FP64 instructions), and
+// the re-evaluated angle is rounded to FP32 once before that sum. Faces
+// arrive as a strip: record k brings vertex w_k and closes
 // face (w_{k-2}, w_{k-1}, w_k). Per face the lane computes one offset, one
 // squared length and one square root (the other two vertices are kept in
 // registers from the previous records) and two law-of-cosines dot products
@@ -25,10 +25,12 @@
 // staged with N = 0 and R1.x = +2^30, which makes its angle 0 without a
 // branch. The flag itself is read (from the records, at a wave-uniform
 // address) only on the rare slow path.
-// Fast path: for each group of three records, the sum of their angles is the
-// argument of the product of the faces' (den + i det) (sum3_f32): one
-// reciprocal and one 5-term polynomial per three faces instead of three of
-// each.
+// Fast path: for each group of six records, the sum of their angles is the
+// argument of the product of the faces' (den + i det) (sum6_f32): one
+// reciprocal and one 5-term polynomial per six faces instead of six of each.
+// A group that fails its test is taken as two groups of three (sum3_f32),
+// and a group of three that fails as single faces; the single faces are
+// rebuilt from the LDS tile, so the fast path keeps no per-face values live.
 // Squared lengths and N are computed in FP64 from the rounded FP32 corners.
 
 #include "wn_device.h"
@@ -69,7 +71,7 @@ __device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec
   // A record that closes no face (flag 0) gets N = 0 and +2^30 in place of
   // -|w_k - w_{k-2}|^2 / 2: det is then 0 exactly and c.a ~ 2^30 makes dp
   // and den positive and finite (see strip_record), so both fast paths give
-  // it angle 0 (0 / den, and a real factor den + 0i of the triple product).
+  // it angle 0 (0 / den, and a real factor den + 0i of a group product).
   const double f = c.w;
   sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
   sm.r1[j] = make_float4(c.w == 0.0 ? 1073741824.0f
@@ -173,7 +175,7 @@ __device__ __forceinline__ float strip_angle(const StripSmem& sm,
 // the argument of the product of the faces' (den + i det), one reciprocal
 // and one polynomial for the three faces; otherwise each face takes
 // strip_angle and the three FP32 angles are added (-0 + x == x: the first
-// add folds away).
+// add folds away). Used for the last three records of a tile.
 __device__ __forceinline__ float strip_triple(const StripSmem& sm,
                                               const long long* __restrict__ flags, int j,
                                               int boff, VSlot& A, VSlot& B, VSlot& C,
@@ -183,7 +185,7 @@ __device__ __forceinline__ float strip_triple(const StripSmem& sm,
   const StripGeom g1 = strip_record(sm, boff, A, B, C, s_carry, ox, oy, oz);
   const StripGeom g2 = strip_record(sm, boff + 16, B, C, A, s_carry, ox, oy, oz);
   const StripGeom g3 = strip_record(sm, boff + 32, C, A, B, s_carry, ox, oy, oz);
-  const Sum3 s = sum3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+  const AngleSum s = sum3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
   if (__builtin_expect(wave_all_sum3(g1.dp, g2.dp, g3.dp, s), 1)) return atan_sum_f32(s.x);
   asm volatile("" ::);  // keep a real wave-uniform branch
   float part = -0.0f;
@@ -191,6 +193,68 @@ __device__ __forceinline__ float strip_triple(const StripSmem& sm,
   part += strip_angle(sm, flags, j + 1, g2, A.l, ox, oy, oz, kappa);
   part += strip_angle(sm, flags, j + 2, g3, B.l, ox, oy, oz, kappa);
   return part;
+}
+
+// The angle of record j's face on its own, its geometry rebuilt from the
+// LDS tile (corners j-2, j-1, j). The operations are strip_record's, in
+// its order (a.b as the previous record formed it), so the values are those
+// the sweep computed. A record with flag 0 closes no face: angle 0. Records
+// 0 and 1 of a tile have flag 0 (every tile starts a strip).
+__device__ __forceinline__ float strip_face_lds(const StripSmem& sm,
+                                                const long long* __restrict__ flags, int j,
+                                                float ox, float oy, float oz, float kappa) {
+  if (flags[4 * j] == 0) return 0.0f;  // wave-uniform
+  const float4 p = sm.r0[j - 2], q = sm.r0[j - 1], c = sm.r0[j];
+  VSlot P, Q, R;
+  strip_vertex(p, ox, oy, oz, P);
+  strip_vertex(q, ox, oy, oz, Q);
+  strip_vertex(c, ox, oy, oz, R);
+  const float s_pq = __builtin_fmaf(P.l2 + Q.l2, 0.5f, q.w);
+  const StripGeom g = strip_geom(c, sm.r1[j], P, Q, R, s_pq);
+  return strip_angle(sm, flags, j, g, R.l, ox, oy, oz, kappa);
+}
+
+// Records j, j+1, j+2 whose product t (prod3_f32) and smallest dp the sweep
+// already formed: the group's FP32 angle sum by sum3_f32's test, else face
+// by face from the tile.
+__device__ __forceinline__ float strip_triple_lds(const StripSmem& sm,
+                                                  const long long* __restrict__ flags, int j,
+                                                  const Prod3& t, float dp_min, float ox,
+                                                  float oy, float oz, float kappa) {
+#pragma clang fp contract(off)  // part += angle rounds the angle first
+  const AngleSum s = angle_sum_of(t.re, t.im);
+  if (__builtin_expect(wave_all_group(dp_min, s), 1)) return atan_sum_f32(s.x);
+  asm volatile("" ::);  // keep a real wave-uniform branch
+  float part = -0.0f;
+  part += strip_face_lds(sm, flags, j, ox, oy, oz, kappa);
+  part += strip_face_lds(sm, flags, j + 1, ox, oy, oz, kappa);
+  part += strip_face_lds(sm, flags, j + 2, ox, oy, oz, kappa);
+  return part;
+}
+
+// Records j .. j+5 (two turns of the slot rotation): the FP32 sum of their
+// six angles, as one product of six (sum6_f32) when every lane meets
+// wave_all_sum6, else as two groups of three.
+__device__ __forceinline__ float strip_hexad(const StripSmem& sm,
+                                             const long long* __restrict__ flags, int j, int boff,
+                                             VSlot& A, VSlot& B, VSlot& C, float& s_carry,
+                                             float ox, float oy, float oz, float kappa) {
+#pragma clang fp contract(off)  // the two groups' sums round before they add
+  const StripGeom g1 = strip_record(sm, boff, A, B, C, s_carry, ox, oy, oz);
+  const StripGeom g2 = strip_record(sm, boff + 16, B, C, A, s_carry, ox, oy, oz);
+  const StripGeom g3 = strip_record(sm, boff + 32, C, A, B, s_carry, ox, oy, oz);
+  const Prod3 t1 = prod3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+  const float m1 = __builtin_fminf(__builtin_fminf(g1.dp, g2.dp), g3.dp);
+  const StripGeom g4 = strip_record(sm, boff + 48, A, B, C, s_carry, ox, oy, oz);
+  const StripGeom g5 = strip_record(sm, boff + 64, B, C, A, s_carry, ox, oy, oz);
+  const StripGeom g6 = strip_record(sm, boff + 80, C, A, B, s_carry, ox, oy, oz);
+  const Prod3 t2 = prod3_f32(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
+  const float m2 = __builtin_fminf(__builtin_fminf(g4.dp, g5.dp), g6.dp);
+  const AngleSum s = sum6_f32(t1, t2);
+  if (__builtin_expect(wave_all_sum6(m1, m2, t1, t2, s), 1)) return atan_sum_f32(s.x);
+  asm volatile("" ::);  // keep a real wave-uniform branch
+  return strip_triple_lds(sm, flags, j, t1, m1, ox, oy, oz, kappa) +
+         strip_triple_lds(sm, flags, j + 3, t2, m2, ox, oy, oz, kappa);
 }
 
 // First thread of the calling wave, in a scalar register (all lanes active).
@@ -257,16 +321,12 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     // from a VGPR: a VGPR byte offset advanced once per three records and
     // read with immediate offsets.
     int boff = 0;
-    // two groups of three records: their angle sums are added in FP32 (one
-    // rounding, a few ulps of the six angles' magnitudes) and joined to the
-    // FP64 sum once; 255 = 42 x 6 + 3, the last group on its own
+    // groups of six records, each group's FP32 angle sum joined to the FP64
+    // sum once; 255 = 42 x 6 + 3, the last three records as a group of three
     int j = 0;
     for (; j + 6 <= kStripTileRecords; j += 6) {
       asm volatile("" : "+v"(boff));
-      const float p1 = strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
-      const float p2 =
-          strip_triple(sm, flags, j + 3, boff + 48, A, B, C, s_carry, ox, oy, oz, kappa);
-      acc += (double)(p1 + p2);
+      acc += (double)strip_hexad(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
       boff += 6 * (int)sizeof(float4);
     }
     acc += (double)strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);

@@ -136,7 +136,7 @@ __device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const Face
     const PairGeom32 g1 = pair_geom32(ra.r0, ra.r1, ra.r2, ra.r3, ox[k], oy[k], oz[k]);
     const PairGeom32 g2 = pair_geom32(rb.r0, rb.r1, rb.r2, rb.r3, ox[k], oy[k], oz[k]);
     const PairGeom32 g3 = pair_geom32(rc.r0, rc.r1, rc.r2, rc.r3, ox[k], oy[k], oz[k]);
-    const Sum3 s = sum3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+    const AngleSum s = sum3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
     float part;
     if (__builtin_expect(wave_all_sum3(g1.dp, g2.dp, g3.dp, s), 1)) {
       part = atan_sum_f32(s.x);

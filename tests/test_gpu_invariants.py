@@ -4,9 +4,11 @@ against the oracle evaluated on the same transformed inputs:
 * translation far from the origin (1e4 .. 1e6): the FP32-compensated paths
   keep their bar only because each block works in FP32 offsets from its own
   origin; plain FP32 coordinates at 1e6 would be off by ~0.06;
-* uniform scale (1e-5, 1e-3, 1e3, 1e6: the FP32 product of three faces'
-  den underflows or overflows at the ends, so the fast paths' per-face
-  fallback is exercised), a rotation, a face permutation, a cyclic
+* uniform scale (1e-5, 1e-3, 1e3, 5e3, 1e4, 2e4, 1e6: the FP32 product of a
+  group of faces' den underflows or overflows at the ends, so the fast
+  paths' smaller groups and per-face fallback are exercised; at 5e3..2e4 it
+  lands between 2^126 and the FP32 maximum, where its reciprocal is no
+  longer a normal number), a rotation, a face permutation, a cyclic
   permutation of every face's corners (w unchanged), and a swap of two
   corners of every face (w negated).
 
@@ -69,6 +71,11 @@ def transforms():
         # groups must fall back to one face at a time
         "scale_1e-5": (V * 1e-5, F, Q * 1e-5, 1.0),
         "scale_1e6": (V * 1e6, F, Q * 1e6, 1.0),
+        # products of 2^126 .. 3.4e38: finite, but 1 / re is below the
+        # normal range (the reciprocal unit returns 0 there)
+        "scale_5e3": (V * 5e3, F, Q * 5e3, 1.0),
+        "scale_1e4": (V * 1e4, F, Q * 1e4, 1.0),
+        "scale_2e4": (V * 2e4, F, Q * 2e4, 1.0),
         "rotate": (V @ R.T, F, Q @ R.T, 1.0),
         "permute_faces": (V, F[perm], Q, 1.0),
         "cycle_corners": (V, F[:, [1, 2, 0]], Q, 1.0),
