@@ -81,15 +81,17 @@ __global__ __launch_bounds__(kBlockThreads) void batch_kernel(
       acc[k] = 0.0;
     }
     double gx, gy, gz;
-    block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz);
+    int er;
+    block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz, er);
+    const double so = pow2_f64(2 - er);
     float ox[QPL], oy[QPL], oz[QPL];
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
-      ox[k] = valid[k] ? (float)(Ox[k] - gx) : 0.0f;
-      oy[k] = valid[k] ? (float)(Oy[k] - gy) : 0.0f;
-      oz[k] = valid[k] ? (float)(Oz[k] - gz) : 0.0f;
+      ox[k] = valid[k] ? (float)((Ox[k] - gx) * so) : 0.0f;
+      oy[k] = valid[k] ? (float)((Oy[k] - gy) * so) : 0.0f;
+      oz[k] = valid[k] ? (float)((Oz[k] - gz) * so) : 0.0f;
     }
-    sweep_f32c<QPL>(tri, ftot, f0, f1, gx, gy, gz, ox, oy, oz, kappa, acc, sm);
+    sweep_f32c<QPL>(tri, ftot, f0, f1, gx, gy, gz, er, ox, oy, oz, kappa, acc, sm);
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
       if (valid[k]) {

@@ -50,6 +50,53 @@ __device__ __forceinline__ float atan2_f32(float y, float x, float& mx) {
 }
 
 // ---------------------------------------------------------------------------
+// Power-of-two frames of the FP32-compensated kernels.
+//
+// Every FP32 quantity of a pair scales with a power of its coordinates (the
+// lengths l, den ~ l^3, a product of six faces' (den + i det) ~ l^18), and
+// the group products leave the FP32 range unless l stays within about
+// [2^-7, 2^6]. Each LDS tile is therefore staged in a frame scaled by
+// sc = 2^(2 - E), where 2^E bounds both |corner - g| over the tile and
+// |query - g| over the block (g the block origin, infinity norm): every
+// pair's |corner - query| is then below 8 sqrt(3) ~ 13.9. Multiplying by a
+// power of two is exact and FP32 rounding commutes with it (for results in
+// the normal range), so the rounded corners and query offsets are the
+// unscaled ones times sc, every angle is unchanged, and a power-of-two
+// rescaling of the whole input gives bit-identical results.
+// ---------------------------------------------------------------------------
+
+// floor(log2(m)) of m >= 0 from its exponent field, clamped to [-1000, 1000]
+// (0 and subnormals give -1000), so that 2^(2 - e) and 2^(2 + 1000) stay
+// normal FP64 numbers.
+__device__ __forceinline__ int frame_exponent(double m) {
+  const int e = (int)((uint64_t)__double_as_longlong(m) >> 52) - 1023;
+  return e < -1000 ? -1000 : (e > 1000 ? 1000 : e);
+}
+
+// High 32 bits of |d| (sign cleared): monotonic in |d|, and its exponent
+// field is d's. A max over these words is a max of |d| to ~20 mantissa bits.
+__device__ __forceinline__ uint32_t abs_hi_word(double d) {
+  return (uint32_t)((uint64_t)__double_as_longlong(d) >> 32) & 0x7fffffffu;
+}
+
+// frame_exponent of the double whose (sign-cleared) high word is w.
+__device__ __forceinline__ int frame_exponent_hi(uint32_t w) {
+  const int e = (int)(w >> 20) - 1023;
+  return e < -1000 ? -1000 : (e > 1000 ? 1000 : e);
+}
+
+// 2^k as FP64, k in [-1022, 1023].
+__device__ __forceinline__ double pow2_f64(int k) {
+  return __longlong_as_double((long long)(k + 1023) << 52);
+}
+
+// 2^k as FP32 for k in [-126, 0]; 0 below (a query offset that small next to
+// the tile's extent is below FP32 resolution of the pair).
+__device__ __forceinline__ float pow2_f32(int k) {
+  return k < -126 ? 0.0f : __int_as_float((k + 127) << 23);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 pair in the reference's exact arithmetic form (cross-product det,
 // dot-product den), with OCML's correctly-rounded sqrt and atan2.
 // ---------------------------------------------------------------------------
@@ -373,6 +420,19 @@ __device__ __forceinline__ double wave_max_uniform_f64(double v) {
   v = fmax(v, dpp_move_f64<0x128>(v));
   v = fmax(v, swizzle_xor16_f64(v));
   return fmax(lane_f64(v, 0), lane_f64(v, 32));
+}
+
+// Wave maximum of an unsigned 32-bit value, every lane active, in a scalar
+// register (the steps of wave_max_uniform_f64 on one 32-bit word).
+__device__ __forceinline__ uint32_t wave_max_uniform_u32(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, true));
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true));
+  v = max(v, (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F));
+  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+  const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+  return a > b ? a : b;
 }
 
 __device__ __forceinline__ double wave_min_f64(double v) {

@@ -328,20 +328,22 @@ __global__ __launch_bounds__(kBlockThreads) void fwd_f32c_kernel(
     Oz[k] = valid[k] ? q[3 * slot + 2] : 0.0;
   }
   double gx, gy, gz;
-  block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz);
+  int er;
+  block_origin<QPL>(Ox, Oy, Oz, valid, sm, gx, gy, gz, er);
+  const double so = pow2_f64(2 - er);
 
   float ox[QPL], oy[QPL], oz[QPL];
   double acc[QPL];
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
-    ox[k] = valid[k] ? (float)(Ox[k] - gx) : 0.0f;
-    oy[k] = valid[k] ? (float)(Oy[k] - gy) : 0.0f;
-    oz[k] = valid[k] ? (float)(Oz[k] - gz) : 0.0f;
+    ox[k] = valid[k] ? (float)((Ox[k] - gx) * so) : 0.0f;
+    oy[k] = valid[k] ? (float)((Oy[k] - gy) * so) : 0.0f;
+    oz[k] = valid[k] ? (float)((Oz[k] - gz) * so) : 0.0f;
     acc[k] = 0.0;
   }
   const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
   const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
-  sweep_f32c<QPL>(tri, nf, f0, f1, gx, gy, gz, ox, oy, oz, kappa, acc, sm);
+  sweep_f32c<QPL>(tri, nf, f0, f1, gx, gy, gz, er, ox, oy, oz, kappa, acc, sm);
 
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {

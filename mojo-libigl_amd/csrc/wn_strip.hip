@@ -18,6 +18,9 @@
 // statically through a 3-way unrolled loop):
 //   R0[j] = (w.x, w.y, w.z, -|w_j - w_{j-1}|^2 / 2)  (block-local FP32 coords)
 //   R1[j] = (-|w_j - w_{j-2}|^2 / 2, N.x, N.y, N.z)  (N = face normal)
+// in the tile's power-of-two frame (frame_exponent in wn_device.h: the
+// block-local coordinates times 2^(2 - E), |corner - query| < 14 in it, so
+// that the group products below stay in the FP32 range at any input scale);
 // so that a dot product is one add and one fma with 0.5 (law of cosines;
 // the halving is exact) and den is formed undoubled.
 // The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
@@ -43,6 +46,7 @@ struct StripSmem {
   float4 r0[kStripTileRecords];
   float4 r1[kStripTileRecords];
   double red[6][kBlockThreads / kWave];
+  uint32_t tmax[kBlockThreads / kWave];  // per wave: abs_hi_word max |corner - g| of the next tile
 };
 
 namespace {
@@ -51,33 +55,70 @@ struct VSlot {
   float ax, ay, az, l2, l;
 };
 
-__device__ __forceinline__ void stage_strip_tile(const double4* __restrict__ rec, int64_t base,
-                                                 int cnt, double gx, double gy, double gz,
-                                                 StripSmem& sm) {
+// One tile's staging, in two steps around a block barrier. strip_tile_extent:
+// every thread of the block (all lanes: the wave reduction needs them) reads
+// its record j, and its wave's max over the tile of |corner - g| (infinity
+// norm) goes to sm.tmax. After the barrier, strip_tile_store reads records
+// j, j-1, j-2 (the record loads of the first step, again: nothing is held
+// across the barrier), scales the tile into the frame sc = 2^(2 - E)
+// (frame_exponent: 2^E bounds |corner - g| over the tile and, through er,
+// |query - g| over the block) and writes the LDS records; it returns the
+// factor 2^(er - E) that takes the lanes' offsets (staged at 2^(2 - er))
+// into the same frame.
+__device__ __forceinline__ void strip_tile_extent(const double4* __restrict__ rec, int64_t base,
+                                                  int cnt, double gx, double gy, double gz,
+                                                  StripSmem& sm) {
   const int j = threadIdx.x;
-  if (j >= cnt) return;
-  const double4 c = rec[base + j];
-  const double4 p = rec[base + (j >= 1 ? j - 1 : j)];
-  const double4 pp = rec[base + (j >= 2 ? j - 2 : j)];
-  const float cx = (float)(c.x - gx), cy = (float)(c.y - gy), cz = (float)(c.z - gz);
-  const float px = (float)(p.x - gx), py = (float)(p.y - gy), pz = (float)(p.z - gz);
-  const float qx = (float)(pp.x - gx), qy = (float)(pp.y - gy), qz = (float)(pp.z - gz);
-  // FP64 on the rounded corners: w_k - w_{k-1}, w_k - w_{k-2}, w_{k-1} - w_{k-2}
-  const double d1x = (double)cx - px, d1y = (double)cy - py, d1z = (double)cz - pz;
-  const double d2x = (double)cx - qx, d2y = (double)cy - qy, d2z = (double)cz - qz;
-  const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
-  // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag: the
-  // lane's dot product with it is det
-  // A record that closes no face (flag 0) gets N = 0 and +2^30 in place of
-  // -|w_k - w_{k-2}|^2 / 2: det is then 0 exactly and c.a ~ 2^30 makes dp
-  // and den positive and finite (see strip_record), so both fast paths give
-  // it angle 0 (0 / den, and a real factor den + 0i of a group product).
-  const double f = c.w;
-  sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
-  sm.r1[j] = make_float4(c.w == 0.0 ? 1073741824.0f
-                                    : (float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
-                         (float)(f * (ey * d2z - ez * d2y)), (float)(f * (ez * d2x - ex * d2z)),
-                         (float)(f * (ex * d2y - ey * d2x)));
+  uint32_t m = 0;
+  if (j < cnt) {
+    const double4 c = rec[base + j];
+    const uint32_t mx = abs_hi_word(c.x - gx), my = abs_hi_word(c.y - gy),
+                   mz = abs_hi_word(c.z - gz);
+    m = max(max(mx, my), mz);
+  }
+  m = wave_max_uniform_u32(m);
+  if ((j & (kWave - 1)) == 0) sm.tmax[j / kWave] = m;
+}
+
+__device__ __forceinline__ float strip_tile_store(const double4* __restrict__ rec, int64_t base,
+                                                  int cnt, double gx, double gy, double gz,
+                                                  int er, StripSmem& sm) {
+  uint32_t mt = sm.tmax[0];
+#pragma unroll
+  for (int w = 1; w < kBlockThreads / kWave; ++w) mt = max(mt, sm.tmax[w]);
+  mt = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt);
+  const int ef = frame_exponent_hi(mt) + 1;
+  const int E = ef > er ? ef : er;
+  const double sc = pow2_f64(2 - E);
+  const int j = threadIdx.x;
+  if (j < cnt) {
+    const double4 c = rec[base + j];
+    const double4 p = rec[base + (j >= 1 ? j - 1 : j)];
+    const double4 pp = rec[base + (j >= 2 ? j - 2 : j)];
+    const float cx = (float)((c.x - gx) * sc), cy = (float)((c.y - gy) * sc),
+                cz = (float)((c.z - gz) * sc);
+    const float px = (float)((p.x - gx) * sc), py = (float)((p.y - gy) * sc),
+                pz = (float)((p.z - gz) * sc);
+    const float qx = (float)((pp.x - gx) * sc), qy = (float)((pp.y - gy) * sc),
+                qz = (float)((pp.z - gz) * sc);
+    // FP64 on the rounded corners: w_k - w_{k-1}, w_k - w_{k-2}, w_{k-1} - w_{k-2}
+    const double d1x = (double)cx - px, d1y = (double)cy - py, d1z = (double)cz - pz;
+    const double d2x = (double)cx - qx, d2y = (double)cy - qy, d2z = (double)cz - qz;
+    const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
+    // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag: the
+    // lane's dot product with it is det
+    // A record that closes no face (flag 0) gets N = 0 and +2^30 in place of
+    // -|w_k - w_{k-2}|^2 / 2: det is then 0 exactly and c.a ~ 2^30 makes dp
+    // and den positive and finite (see strip_record), so both fast paths give
+    // it angle 0 (0 / den, and a real factor den + 0i of a group product).
+    const double f = c.w;
+    sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
+    sm.r1[j] = make_float4(c.w == 0.0 ? 1073741824.0f
+                                      : (float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
+                           (float)(f * (ey * d2z - ez * d2y)), (float)(f * (ez * d2x - ex * d2z)),
+                           (float)(f * (ex * d2y - ey * d2x)));
+  }
+  return pow2_f32(er - E);
 }
 
 // FP64 re-evaluation of face (j-2, j-1, j) from the LDS corners, oriented.
@@ -125,7 +166,7 @@ __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& 
 // Record j of the tile closes face (P, Q, R) (P and Q from the two previous
 // records, R computed here): its FP32 geometry. Every record takes the same
 // instructions: one that closes no face (flag 0: the first two of a strip,
-// tile padding) was staged (stage_strip_tile) with N = 0 and c.a = +2^30,
+// tile padding) was staged (strip_tile_store) with N = 0 and c.a = +2^30,
 // so det = 0 exactly and, with Q's length above ~1e-9 of the mesh's scale,
 // dp and den are positive: its angle is 0 on both fast paths.
 // boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
@@ -273,10 +314,11 @@ __device__ __forceinline__ int32_t thread_index(int32_t base) {
 }
 
 // Block-local origin: centre of the bbox of the block's valid queries (the
-// same reduction as the face-record kernels). Begins and ends with a barrier.
+// same reduction as the face-record kernels), and er = frame_exponent of the
+// bbox's largest side (|query - g| < 2^er). Begins and ends with a barrier.
 __device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double Oy, double Oz,
                                                    StripSmem& sm, double& gx, double& gy,
-                                                   double& gz) {
+                                                   double& gz, int& er) {
   const int tid = threadIdx.x;
   double lo0 = valid ? Ox : __builtin_inf(), hi0 = valid ? Ox : -__builtin_inf();
   double lo1 = valid ? Oy : __builtin_inf(), hi1 = valid ? Oy : -__builtin_inf();
@@ -300,22 +342,29 @@ __device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double
   gx = uniform_f64(0.5 * (lo0 + hi0));
   gy = uniform_f64(0.5 * (lo1 + hi1));
   gz = uniform_f64(0.5 * (lo2 + hi2));
+  er = frame_exponent(uniform_f64(fmax(fmax(hi0 - lo0, hi1 - lo1), hi2 - lo2)));
   __syncthreads();
 }
 
-// Unscaled angle sum of one query (block-local FP32 offset o) over strip
-// tiles [t_begin, t_end). All threads of the block call it together.
+// Unscaled angle sum of one query over strip tiles [t_begin, t_end); o is
+// the query's block-local FP32 offset in the frame 2^(2 - er)
+// (strip_block_origin), rescaled into each tile's frame (strip_tile_store).
+// All threads of the block call it together.
 __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, int64_t t_begin,
                                               int64_t t_end, double gx, double gy, double gz,
-                                              float ox, float oy, float oz, float kappa,
-                                              StripSmem& sm) {
+                                              int er, float ox0, float oy0, float oz0,
+                                              float kappa, StripSmem& sm) {
   double acc = 0.0;
   VSlot A{0.f, 0.f, 0.f, 0.f, 0.f}, B = A, C = A;
   float s_carry = 0.0f;
   for (int64_t t = t_begin; t < t_end; ++t) {
+    strip_tile_extent(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
+    __syncthreads();  // the previous tile is swept; sm.tmax holds this tile's extents
+    const float f = strip_tile_store(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz,
+                                     er, sm);
     __syncthreads();
-    stage_strip_tile(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
-    __syncthreads();
+    // exact: a power of two no larger than 1
+    const float ox = ox0 * f, oy = oy0 * f, oz = oz0 * f;
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     // The record offset is wave-uniform, but an LDS read takes its address
     // from a VGPR: a VGPR byte offset advanced once per three records and
@@ -351,14 +400,16 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
   const double Oz = valid ? q[3 * (int64_t)slot + 2] : 0.0;
 
   double gx, gy, gz;
-  strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz);
-  const float ox = valid ? (float)(Ox - gx) : 0.0f;
-  const float oy = valid ? (float)(Oy - gy) : 0.0f;
-  const float oz = valid ? (float)(Oz - gz) : 0.0f;
+  int er;
+  strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz, er);
+  const double so = pow2_f64(2 - er);
+  const float ox = valid ? (float)((Ox - gx) * so) : 0.0f;
+  const float oy = valid ? (float)((Oy - gy) * so) : 0.0f;
+  const float oz = valid ? (float)((Oz - gz) * so) : 0.0f;
 
   const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
   const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
-  const double acc = strip_sweep(rec, t_begin, t_end, gx, gy, gz, ox, oy, oz, kappa, sm);
+  const double acc = strip_sweep(rec, t_begin, t_end, gx, gy, gz, er, ox, oy, oz, kappa, sm);
   const int32_t oslot = (int32_t)blockIdx.x * kBlockThreads + thread_index(base);
   if (oslot < nq) {
     if (splits == 1)
@@ -399,14 +450,16 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
     const double Oy = valid ? q[3 * slot + 1] : 0.0;
     const double Oz = valid ? q[3 * slot + 2] : 0.0;
     double gx, gy, gz;
-    strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz);
-    const float ox = valid ? (float)(Ox - gx) : 0.0f;
-    const float oy = valid ? (float)(Oy - gy) : 0.0f;
-    const float oz = valid ? (float)(Oz - gz) : 0.0f;
+    int er;
+    strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz, er);
+    const double so = pow2_f64(2 - er);
+    const float ox = valid ? (float)((Ox - gx) * so) : 0.0f;
+    const float oy = valid ? (float)((Oy - gy) * so) : 0.0f;
+    const float oz = valid ? (float)((Oz - gz) * so) : 0.0f;
     const int64_t t0 = m.face_off + (int64_t)item.chunk * chunk_tiles;
     int64_t t1 = t0 + chunk_tiles;
     if (t1 > m.face_off + m.nf) t1 = m.face_off + m.nf;
-    const double acc = strip_sweep(rec, t0, t1, gx, gy, gz, ox, oy, oz, kappa, sm);
+    const double acc = strip_sweep(rec, t0, t1, gx, gy, gz, er, ox, oy, oz, kappa, sm);
     const int32_t local = item.qtile * kBlockThreads + thread_index(base);  // slot - query_off
     if (local < m.nq) {
       const int64_t oslot = m.query_off + local;

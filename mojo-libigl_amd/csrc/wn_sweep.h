@@ -20,6 +20,7 @@ struct SweepF64Smem {
 struct SweepF32Smem {
   FaceRec32 face[kFaceTile];
   double red[6][kBlockThreads / kWave];
+  uint32_t tmax[kBlockThreads / kWave];  // per wave: abs_hi_word max |corner - g| of the next tile
 };
 
 template <int QPL>
@@ -54,13 +55,14 @@ __device__ __forceinline__ void sweep_f64(const double* __restrict__ tri, int64_
   __syncthreads();
 }
 
-// Centre of the bbox of the block's valid queries (valid[k] false = ignored).
+// Centre of the bbox of the block's valid queries (valid[k] false = ignored)
+// and er = frame_exponent of the bbox's largest side (|query - g| < 2^er).
 // At least one thread of the block must hold a valid query.
 template <int QPL>
 __device__ __forceinline__ void block_origin(const double (&Ox)[QPL], const double (&Oy)[QPL],
                                              const double (&Oz)[QPL], const bool (&valid)[QPL],
                                              SweepF32Smem& sm, double& gx, double& gy,
-                                             double& gz) {
+                                             double& gz, int& er) {
   const int tid = threadIdx.x;
   double lo0 = __builtin_inf(), lo1 = __builtin_inf(), lo2 = __builtin_inf();
   double hi0 = -__builtin_inf(), hi1 = -__builtin_inf(), hi2 = -__builtin_inf();
@@ -89,6 +91,7 @@ __device__ __forceinline__ void block_origin(const double (&Ox)[QPL], const doub
   gx = 0.5 * (lo0 + hi0);
   gy = 0.5 * (lo1 + hi1);
   gz = 0.5 * (lo2 + hi2);
+  er = frame_exponent(fmax(fmax(hi0 - lo0, hi1 - lo1), hi2 - lo2));
 }
 
 // One face of a tile for QPL queries: its FP32 angle is added to part[k],
@@ -153,24 +156,53 @@ __device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const Face
   }
 }
 
+// Faces [f0, f1) for QPL queries per lane; o (ox0 ..) the queries' block-local
+// FP32 offsets in the frame 2^(2 - er) (block_origin). Each tile is staged in
+// its own power-of-two frame sc = 2^(2 - E), 2^E bounding |corner - g| over
+// the tile and |query - g| over the block, and the offsets are rescaled into
+// it by the exact factor 2^(er - E) <= 1 (see frame_exponent).
 template <int QPL>
 __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64_t stride,
                                            int64_t f0, int64_t f1, double gx, double gy,
-                                           double gz, const float (&ox)[QPL],
-                                           const float (&oy)[QPL], const float (&oz)[QPL],
+                                           double gz, int er, const float (&ox0)[QPL],
+                                           const float (&oy0)[QPL], const float (&oz0)[QPL],
                                            float kappa, double (&acc)[QPL], SweepF32Smem& sm) {
   const int tid = threadIdx.x;
   for (int64_t t0 = f0; t0 < f1; t0 += kFaceTile) {
     const int cnt = (int)(f1 - t0 < kFaceTile ? f1 - t0 : kFaceTile);
-    __syncthreads();
+    // corners relative to g in FP64 (exact rounding of the difference, as
+    // before any scaling); threads without a face hold zeros
+    double d[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (tid < cnt) {
       const int64_t f = t0 + tid;
-      sm.face[tid] = make_face_rec32(tri[f], tri[stride + f], tri[2 * stride + f],
-                                     tri[3 * stride + f], tri[4 * stride + f],
-                                     tri[5 * stride + f], tri[6 * stride + f],
-                                     tri[7 * stride + f], tri[8 * stride + f], gx, gy, gz);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d[k] = tri[k * stride + f] - (k % 3 == 0 ? gx : k % 3 == 1 ? gy : gz);
     }
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m = max(m, abs_hi_word(d[k]));
+    m = wave_max_uniform_u32(m);
+    if ((tid & (kWave - 1)) == 0) sm.tmax[tid / kWave] = m;
+    __syncthreads();  // the previous tile is swept; sm.tmax holds this tile's extents
+    uint32_t mt = sm.tmax[0];
+#pragma unroll
+    for (int w = 1; w < kBlockThreads / kWave; ++w) mt = max(mt, sm.tmax[w]);
+    mt = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt);
+    const int ef = frame_exponent_hi(mt) + 1;
+    const int E = ef > er ? ef : er;
+    const double sc = pow2_f64(2 - E);
+    if (tid < cnt)
+      sm.face[tid] = make_face_rec32(d[0] * sc, d[1] * sc, d[2] * sc, d[3] * sc, d[4] * sc,
+                                     d[5] * sc, d[6] * sc, d[7] * sc, d[8] * sc, 0.0, 0.0, 0.0);
     __syncthreads();
+    const float fs = pow2_f32(er - E);  // exact rescaling: a power of two <= 1
+    float ox[QPL], oy[QPL], oz[QPL];
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      ox[k] = ox0[k] * fs;
+      oy[k] = oy0[k] * fs;
+      oz[k] = oz0[k] * fs;
+    }
     // the angles of three consecutive faces are summed in FP32 (the
     // argument of their product, or three FP32 angles added: at most a few
     // ulps of their magnitudes, the bound of a compensated sum), two such

@@ -4,13 +4,17 @@ against the oracle evaluated on the same transformed inputs:
 * translation far from the origin (1e4 .. 1e6): the FP32-compensated paths
   keep their bar only because each block works in FP32 offsets from its own
   origin; plain FP32 coordinates at 1e6 would be off by ~0.06;
-* uniform scale (1e-5, 1e-3, 1e3, 5e3, 1e4, 2e4, 1e6: the FP32 product of a
-  group of faces' den underflows or overflows at the ends, so the fast
-  paths' smaller groups and per-face fallback are exercised; at 5e3..2e4 it
-  lands between 2^126 and the FP32 maximum, where its reciprocal is no
-  longer a normal number), a rotation, a face permutation, a cyclic
-  permutation of every face's corners (w unchanged), and a swap of two
-  corners of every face (w negated).
+* uniform scale (1e-5, 1e-3, 1e3, 5e3, 1e4, 2e4, 1e6: unscaled, the FP32
+  product of a group of faces' den would underflow or overflow at the ends,
+  or land between 2^126 and the FP32 maximum, where its reciprocal is no
+  longer a normal number; the kernels stage every tile in a power-of-two
+  frame that keeps the products in range), a rotation, a face permutation,
+  a cyclic permutation of every face's corners (w unchanged), and a swap of
+  two corners of every face (w negated);
+* a power-of-two scale (2^-40 .. 2^40) of the whole input leaves the
+  FP32-compensated results bit for bit unchanged on a closed mesh, and
+  within FP64 rounding (1e-15) on an open one, whose boundary correction
+  adds FP64 closed forms through the library atan2.
 
 Paths: FP64 face kernel (wn_winding_gpu_vf, the path of igl.winding_number),
 FP32c face kernel, FP32c strip kernel (igl.Mesh), FP64 strip kernel
@@ -102,6 +106,34 @@ def test_invariance(require_gpu, base, path, name):
     print(f"{path} {name}: max|w - oracle| {err:.3e}, oracle drift {drift:.3e}", flush=True)
     assert err <= BARS[path]
     assert drift <= 1e-12
+
+
+@pytest.mark.parametrize("path", ["fp32c_face", "fp32c_strip"])
+@pytest.mark.parametrize("mesh", ["closed", "open"])
+def test_power_of_two_scale_is_exact(require_gpu, path, mesh):
+    """Every FP32 quantity of the FP32-compensated kernels is computed in a
+    power-of-two frame of its tile, and FP32 rounding commutes with a
+    power-of-two factor, so scaling V and Q by 2^k leaves every angle, and w,
+    unchanged bit for bit (for coordinates that stay normal numbers). The
+    open mesh adds the boundary correction, whose FP64 closed forms call the
+    library atan2: equal there to FP64 rounding."""
+    if mesh == "closed":
+        V, F, Q = W.config1()
+    else:
+        V, F = W.bench_grid(40)
+        Q = W.padded_box_queries(V, 1024, np.random.default_rng(5))
+    w0 = run(path, V, F, Q)
+    ref = oracle.winding(V, F, Q)
+    assert np.abs(w0 - ref).max() <= BARS[path]
+    for k in (-40, -12, 13, 40):
+        s = 2.0 ** k
+        w = run(path, V * s, F, Q * s)
+        diff = np.abs(w - w0).max()
+        print(f"{path} {mesh} 2^{k}: max|w - w(2^0)| {diff:.3e}", flush=True)
+        if mesh == "closed":
+            assert np.array_equal(w, w0), (k, diff)
+        else:
+            assert diff <= 1e-15, (k, diff)
 
 
 def soup(seed):
