@@ -267,6 +267,12 @@ int64_t host_path(const HostMesh& mesh, const double* q, double* dst, int64_t nf
     }
     ctx.cap = need;
   }
+  // failures after the first copy is queued wait for the stream, so that no
+  // copy still reads or writes the caller's host buffers after the return
+  auto fail_queued = [&ctx](int64_t code, const std::string& msg) {
+    (void)hipStreamSynchronize(ctx.stream);
+    return fail(code, msg);
+  };
   char* base = (char*)ctx.buf;
   double* d_tri = (double*)base;
   double* d_q = (double*)(base + tri_b);
@@ -276,7 +282,7 @@ int64_t host_path(const HostMesh& mesh, const double* q, double* dst, int64_t nf
   if (nf > 0 && !gather) {
     e = hipMemcpyAsync(d_tri, mesh.tri, (size_t)nf * 9 * sizeof(double), hipMemcpyHostToDevice,
                        ctx.stream);
-    if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy tri", e));
+    if (e != hipSuccess) return fail_queued(kErrHip, hip_msg("hipMemcpy tri", e));
   } else if (nf > 0) {
     char* d_v = base + tri_b + q_b + dst_b + ws_b;
     char* d_f = d_v + v_b;
@@ -285,27 +291,19 @@ int64_t host_path(const HostMesh& mesh, const double* q, double* dst, int64_t nf
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_f, mesh.F, (size_t)nf * 3 * (mesh.fdtype == 0 ? 8 : 4),
                          hipMemcpyHostToDevice, ctx.stream);
-    if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy V/F", e));
+    if (e != hipSuccess) return fail_queued(kErrHip, hip_msg("hipMemcpy V/F", e));
     // indices were checked above: the gather reads only inside V
     e = wn::pack_soa_launch(d_v, d_f, mesh.nv, nf, 0, mesh.fdtype, d_tri, nullptr, ctx.stream);
-    if (e != hipSuccess) {
-      (void)hipStreamSynchronize(ctx.stream);
-      return fail(kErrHip, hip_msg("pack launch", e));
-    }
+    if (e != hipSuccess) return fail_queued(kErrHip, hip_msg("pack launch", e));
   }
   e = hipMemcpyAsync(d_q, q, (size_t)nq * 3 * sizeof(double), hipMemcpyHostToDevice, ctx.stream);
-  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy q", e));
-  if (injected("launch")) {
-    (void)hipStreamSynchronize(ctx.stream);
-    return fail(kErrInjected, "injected launch failure (WN_FAULT=launch)");
-  }
+  if (e != hipSuccess) return fail_queued(kErrHip, hip_msg("hipMemcpy q", e));
+  if (injected("launch"))
+    return fail_queued(kErrInjected, "injected launch failure (WN_FAULT=launch)");
   e = wn::fwd_launch(d_tri, d_q, d_dst, nf, nq, cfg, d_ws, ws_b, ctx.stream);
-  if (e != hipSuccess) {
-    (void)hipStreamSynchronize(ctx.stream);
-    return fail(kErrHip, hip_msg("kernel launch", e));
-  }
+  if (e != hipSuccess) return fail_queued(kErrHip, hip_msg("kernel launch", e));
   e = hipMemcpyAsync(dst, d_dst, (size_t)nq * sizeof(double), hipMemcpyDeviceToHost, ctx.stream);
-  if (e != hipSuccess) return fail(kErrHip, hip_msg("hipMemcpy dst", e));
+  if (e != hipSuccess) return fail_queued(kErrHip, hip_msg("hipMemcpy dst", e));
   e = hipStreamSynchronize(ctx.stream);
   if (e != hipSuccess) return fail(kErrHip, hip_msg("hipStreamSynchronize", e));
   if (injected("sync")) return fail(kErrInjected, "injected synchronisation failure (WN_FAULT=sync)");
