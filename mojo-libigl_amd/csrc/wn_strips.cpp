@@ -21,11 +21,13 @@
 // matching is a radix sort of packed edge keys. From each
 // unvisited face (in index order) the walk tries the three rotations of its
 // corners and keeps the longest strip of unvisited faces, always crossing the
-// edge (w_{k-1}, w_k).
+// edge (w_{k-1}, w_k). The key build and the matching of large meshes run
+// on several threads (host_threads) with the serial result.
 
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -42,6 +44,22 @@ int64_t corner(const FT* F, int64_t f, int c) {
   return (int64_t)F[3 * f + c];
 }
 
+// Host threads for a pass over nf faces: one below 131,072, else up to 16.
+int host_threads(int64_t nf) {
+  if (nf < 131072) return 1;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::min<unsigned>(16u, hw);
+}
+
+// fn(t) for t in [0, T), on T - 1 new threads and the caller.
+template <typename Fn>
+void run_threads(int T, Fn&& fn) {
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(fn, t);
+  fn(0);
+  for (auto& th : pool) th.join();
+}
+
 template <typename FT>
 struct Stripifier {
   const FT* F;
@@ -53,28 +71,57 @@ struct Stripifier {
   // key (lo << 32 | hi); the payload is (face << 2 | local edge). Entries are
   // generated in face order and the radix sort is stable, so each edge's
   // faces stay in ascending order, as a sort by (lo, hi, face) leaves them.
+  // Large meshes use several threads (host_threads) for the key build and
+  // the matching: keys are written in face order (per-thread face ranges at
+  // counted offsets) and the sorted runs are split at run starts, so the
+  // result does not depend on the thread count.
   Stripifier(const FT* F_, int64_t nf_) : F(F_), nf(nf_), nbr(3 * nf_, -1), used(nf_, 0) {
-    std::vector<uint64_t> key, pay;
-    key.reserve((size_t)(3 * nf));
-    pay.reserve((size_t)(3 * nf));
-    for (int64_t f = 0; f < nf; ++f)
-      for (int k = 0; k < 3; ++k) {
-        const int64_t a = corner(F, f, k), b = corner(F, f, (k + 1) % 3);
-        if (a == b) continue;
-        key.push_back((uint64_t)std::min(a, b) << 32 | (uint64_t)std::max(a, b));
-        pay.push_back((uint64_t)f << 2 | (uint64_t)k);
-      }
+    const int T = host_threads(nf);
+    std::vector<int64_t> fcut(T + 1);
+    for (int t = 0; t <= T; ++t) fcut[t] = nf * t / T;
+    std::vector<size_t> at(T + 1, 0);
+    run_threads(T, [&](int t) {
+      size_t c = 0;
+      for (int64_t f = fcut[t]; f < fcut[t + 1]; ++f)
+        for (int k = 0; k < 3; ++k) c += corner(F, f, k) != corner(F, f, (k + 1) % 3);
+      at[t + 1] = c;
+    });
+    for (int t = 0; t < T; ++t) at[t + 1] += at[t];
+    std::vector<uint64_t> key(at[T]), pay(at[T]);
+    run_threads(T, [&](int t) {
+      size_t i = at[t];
+      for (int64_t f = fcut[t]; f < fcut[t + 1]; ++f)
+        for (int k = 0; k < 3; ++k) {
+          const int64_t a = corner(F, f, k), b = corner(F, f, (k + 1) % 3);
+          if (a == b) continue;
+          key[i] = (uint64_t)std::min(a, b) << 32 | (uint64_t)std::max(a, b);
+          pay[i] = (uint64_t)f << 2 | (uint64_t)k;
+          ++i;
+        }
+    });
     radix_sort(key, &pay);
-    for (size_t i = 0; i < key.size();) {
-      size_t j = i;
-      while (j < key.size() && key[j] == key[i]) ++j;
-      if (j - i == 2) {
-        const int64_t f0 = (int64_t)(pay[i] >> 2), f1 = (int64_t)(pay[i + 1] >> 2);
-        nbr[3 * f0 + (int64_t)(pay[i] & 3)] = f1;
-        nbr[3 * f1 + (int64_t)(pay[i + 1] & 3)] = f0;
-      }
-      i = j;
+    // runs of equal keys, split between threads at run starts; an edge
+    // shared by exactly two faces links them (each (face, edge) slot is
+    // written by the one thread holding its run)
+    const size_t n = key.size();
+    std::vector<size_t> kcut(T + 1);
+    for (int t = 0; t <= T; ++t) {
+      size_t c = n * (size_t)t / (size_t)T;
+      while (c > 0 && c < n && key[c] == key[c - 1]) ++c;
+      kcut[t] = c;
     }
+    run_threads(T, [&](int t) {
+      for (size_t i = kcut[t]; i < kcut[t + 1];) {
+        size_t j = i;
+        while (j < n && key[j] == key[i]) ++j;
+        if (j - i == 2) {
+          const int64_t f0 = (int64_t)(pay[i] >> 2), f1 = (int64_t)(pay[i + 1] >> 2);
+          nbr[3 * f0 + (int64_t)(pay[i] & 3)] = f1;
+          nbr[3 * f1 + (int64_t)(pay[i + 1] & 3)] = f0;
+        }
+        i = j;
+      }
+    });
   }
 
   // Local edge index of edge {a, b} in face f, -1 if absent.

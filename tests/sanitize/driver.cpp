@@ -143,8 +143,10 @@ void winding_sweep() {
 }
 
 void strips_and_edges() {
+  // the torus (358,400 faces) is above the stripifier's threaded threshold
+  // (131,072 faces) and its 1,075,200 edge keys above the radix sort's (2^20)
   for (int which = 0; which < 2; ++which) {
-    const Mesh m = which ? open_grid(60) : torus(64, 50);
+    const Mesh m = which ? open_grid(60) : torus(640, 280);
     const int64_t nf = m.nf();
     std::vector<int32_t> F32(m.F.begin(), m.F.end());
     std::vector<double> rec64, rec32;
