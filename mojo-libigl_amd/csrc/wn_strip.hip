@@ -36,6 +36,10 @@
 // rebuilt from the LDS tile, so the fast path keeps no per-face values live.
 // Squared lengths and N are computed in FP64 from the rounded FP32 corners.
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "wn_device.h"
 #include "wn_internal.h"
 #include "wn_sweep.h"
@@ -55,6 +59,22 @@ struct VSlot {
   float ax, ay, az, l2, l;
 };
 
+// First thread of the calling wave, in a scalar register (all lanes active).
+__device__ __forceinline__ int32_t wave_base() {
+  return __builtin_amdgcn_readfirstlane((int32_t)threadIdx.x) & ~(kWave - 1);
+}
+
+// threadIdx.x rebuilt from the wave base and the lane's position in the
+// wave, computed where it is called: an output index formed from it after a
+// sweep needs no vector register kept live across the sweep. (The mbcnt pair
+// sits in a volatile asm so the compiler cannot hoist it above the sweep.)
+__device__ __forceinline__ int32_t thread_index(int32_t base) {
+  int32_t lane;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+  return base + lane;
+}
+
+
 // One tile's staging, in two steps around a block barrier. strip_tile_extent:
 // every thread of the block (all lanes: the wave reduction needs them) reads
 // its record j, and its wave's max over the tile of |corner - g| (infinity
@@ -67,8 +87,8 @@ struct VSlot {
 // into the same frame.
 __device__ __forceinline__ void strip_tile_extent(const double4* __restrict__ rec, int64_t base,
                                                   int cnt, double gx, double gy, double gz,
-                                                  StripSmem& sm) {
-  const int j = threadIdx.x;
+                                                  int32_t wbase, StripSmem& sm) {
+  const int j = thread_index(wbase);
   uint32_t m = 0;
   if (j < cnt) {
     const double4 c = rec[base + j];
@@ -82,7 +102,7 @@ __device__ __forceinline__ void strip_tile_extent(const double4* __restrict__ re
 
 __device__ __forceinline__ float strip_tile_store(const double4* __restrict__ rec, int64_t base,
                                                   int cnt, double gx, double gy, double gz,
-                                                  int er, StripSmem& sm) {
+                                                  int er, int32_t wbase, StripSmem& sm) {
   uint32_t mt = sm.tmax[0];
 #pragma unroll
   for (int w = 1; w < kBlockThreads / kWave; ++w) mt = max(mt, sm.tmax[w]);
@@ -90,7 +110,7 @@ __device__ __forceinline__ float strip_tile_store(const double4* __restrict__ re
   const int ef = frame_exponent_hi(mt) + 1;
   const int E = ef > er ? ef : er;
   const double sc = pow2_f64(2 - E);
-  const int j = threadIdx.x;
+  const int j = thread_index(wbase);
   if (j < cnt) {
     const double4 c = rec[base + j];
     const double4 p = rec[base + (j >= 1 ? j - 1 : j)];
@@ -298,21 +318,6 @@ __device__ __forceinline__ float strip_hexad(const StripSmem& sm,
          strip_triple_lds(sm, flags, j + 3, t2, m2, ox, oy, oz, kappa);
 }
 
-// First thread of the calling wave, in a scalar register (all lanes active).
-__device__ __forceinline__ int32_t wave_base() {
-  return __builtin_amdgcn_readfirstlane((int32_t)threadIdx.x) & ~(kWave - 1);
-}
-
-// threadIdx.x rebuilt from the wave base and the lane's position in the
-// wave, computed where it is called: an output index formed from it after a
-// sweep needs no vector register kept live across the sweep. (The mbcnt pair
-// sits in a volatile asm so the compiler cannot hoist it above the sweep.)
-__device__ __forceinline__ int32_t thread_index(int32_t base) {
-  int32_t lane;
-  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
-  return base + lane;
-}
-
 // Block-local origin: centre of the bbox of the block's valid queries (the
 // same reduction as the face-record kernels), and er = frame_exponent of the
 // bbox's largest side (|query - g| < 2^er). Begins and ends with a barrier.
@@ -349,19 +354,22 @@ __device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double
 // Unscaled angle sum of one query over strip tiles [t_begin, t_end); o is
 // the query's block-local FP32 offset in the frame 2^(2 - er)
 // (strip_block_origin), rescaled into each tile's frame (strip_tile_store).
-// All threads of the block call it together.
+// All threads of the block call it together; wbase = wave_base(). The
+// staging rebuilds the thread index from wbase on every tile (thread_index),
+// so no per-lane index or address stays live across the sweep (with the
+// sweep's 64 registers, such values were spilled to scratch per block).
 __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, int64_t t_begin,
                                               int64_t t_end, double gx, double gy, double gz,
                                               int er, float ox0, float oy0, float oz0,
-                                              float kappa, StripSmem& sm) {
+                                              float kappa, int32_t wbase, StripSmem& sm) {
   double acc = 0.0;
   VSlot A{0.f, 0.f, 0.f, 0.f, 0.f}, B = A, C = A;
   float s_carry = 0.0f;
   for (int64_t t = t_begin; t < t_end; ++t) {
-    strip_tile_extent(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, sm);
+    strip_tile_extent(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz, wbase, sm);
     __syncthreads();  // the previous tile is swept; sm.tmax holds this tile's extents
     const float f = strip_tile_store(rec, t * kStripTileRecords, kStripTileRecords, gx, gy, gz,
-                                     er, sm);
+                                     er, wbase, sm);
     __syncthreads();
     // exact: a power of two no larger than 1
     const float ox = ox0 * f, oy = oy0 * f, oz = oz0 * f;
@@ -384,15 +392,34 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
   return acc;
 }
 
+// One block per query block, or per (query block, face split) in the tail.
+// Blocks [0, whole) sweep query block b over every tile and write w through
+// perm. The remaining query blocks (the tail: one round of the device's
+// resident blocks) are split over `splits` tile ranges, split-major: block
+// whole + l takes split l / ntail_qb of tail query block l % ntail_qb, so
+// that the blocks running together sweep the same tiles (the order of a
+// (query block, split) grid; measured 4 % faster than dispatching one query
+// block's splits together). Tail blocks write unscaled partials
+// partial[split * ntail + (slot - 256 whole)], ntail = nq - 256 whole, summed
+// in split order afterwards. The whole blocks fill the first round and the
+// short split blocks the last one: the end is as even as with every query
+// block split, for half the partials.
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8))) void fwd_strip_kernel(
     const double4* __restrict__ rec, int64_t ntiles, const double* __restrict__ q,
-    const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nq,
-    int64_t tiles_per_split, int splits, float kappa) {
+    const int32_t* __restrict__ perm, double* __restrict__ dst, double* __restrict__ partial,
+    int64_t nq, int32_t whole, int64_t tiles_per_split, int splits, float kappa) {
   __shared__ StripSmem sm;
+  const int32_t nqb = (int32_t)((nq + kBlockThreads - 1) / kBlockThreads);
+  int32_t qb = (int32_t)blockIdx.x, split = -1;
+  if (qb >= whole) {
+    const int32_t l = qb - whole, tail = nqb - whole;
+    qb = whole + l % tail;
+    split = l / tail;
+  }
   // 32-bit slot (nq < 2^31), and the output index read only after the sweep:
   // nothing 64-bit stays live across the sweep (it would be spilled)
   const int32_t base = wave_base();
-  const int32_t slot = (int32_t)blockIdx.x * kBlockThreads + (int32_t)threadIdx.x;
+  const int32_t slot = qb * kBlockThreads + (int32_t)threadIdx.x;
   const bool valid = slot < nq;
   // q is in launch order (gathered through perm beforehand when sorted)
   const double Ox = valid ? q[3 * (int64_t)slot] : 0.0;
@@ -407,15 +434,19 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
   const float oy = valid ? (float)((Oy - gy) * so) : 0.0f;
   const float oz = valid ? (float)((Oz - gz) * so) : 0.0f;
 
-  const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
-  const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
-  const double acc = strip_sweep(rec, t_begin, t_end, gx, gy, gz, er, ox, oy, oz, kappa, sm);
-  const int32_t oslot = (int32_t)blockIdx.x * kBlockThreads + thread_index(base);
+  int64_t t_begin = 0, t_end = ntiles;
+  if (split >= 0) {
+    t_begin = (int64_t)split * tiles_per_split;
+    t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
+  }
+  const double acc = strip_sweep(rec, t_begin, t_end, gx, gy, gz, er, ox, oy, oz, kappa, base, sm);
+  const int32_t oslot = qb * kBlockThreads + thread_index(base);
   if (oslot < nq) {
-    if (splits == 1)
-      out[perm ? (int64_t)perm[oslot] : (int64_t)oslot] = acc * kInv2Pi;
+    if (split < 0)
+      dst[perm ? (int64_t)perm[oslot] : (int64_t)oslot] = acc * kInv2Pi;
     else
-      out[(int64_t)blockIdx.y * nq + oslot] = acc;
+      partial[(int64_t)split * (nq - (int64_t)whole * kBlockThreads) +
+              (oslot - whole * kBlockThreads)] = acc;
   }
 }
 
@@ -459,7 +490,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
     const int64_t t0 = m.face_off + (int64_t)item.chunk * chunk_tiles;
     int64_t t1 = t0 + chunk_tiles;
     if (t1 > m.face_off + m.nf) t1 = m.face_off + m.nf;
-    const double acc = strip_sweep(rec, t0, t1, gx, gy, gz, er, ox, oy, oz, kappa, sm);
+    const double acc = strip_sweep(rec, t0, t1, gx, gy, gz, er, ox, oy, oz, kappa, base, sm);
     const int32_t local = item.qtile * kBlockThreads + thread_index(base);  // slot - query_off
     if (local < m.nq) {
       const int64_t oslot = m.query_off + local;
@@ -483,6 +514,23 @@ int strip_resident_blocks() {
     return 0;
   }
   return cus * per_cu;
+}
+
+// strip_resident_blocks() of the current device, asked once per device.
+int strip_resident_blocks_cached() {
+  static std::mutex mu;
+  static std::map<int, int> by_device;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = by_device.find(dev);
+  if (it != by_device.end()) return it->second;
+  const int r = strip_resident_blocks();
+  by_device[dev] = r;
+  return r;
 }
 
 size_t strip_workspace_bytes(int64_t nq, int splits, int sort, int64_t nbe) {
@@ -523,15 +571,23 @@ hipError_t strip_launch(const double4* rec, int64_t nrec, const double* q, doubl
   if (ntiles == 0) return fill_launch(dst, nq, 0.0, stream);
   const int64_t tps = (ntiles + splits - 1) / splits;
   const int used_splits = (int)((ntiles + tps - 1) / tps);
-  const dim3 grid((unsigned)((nq + kBlockThreads - 1) / kBlockThreads), (unsigned)used_splits);
+  // whole query blocks up to one round of resident blocks before the end;
+  // the last round's query blocks are split (none when there is one split)
+  const int64_t nqb = (nq + kBlockThreads - 1) / kBlockThreads;
+  const int64_t resident = std::max<int64_t>(strip_resident_blocks_cached(), 1);
+  const int64_t whole = used_splits == 1 ? nqb : std::max<int64_t>(0, nqb - resident);
+  const int64_t tail_qb = nqb - whole;
+  const int64_t nblocks = whole + (int64_t)used_splits * tail_qb;
   timer_begin(stream);
-  fwd_strip_kernel<<<grid, kBlockThreads, 0, stream>>>(rec, ntiles, q, perm,
-                                                       used_splits > 1 ? partial : dst, nq, tps,
-                                                       used_splits, kappa);
+  fwd_strip_kernel<<<dim3((unsigned)nblocks), kBlockThreads, 0, stream>>>(
+      rec, ntiles, q, perm, dst, partial, nq, (int32_t)whole, tps, used_splits, kappa);
   hipError_t e = hipGetLastError();
   timer_end(stream);
-  if (e == hipSuccess && used_splits > 1)
-    e = reduce_splits_launch(partial, perm, dst, nq, used_splits, stream);
+  if (e == hipSuccess && tail_qb > 0) {
+    const int64_t off = whole * kBlockThreads;
+    e = reduce_splits_launch(partial, perm ? perm + off : nullptr, perm ? dst : dst + off,
+                             nq - off, used_splits, stream);
+  }
   if (e != hipSuccess || nbe <= 0) return e;
   // boundary correction of an open mesh, for the same query blocks
   RibbonSrc src = *boundary;
