@@ -306,7 +306,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
 //    FP32 record (pair_rec32_f64).
 // --------------------------------------------------------------------------
 template <int QPL>
-__global__ __launch_bounds__(kBlockThreads) void fwd_f32c_kernel(
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(QPL == 1 ? 8 : 4))) void fwd_f32c_kernel(
     const double* __restrict__ tri, const double* __restrict__ q,
     const int32_t* __restrict__ perm, double* __restrict__ out, int64_t nf,
     int64_t nq, int64_t faces_per_split, int splits, float kappa) {

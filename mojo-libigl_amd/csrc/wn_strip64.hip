@@ -158,7 +158,8 @@ __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ r
   return acc + acc_slow;
 }
 
-// 6 waves per SIMD: 80 VGPRs, no spills
+// 6 waves per SIMD: 80 VGPRs, no spills on the fast path (the slow-path pair
+// is a call, around which registers are saved)
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6))) void fwd_strip64_kernel(
     const StripRec64* __restrict__ rec64, const double4* __restrict__ rec, int64_t ntiles,
     const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ out,
