@@ -375,16 +375,19 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     const float ox = ox0 * f, oy = oy0 * f, oz = oz0 * f;
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     // The record offset is wave-uniform, but an LDS read takes its address
-    // from a VGPR: a VGPR byte offset advanced once per three records and
-    // read with immediate offsets.
+    // from a VGPR: a VGPR byte offset advanced once per six records and read
+    // with immediate offsets. (The empty asm after the increment keeps it one
+    // VGPR updated in place: placed before the use, it cost a copy per
+    // iteration.)
     int boff = 0;
+    asm volatile("" : "+v"(boff));
     // groups of six records, each group's FP32 angle sum joined to the FP64
     // sum once; 255 = 42 x 6 + 3, the last three records as a group of three
     int j = 0;
     for (; j + 6 <= kStripTileRecords; j += 6) {
-      asm volatile("" : "+v"(boff));
       acc += (double)strip_hexad(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
       boff += 6 * (int)sizeof(float4);
+      asm volatile("" : "+v"(boff));
     }
     acc += (double)strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
   }
