@@ -91,6 +91,34 @@ def test_mesh_matches_face_kernel_full_size(require_gpu):
     assert d.max() <= 1e-6
 
 
+@pytest.mark.parametrize("nq,splits,sort", [
+    (1, 0, -1), (300, 3, -1), (524_287, 0, -1), (524_289, 0, -1), (533_765, 7, -1),
+    (700_001, 64, -1), (1_048_593, 0, -1), (1_048_593, 1, -1), (600_000, 0, 0)])
+def test_strip_layout_boundaries(require_gpu, nq, splits, sort):
+    """The strip kernel's block layout: whole query blocks for all but the
+    last round of resident blocks (2,048 on MI355X: 524,288 queries), that
+    round's query blocks split over tile ranges. Query counts around that
+    boundary, a partial last block, more splits than tiles (64 on this mesh's
+    21 tiles), one split, and unsorted queries (partials written without the
+    Morton permutation): FP32c within 1e-6 of the FP64 strip kernel (which
+    splits every query block), and 1e-6 of the oracle on a subsample."""
+    V, F, _ = W.config1()
+    rng = np.random.default_rng(nq)
+    Q = rng.uniform(-1.5, 1.5, size=(nq, 3))
+    m = _mesh(V, F)
+    try:
+        print(f"strip layout: nq {nq}, splits {splits}, sort {sort}", flush=True)
+        w32 = m.winding_number(Q, splits=splits, sort=sort)
+        w64 = m.winding_number(Q, precision="fp64")
+    finally:
+        m.close()
+    assert np.isfinite(w32).all()
+    assert np.abs(w32 - w64).max() <= 1e-6
+    idx = np.unique(np.concatenate([np.arange(min(nq, 64)), rng.integers(0, nq, 64), [nq - 1]]))
+    ref = oracle.winding(V, F, Q[idx])
+    assert np.abs(w32[idx] - ref).max() <= 1e-6
+
+
 def test_mesh_rejects_bad_faces(require_gpu):
     import igl
 
