@@ -156,6 +156,60 @@ __device__ __forceinline__ void face_triple_f32c(const FaceRec32& ra, const Face
   }
 }
 
+// Three faces whose product t (prod3_f32) and smallest dp the caller formed:
+// the FP32 sum of their angles by sum3_f32's test, else face by face as
+// face_step_f32c (from the records again).
+__device__ __forceinline__ float face_triple_of(const FaceRec32& ra, const FaceRec32& rb,
+                                                const FaceRec32& rc, const Prod3& t,
+                                                float dp_min, float ox, float oy, float oz,
+                                                float kappa) {
+  const AngleSum s = angle_sum_of(t.re, t.im);
+  if (__builtin_expect(wave_all_group(dp_min, s), 1)) return atan_sum_f32(s.x);
+  asm volatile("" ::);  // keep a real wave-uniform branch
+  float one[1] = {-0.0f};
+  const float qx[1] = {ox}, qy[1] = {oy}, qz[1] = {oz};
+  face_step_f32c<1>(ra, qx, qy, qz, kappa, one);
+  face_step_f32c<1>(rb, qx, qy, qz, kappa, one);
+  face_step_f32c<1>(rc, qx, qy, qz, kappa, one);
+  return one[0];
+}
+
+// Six faces for QPL queries: the FP32 sum of their angles into sum[k], as the
+// argument of the product of their (den + i det) when every lane meets
+// wave_all_sum6 (one reciprocal and one polynomial for the six), otherwise
+// as two groups of three (face_triple_of). Only the two triples' products
+// and smallest dp stay live; a group that fails rebuilds its faces' geometry
+// from the records.
+template <int QPL>
+__device__ __forceinline__ void face_hexad_f32c(const FaceRec32* rec, const float (&ox)[QPL],
+                                                const float (&oy)[QPL], const float (&oz)[QPL],
+                                                float kappa, float (&sum)[QPL]) {
+#pragma clang fp contract(off)  // the two triples' sums round before they add
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const PairGeom32 g1 = pair_geom32(rec[0].r0, rec[0].r1, rec[0].r2, rec[0].r3, ox[k], oy[k], oz[k]);
+    const PairGeom32 g2 = pair_geom32(rec[1].r0, rec[1].r1, rec[1].r2, rec[1].r3, ox[k], oy[k], oz[k]);
+    const PairGeom32 g3 = pair_geom32(rec[2].r0, rec[2].r1, rec[2].r2, rec[2].r3, ox[k], oy[k], oz[k]);
+    const Prod3 t1 = prod3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+    const float m1 = __builtin_fminf(__builtin_fminf(g1.dp, g2.dp), g3.dp);
+    const PairGeom32 g4 = pair_geom32(rec[3].r0, rec[3].r1, rec[3].r2, rec[3].r3, ox[k], oy[k], oz[k]);
+    const PairGeom32 g5 = pair_geom32(rec[4].r0, rec[4].r1, rec[4].r2, rec[4].r3, ox[k], oy[k], oz[k]);
+    const PairGeom32 g6 = pair_geom32(rec[5].r0, rec[5].r1, rec[5].r2, rec[5].r3, ox[k], oy[k], oz[k]);
+    const Prod3 t2 = prod3_f32(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
+    const float m2 = __builtin_fminf(__builtin_fminf(g4.dp, g5.dp), g6.dp);
+    const AngleSum s = sum6_f32(t1, t2);
+    float part;
+    if (__builtin_expect(wave_all_sum6(m1, m2, t1, t2, s), 1)) {
+      part = atan_sum_f32(s.x);
+    } else {
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      part = face_triple_of(rec[0], rec[1], rec[2], t1, m1, ox[k], oy[k], oz[k], kappa) +
+             face_triple_of(rec[3], rec[4], rec[5], t2, m2, ox[k], oy[k], oz[k], kappa);
+    }
+    sum[k] = part;
+  }
+}
+
 // Faces [f0, f1) for QPL queries per lane; o (ox0 ..) the queries' block-local
 // FP32 offsets in the frame 2^(2 - er) (block_origin). Each tile is staged in
 // its own power-of-two frame sc = 2^(2 - E), 2^E bounding |corner - g| over
@@ -203,19 +257,17 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
       oy[k] = oy0[k] * fs;
       oz[k] = oz0[k] * fs;
     }
-    // the angles of three consecutive faces are summed in FP32 (the
-    // argument of their product, or three FP32 angles added: at most a few
-    // ulps of their magnitudes, the bound of a compensated sum), two such
-    // sums are added in FP32 (one more rounding) and joined to the FP64 sum
+    // the angles of six consecutive faces are summed in FP32 (the argument
+    // of their product, or two groups of three: the argument of a triple's
+    // product or three FP32 angles added; at most a few ulps of their
+    // magnitudes, the bound of a compensated sum) and joined to the FP64 sum
     // once
     int j = 0;
     for (; j + 6 <= cnt; j += 6) {
-      float s1[QPL], s2[QPL];
-      face_triple_f32c<QPL>(sm.face[j], sm.face[j + 1], sm.face[j + 2], ox, oy, oz, kappa, s1);
-      face_triple_f32c<QPL>(sm.face[j + 3], sm.face[j + 4], sm.face[j + 5], ox, oy, oz, kappa,
-                            s2);
+      float s6[QPL];
+      face_hexad_f32c<QPL>(&sm.face[j], ox, oy, oz, kappa, s6);
 #pragma unroll
-      for (int k = 0; k < QPL; ++k) acc[k] += (double)(s1[k] + s2[k]);
+      for (int k = 0; k < QPL; ++k) acc[k] += (double)s6[k];
     }
     for (; j + 3 <= cnt; j += 3) {
       float s1[QPL];
