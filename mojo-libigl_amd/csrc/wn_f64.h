@@ -127,6 +127,46 @@ __device__ __forceinline__ Sum3F64 sum3_f64(double e1, double d1, double e2, dou
   return s;
 }
 
+// The product of three faces' (den + i det) (sum3_f64 without the
+// quotient), for groups of six: the product of two such triples.
+struct Prod3F64 {
+  double re, im;
+};
+
+__device__ __forceinline__ Prod3F64 prod3_f64(double e1, double d1, double e2, double d2,
+                                              double e3, double d3) {
+  const double re12 = __builtin_fma(-e1, e2, d1 * d2);
+  const double im12 = __builtin_fma(d1, e2, e1 * d2);
+  Prod3F64 t;
+  t.re = __builtin_fma(-im12, e3, re12 * d3);
+  t.im = __builtin_fma(re12, e3, im12 * d3);
+  return t;
+}
+
+// Lanes where v is a positive normal FP64 number: one v_cmp_class_f64.
+__device__ __forceinline__ uint64_t lanes_positive_normal_f64(double v) {
+  uint64_t m;
+  asm volatile("v_cmp_class_f64_e64 %0, %1, %2" : "=s"(m) : "v"(v), "s"(1 << 8));
+  return m;
+}
+
+// A group of faces (three, or six as two triples) on every active lane: the
+// smallest dp_k positive, re (and, for six, each triple's re: no
+// precision lost to underflow before the last product) a positive normal
+// number, |x| < 0.4 for x = Im / Re. The group's angle sum is then atan(x)
+// (the argument reasoning of sum3_f32 in wn_device.h: each angle within
+// +-pi/4, six within +-3pi/2, and an argument within +-0.39 is the sum). A
+// NaN dp, which a min skips, comes with a NaN den and re, which the class
+// test rejects.
+__device__ __forceinline__ bool wave_all_group_f64(double dp_min, double re_min, double re,
+                                                   double x) {
+  const uint64_t m1 = __builtin_amdgcn_ballot_w64(dp_min > 0.0);
+  const uint64_t m2 = lanes_positive_normal_f64(re);
+  const uint64_t m3 = lanes_positive_normal_f64(re_min);
+  const uint64_t m4 = __builtin_amdgcn_ballot_w64(__builtin_fabs(x) < 0.4);
+  return (m1 & m2 & m3 & m4) == __builtin_amdgcn_read_exec();
+}
+
 __device__ __forceinline__ bool wave_all_sum3_f64(double dp1, double dp2, double dp3,
                                                   const Sum3F64& s) {
   const uint64_t m1 = __builtin_amdgcn_ballot_w64(dp1 > 0.0);

@@ -190,6 +190,65 @@ __device__ __forceinline__ void face64_triple(const FaceRec64* __restrict__ r,
   }
 }
 
+// Three faces whose product t and smallest dp the caller formed: the group's
+// angle when every lane meets the group test, else face by face, each
+// face's geometry rebuilt from its record (the values the caller formed).
+// Out of line (the fallback of a failed group of six, rare): its registers
+// are not reserved for the sweep. Returns (fast-path sum, plain-formula sum).
+__device__ __noinline__ double2 face64_triple_of(const FaceRec64* __restrict__ r, double t_re,
+                                                 double t_im, double dp_min, double ox,
+                                                 double oy, double oz) {
+  double acc = 0.0, acc_slow = 0.0;
+  const double x = div_f64(t_im, t_re);  // used only where t_re is positive normal
+  if (__builtin_expect(wave_all_group_f64(dp_min, t_re, t_re, x), 1)) {
+    acc = atan_sum_f64(x);
+  } else {
+    asm volatile("" ::);  // keep a real wave-uniform branch
+    for (int i = 0; i < 3; ++i)
+      face64_single(r + i, face64_geom(r[i], ox, oy, oz), ox, oy, oz, acc, acc_slow);
+  }
+  return make_double2(acc, acc_slow);
+}
+
+// Six faces for the lane's QPL queries: when every lane meets the group test
+// their angles are taken together as the argument of the product of their
+// (den + i det), one quotient and one polynomial for the six; otherwise as
+// two triples (face64_triple_of). Only the triples' products and smallest
+// dp stay live.
+template <int QPL>
+__device__ __forceinline__ void face64_hexad(const FaceRec64* __restrict__ r,
+                                             const double (&ox)[QPL], const double (&oy)[QPL],
+                                             const double (&oz)[QPL], double (&acc)[QPL],
+                                             double (&acc_slow)[QPL]) {
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const FaceGeom64 g1 = face64_geom(r[0], ox[k], oy[k], oz[k]);
+    const FaceGeom64 g2 = face64_geom(r[1], ox[k], oy[k], oz[k]);
+    const FaceGeom64 g3 = face64_geom(r[2], ox[k], oy[k], oz[k]);
+    const Prod3F64 t1 = prod3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+    const double m1 = fmin(fmin(g1.dp, g2.dp), g3.dp);
+    const FaceGeom64 g4 = face64_geom(r[3], ox[k], oy[k], oz[k]);
+    const FaceGeom64 g5 = face64_geom(r[4], ox[k], oy[k], oz[k]);
+    const FaceGeom64 g6 = face64_geom(r[5], ox[k], oy[k], oz[k]);
+    const Prod3F64 t2 = prod3_f64(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
+    const double m2 = fmin(fmin(g4.dp, g5.dp), g6.dp);
+    const double re = __builtin_fma(-t1.im, t2.im, t1.re * t2.re);
+    const double im = __builtin_fma(t1.re, t2.im, t1.im * t2.re);
+    const double x = div_f64(im, re);  // used only where re is positive normal
+    if (__builtin_expect(wave_all_group_f64(fmin(m1, m2), fmin(t1.re, t2.re), re, x), 1)) {
+      acc[k] += atan_sum_f64(x);
+    } else {
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      const double2 a = face64_triple_of(r, t1.re, t1.im, m1, ox[k], oy[k], oz[k]);
+      const double2 b = face64_triple_of(r + 3, t2.re, t2.im, m2, ox[k], oy[k], oz[k]);
+      acc[k] += a.x;
+      acc[k] += b.x;
+      acc_slow[k] += a.y;
+      acc_slow[k] += b.y;
+    }
+  }
+}
+
 template <int QPL>
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(QPL == 1 ? 6 : 4))) void fwd_f64f_kernel(
     const FaceRec64* __restrict__ rec, const double* __restrict__ q, double* __restrict__ out,
@@ -214,6 +273,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
     const FaceRec64* r = rec + c0;
     const int cnt = (int)(f1 - c0 < ((int64_t)1 << 30) ? f1 - c0 : ((int64_t)1 << 30));
     int j = 0;
+    for (; j + 6 <= cnt; j += 6) face64_hexad<QPL>(r + j, ox, oy, oz, acc, acc_slow);
     for (; j + 3 <= cnt; j += 3) face64_triple<QPL>(r + j, ox, oy, oz, acc, acc_slow);
     for (; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
   }
@@ -275,6 +335,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
     const FaceRec64* r = rec + f0;
     const int cnt = (int)(f1 - f0);  // a mesh has at most 2^31-1 faces
     int j = 0;
+    for (; j + 6 <= cnt; j += 6) face64_hexad<QPL>(r + j, ox, oy, oz, acc, acc_slow);
     for (; j + 3 <= cnt; j += 3) face64_triple<QPL>(r + j, ox, oy, oz, acc, acc_slow);
     for (; j < cnt; ++j) face64_step<QPL>(r[j], ox, oy, oz, acc, acc_slow);
 #pragma unroll

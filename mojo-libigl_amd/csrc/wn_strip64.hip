@@ -123,12 +123,57 @@ __device__ __forceinline__ void strip64_face(const StripRec64* __restrict__ r, i
   }
 }
 
+// Geometry of the face closed by record j (j >= 2 in its tile) rebuilt from
+// records j-2, j-1, j with strip64_geom's operations: the values the sweep
+// formed for it.
+__device__ __forceinline__ Geom64 strip64_face_geom(const StripRec64* __restrict__ r, int j,
+                                                    double ox, double oy, double oz) {
+  Slot64 P, Q, R;
+  double s_carry = 0.0;
+  P.ax = r[j - 2].x - ox;
+  P.ay = r[j - 2].y - oy;
+  P.az = r[j - 2].z - oz;
+  P.l2 = __builtin_fma(P.az, P.az, __builtin_fma(P.ay, P.ay, P.ax * P.ax));
+  P.l = sqrt_gs_f64(P.l2);
+  Q.ax = r[j - 1].x - ox;
+  Q.ay = r[j - 1].y - oy;
+  Q.az = r[j - 1].z - oz;
+  Q.l2 = __builtin_fma(Q.az, Q.az, __builtin_fma(Q.ay, Q.ay, Q.ax * Q.ax));
+  Q.l = sqrt_gs_f64(Q.l2);
+  s_carry = half_sum_plus(P.l2, Q.l2, r[j - 1].e1);  // a.b, as record j-1 formed it
+  return strip64_geom(r[j], P, Q, R, s_carry, ox, oy, oz);
+}
+
+// Records j, j+1, j+2 whose product t and smallest dp the sweep formed: the
+// group's angle when all three close a face and every lane meets the group
+// test, else face by face (strip64_face, geometry rebuilt from the records).
+// Out of line (the fallback of a failed group of six): its registers are not
+// reserved for the sweep. Returns (fast-path sum, plain-formula sum).
+__device__ __noinline__ double2 strip64_triple_of(const StripRec64* __restrict__ r, int j,
+                                                  long long f1, long long f2, long long f3,
+                                                  double t_re, double t_im, double dp_min,
+                                                  double ox, double oy, double oz) {
+  double acc = 0.0, acc_slow = 0.0;
+  if (f1 != 0 && f2 != 0 && f3 != 0) {
+    const double x = div_f64(t_im, t_re);  // used only where t_re is positive normal
+    if (wave_all_group_f64(dp_min, t_re, t_re, x)) return make_double2(atan_sum_f64(x), 0.0);
+  }
+  const long long f[3] = {f1, f2, f3};
+  for (int i = 0; i < 3; ++i)
+    if (f[i] != 0)
+      strip64_face(r, j + i, f[i], strip64_face_geom(r, j + i, ox, oy, oz), acc, acc_slow, ox,
+                   oy, oz);
+  return make_double2(acc, acc_slow);
+}
+
 // Unscaled FP64 angle sum of one query over strip tiles [t_begin, t_end).
-// Records go in groups of three (a tile's 255 = 85 x 3, so the register
-// slots rotate statically). When all three close a face (flags: scalar) and
-// every lane meets sum3_f64's conditions, the group's angle is the
-// argument of the product of its faces' (den + i det): one quotient and one
-// 11-term polynomial for the three; otherwise each face on its own.
+// Records go in groups of six (two turns of the three register slots; a
+// tile's 255 = 42 x 6 + 3). When all six close a face (flags: scalar) and
+// every lane meets the group test, the group's angle is the argument of the
+// product of its faces' (den + i det): one quotient and one 11-term
+// polynomial for the six; otherwise as two groups of three
+// (strip64_triple_of), and a triple that fails face by face. The last three
+// records of a tile go as a group of three.
 __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ rec64,
                                                 const double4* __restrict__ rec, int64_t t_begin,
                                                 int64_t t_end, double ox, double oy, double oz) {
@@ -137,30 +182,55 @@ __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ r
   for (int64_t t = t_begin; t < t_end; ++t) {
     const StripRec64* r = rec64 + t * kStripTileRecords;
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
-    for (int j = 0; j < kStripTileRecords; j += 3) {
+    int j = 0;
+    for (; j + 6 <= kStripTileRecords; j += 6) {
       const long long f1 = flags[4 * j], f2 = flags[4 * (j + 1)], f3 = flags[4 * (j + 2)];
+      const long long f4 = flags[4 * (j + 3)], f5 = flags[4 * (j + 4)], f6 = flags[4 * (j + 5)];
       const Geom64 g1 = strip64_geom(r[j], A, B, C, s_carry, ox, oy, oz);
       const Geom64 g2 = strip64_geom(r[j + 1], B, C, A, s_carry, ox, oy, oz);
       const Geom64 g3 = strip64_geom(r[j + 2], C, A, B, s_carry, ox, oy, oz);
-      if (__builtin_expect(f1 != 0 && f2 != 0 && f3 != 0, 1)) {
-        const Sum3F64 s = sum3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
-        if (__builtin_expect(wave_all_sum3_f64(g1.dp, g2.dp, g3.dp, s), 1)) {
-          acc += atan_sum_f64(s.x);
+      const Prod3F64 t1 = prod3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+      const double m1 = fmin(fmin(g1.dp, g2.dp), g3.dp);
+      const Geom64 g4 = strip64_geom(r[j + 3], A, B, C, s_carry, ox, oy, oz);
+      const Geom64 g5 = strip64_geom(r[j + 4], B, C, A, s_carry, ox, oy, oz);
+      const Geom64 g6 = strip64_geom(r[j + 5], C, A, B, s_carry, ox, oy, oz);
+      const Prod3F64 t2 = prod3_f64(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
+      const double m2 = fmin(fmin(g4.dp, g5.dp), g6.dp);
+      if (__builtin_expect(f1 != 0 && f2 != 0 && f3 != 0 && f4 != 0 && f5 != 0 && f6 != 0, 1)) {
+        const double re = __builtin_fma(-t1.im, t2.im, t1.re * t2.re);
+        const double im = __builtin_fma(t1.re, t2.im, t1.im * t2.re);
+        const double x = div_f64(im, re);  // used only where re is positive normal
+        if (__builtin_expect(wave_all_group_f64(fmin(m1, m2), fmin(t1.re, t2.re), re, x), 1)) {
+          acc += atan_sum_f64(x);
           continue;
         }
       }
       asm volatile("" ::);  // keep a real wave-uniform branch
-      if (f1 != 0) strip64_face(r, j, f1, g1, acc, acc_slow, ox, oy, oz);
-      if (f2 != 0) strip64_face(r, j + 1, f2, g2, acc, acc_slow, ox, oy, oz);
-      if (f3 != 0) strip64_face(r, j + 2, f3, g3, acc, acc_slow, ox, oy, oz);
+      const double2 a = strip64_triple_of(r, j, f1, f2, f3, t1.re, t1.im, m1, ox, oy, oz);
+      const double2 b = strip64_triple_of(r, j + 3, f4, f5, f6, t2.re, t2.im, m2, ox, oy, oz);
+      acc += a.x;
+      acc += b.x;
+      acc_slow += a.y;
+      acc_slow += b.y;
     }
+    // the tile's last three records
+    const long long f1 = flags[4 * j], f2 = flags[4 * (j + 1)], f3 = flags[4 * (j + 2)];
+    const Geom64 g1 = strip64_geom(r[j], A, B, C, s_carry, ox, oy, oz);
+    const Geom64 g2 = strip64_geom(r[j + 1], B, C, A, s_carry, ox, oy, oz);
+    const Geom64 g3 = strip64_geom(r[j + 2], C, A, B, s_carry, ox, oy, oz);
+    const Prod3F64 tl = prod3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+    const double2 a = strip64_triple_of(r, j, f1, f2, f3, tl.re, tl.im,
+                                        fmin(fmin(g1.dp, g2.dp), g3.dp), ox, oy, oz);
+    acc += a.x;
+    acc_slow += a.y;
   }
   return acc + acc_slow;
 }
 
-// 6 waves per SIMD: 80 VGPRs, no spills on the fast path (the slow-path pair
-// is a call, around which registers are saved)
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6))) void fwd_strip64_kernel(
+// 5 waves per SIMD: 96 VGPRs, no spills on the fast path (the slow paths
+// are calls, around which registers are saved; at 6 waves and 80 VGPRs the
+// groups of six spilled in the loop, 6.5 % slower)
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5))) void fwd_strip64_kernel(
     const StripRec64* __restrict__ rec64, const double4* __restrict__ rec, int64_t ntiles,
     const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ out,
     int64_t nq, int64_t tiles_per_split, int splits) {
@@ -185,7 +255,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6
 // tile, tile chunk), with the FP32 strip batch kernel's item protocol
 // (atomic counter, block-uniform exit) and output rule (single-chunk meshes
 // write w, others unscaled chunk-major partials summed in chunk order).
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(6))) void batch_strip64_kernel(
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5))) void batch_strip64_kernel(
     const StripRec64* __restrict__ rec64, const double4* __restrict__ rec,
     const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ dst,
     double* __restrict__ partial, const BatchMesh* __restrict__ meshes,
