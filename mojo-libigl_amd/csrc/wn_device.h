@@ -260,10 +260,16 @@ __device__ __forceinline__ AngleSum sum6_f32(const Prod3& t1, const Prod3& t2) {
 
 // 1/re positive normal (see above): one v_cmp_class_f32 writing the lane
 // mask (the class builtin goes through a VGPR select and a second compare
-// before its ballot).
+// before its ballot). Its operand is often a v_rcp_f32 result: a VALU
+// instruction reading a transcendental's result needs one wait state after
+// it, which the compiler inserts for the instructions it emits but not for
+// inline asm operands, so the asm starts with its own s_nop (without it, a
+// schedule that put the compare right after the reciprocal read a stale
+// register, and the group test, and so the rounding path, varied from run
+// to run).
 __device__ __forceinline__ uint64_t lanes_positive_normal(float v) {
   uint64_t m;
-  asm volatile("v_cmp_class_f32_e64 %0, %1, %2" : "=s"(m) : "v"(v), "s"(1 << 8));
+  asm volatile("s_nop 0\n\tv_cmp_class_f32_e64 %0, %1, %2" : "=s"(m) : "v"(v), "s"(1 << 8));
   return m;
 }
 

@@ -143,7 +143,10 @@ __device__ __forceinline__ Prod3F64 prod3_f64(double e1, double d1, double e2, d
   return t;
 }
 
-// Lanes where v is a positive normal FP64 number: one v_cmp_class_f64.
+// Lanes where v is a positive normal FP64 number: one v_cmp_class_f64. Its
+// operands here are FMA / min results, never a transcendental's (which an
+// inline asm operand would have to wait one state for, as in
+// lanes_positive_normal).
 __device__ __forceinline__ uint64_t lanes_positive_normal_f64(double v) {
   uint64_t m;
   asm volatile("v_cmp_class_f64_e64 %0, %1, %2" : "=s"(m) : "v"(v), "s"(1 << 8));

@@ -18,14 +18,15 @@
 // statically through a 3-way unrolled loop):
 //   R0[j] = (w.x, w.y, w.z, -|w_j - w_{j-1}|^2 / 2)  (block-local FP32 coords)
 //   R1[j] = (-|w_j - w_{j-2}|^2 / 2, N.x, N.y, N.z)  (N = face normal)
-// in the tile's power-of-two frame (frame_exponent in wn_device.h: the
-// block-local coordinates times 2^(2 - E), |corner - query| < 14 in it, so
-// that the group products below stay in the FP32 range at any input scale);
+// in the tile's power-of-two frame (frame_exponent in wn_device.h: here the
+// block-local coordinates times 2^(-E), |corner - query| < 2 sqrt(3) ~ 3.5
+// in it, so that the products of twelve faces below stay in the FP32 range
+// at any input scale);
 // so that a dot product is one add and one fma with 0.5 (law of cosines;
 // the halving is exact) and den is formed undoubled.
 // The face orientation (record flag: 0 = no face ends at j, +1 / -1 = face
 // orientation vs strip order) is folded into N; a record with flag 0 is
-// staged with N = 0 and R1.x = +2^30, which makes its angle 0 without a
+// staged with N = 0 and R1.x = +2^20, which makes its angle 0 without a
 // branch. The flag itself is read (from the records, at a wave-uniform
 // address) only on the rare slow path.
 // Fast path: for each group of six records, the sum of their angles is the
@@ -80,10 +81,10 @@ __device__ __forceinline__ int32_t thread_index(int32_t base) {
 // its record j, and its wave's max over the tile of |corner - g| (infinity
 // norm) goes to sm.tmax. After the barrier, strip_tile_store reads records
 // j, j-1, j-2 (the record loads of the first step, again: nothing is held
-// across the barrier), scales the tile into the frame sc = 2^(2 - E)
+// across the barrier), scales the tile into the frame sc = 2^(-E)
 // (frame_exponent: 2^E bounds |corner - g| over the tile and, through er,
 // |query - g| over the block) and writes the LDS records; it returns the
-// factor 2^(er - E) that takes the lanes' offsets (staged at 2^(2 - er))
+// factor 2^(er - E) that takes the lanes' offsets (staged at 2^(-er))
 // into the same frame.
 __device__ __forceinline__ void strip_tile_extent(const double4* __restrict__ rec, int64_t base,
                                                   int cnt, double gx, double gy, double gz,
@@ -109,7 +110,7 @@ __device__ __forceinline__ float strip_tile_store(const double4* __restrict__ re
   mt = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt);
   const int ef = frame_exponent_hi(mt) + 1;
   const int E = ef > er ? ef : er;
-  const double sc = pow2_f64(2 - E);
+  const double sc = pow2_f64(-E);
   const int j = thread_index(wbase);
   if (j < cnt) {
     const double4 c = rec[base + j];
@@ -127,13 +128,13 @@ __device__ __forceinline__ float strip_tile_store(const double4* __restrict__ re
     const double ex = (double)px - qx, ey = (double)py - qy, ez = (double)pz - qz;
     // N = (w_{k-1} - w_{k-2}) x (w_k - w_{k-2}), oriented by the flag: the
     // lane's dot product with it is det
-    // A record that closes no face (flag 0) gets N = 0 and +2^30 in place of
-    // -|w_k - w_{k-2}|^2 / 2: det is then 0 exactly and c.a ~ 2^30 makes dp
+    // A record that closes no face (flag 0) gets N = 0 and +2^20 in place of
+    // -|w_k - w_{k-2}|^2 / 2: det is then 0 exactly and c.a ~ 2^20 makes dp
     // and den positive and finite (see strip_record), so both fast paths give
     // it angle 0 (0 / den, and a real factor den + 0i of a group product).
     const double f = c.w;
     sm.r0[j] = make_float4(cx, cy, cz, (float)(-0.5 * (d1x * d1x + d1y * d1y + d1z * d1z)));
-    sm.r1[j] = make_float4(c.w == 0.0 ? 1073741824.0f
+    sm.r1[j] = make_float4(c.w == 0.0 ? 1048576.0f
                                       : (float)(-0.5 * (d2x * d2x + d2y * d2y + d2z * d2z)),
                            (float)(f * (ey * d2z - ez * d2y)), (float)(f * (ez * d2x - ex * d2z)),
                            (float)(f * (ex * d2y - ey * d2x)));
@@ -186,9 +187,11 @@ __device__ __forceinline__ StripGeom strip_geom(const float4& r0, const float4& 
 // Record j of the tile closes face (P, Q, R) (P and Q from the two previous
 // records, R computed here): its FP32 geometry. Every record takes the same
 // instructions: one that closes no face (flag 0: the first two of a strip,
-// tile padding) was staged (strip_tile_store) with N = 0 and c.a = +2^30,
-// so det = 0 exactly and, with Q's length above ~1e-9 of the mesh's scale,
-// dp and den are positive: its angle is 0 on both fast paths.
+// tile padding) was staged (strip_tile_store) with N = 0 and c.a = +2^20,
+// so det = 0 exactly and, with Q's length above ~1e-4 of the frame's unit
+// (|corner - query| < 3.5 in it), dp and den are positive: its angle is 0
+// on the fast paths (a real factor of at most 2^21.8 in a group's product:
+// two of them and ten far faces' 4 la lb lc <= 2^7.4 stay below 2^126).
 // boff: byte offset of record j in r0 and r1, held in a VGPR (see strip_sweep).
 __device__ __forceinline__ StripGeom strip_record(const StripSmem& sm, int boff, const VSlot& P,
                                                   const VSlot& Q, VSlot& R, float& s_carry,
@@ -293,14 +296,60 @@ __device__ __forceinline__ float strip_triple_lds(const StripSmem& sm,
   return part;
 }
 
-// Records j .. j+5 (two turns of the slot rotation): the FP32 sum of their
-// six angles, as one product of six (sum6_f32) when every lane meets
-// wave_all_sum6, else as two groups of three.
-__device__ __forceinline__ float strip_hexad(const StripSmem& sm,
-                                             const long long* __restrict__ flags, int j, int boff,
-                                             VSlot& A, VSlot& B, VSlot& C, float& s_carry,
-                                             float ox, float oy, float oz, float kappa) {
-#pragma clang fp contract(off)  // the two groups' sums round before they add
+// The geometry of record j's face rebuilt from the LDS tile (corners j-2,
+// j-1, j; j >= 2) with strip_record's operations in its order (a.b as the
+// previous record formed it): the values the sweep computed. rl: the length
+// of the newest corner.
+__device__ __forceinline__ StripGeom strip_geom_lds(const StripSmem& sm, int j, float ox,
+                                                    float oy, float oz, float& rl) {
+  const float4 p = sm.r0[j - 2], q = sm.r0[j - 1], c = sm.r0[j];
+  VSlot P, Q, R;
+  strip_vertex(p, ox, oy, oz, P);
+  strip_vertex(q, ox, oy, oz, Q);
+  strip_vertex(c, ox, oy, oz, R);
+  const float s_pq = __builtin_fmaf(P.l2 + Q.l2, 0.5f, q.w);
+  rl = R.l;
+  return strip_geom(c, sm.r1[j], P, Q, R, s_pq);
+}
+
+// Records j, j+1, j+2 rebuilt from the tile: the product of their faces'
+// (den + i det) and the smallest dp, a record that closes no face (flag 0;
+// read at wave-uniform addresses) taking the factor 1 and dp 1 (its angle is
+// 0). Used when a group of six that the sweep formed fails its test.
+__device__ __forceinline__ void strip_triple_rebuild(const StripSmem& sm,
+                                                     const long long* __restrict__ flags,
+                                                     int j, float ox, float oy, float oz,
+                                                     Prod3& t, float& dp_min) {
+  float det[3], den[3], dp[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    det[i] = 0.0f;
+    den[i] = 1.0f;
+    dp[i] = 1.0f;
+    if (flags[4 * (j + i)] != 0) {
+      float rl;
+      const StripGeom g = strip_geom_lds(sm, j + i, ox, oy, oz, rl);
+      det[i] = g.det;
+      den[i] = g.den;
+      dp[i] = g.dp;
+    }
+  }
+  t = prod3_f32(det[0], den[0], det[1], den[1], det[2], den[2]);
+  dp_min = __builtin_fminf(__builtin_fminf(dp[0], dp[1]), dp[2]);
+}
+
+// Six records' product of their faces' (den + i det), and the lanes meeting
+// the six-group conditions other than the quotient's (every dp > 0, both
+// triples' re and the product's re positive normal: no precision lost to
+// underflow), as a lane mask in scalar registers.
+struct Hex32 {
+  float re, im;
+  uint64_t ok;
+};
+
+__device__ __forceinline__ Hex32 strip_hex_product(const StripSmem& sm, int boff, VSlot& A,
+                                                   VSlot& B, VSlot& C, float& s_carry, float ox,
+                                                   float oy, float oz) {
   const StripGeom g1 = strip_record(sm, boff, A, B, C, s_carry, ox, oy, oz);
   const StripGeom g2 = strip_record(sm, boff + 16, B, C, A, s_carry, ox, oy, oz);
   const StripGeom g3 = strip_record(sm, boff + 32, C, A, B, s_carry, ox, oy, oz);
@@ -311,11 +360,57 @@ __device__ __forceinline__ float strip_hexad(const StripSmem& sm,
   const StripGeom g6 = strip_record(sm, boff + 80, C, A, B, s_carry, ox, oy, oz);
   const Prod3 t2 = prod3_f32(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
   const float m2 = __builtin_fminf(__builtin_fminf(g4.dp, g5.dp), g6.dp);
-  const AngleSum s = sum6_f32(t1, t2);
-  if (__builtin_expect(wave_all_sum6(m1, m2, t1, t2, s), 1)) return atan_sum_f32(s.x);
+  Hex32 h;
+  h.re = __builtin_fmaf(-t1.im, t2.im, t1.re * t2.re);
+  h.im = __builtin_fmaf(t1.re, t2.im, t1.im * t2.re);
+  h.ok = __builtin_amdgcn_ballot_w64(__builtin_fminf(m1, m2) > 0.0f) &
+         lanes_positive_normal(__builtin_fminf(t1.re, t2.re)) & lanes_positive_normal(h.re);
+  return h;
+}
+
+// Six records j .. j+5 whose product h the sweep formed: their FP32 angle sum
+// when every lane meets the six-group test (h.ok, 1/re positive normal and
+// |x| < 0.4), else as two triples rebuilt from the tile (strip_triple_lds).
+__device__ __forceinline__ float strip_hexad_of(const StripSmem& sm,
+                                                const long long* __restrict__ flags, int j,
+                                                const Hex32& h, float ox, float oy, float oz,
+                                                float kappa) {
+#pragma clang fp contract(off)  // the two triples' sums round before they add
+  const AngleSum s = angle_sum_of(h.re, h.im);
+  const uint64_t m = h.ok & lanes_positive_normal(s.rc) &
+                     __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
+  if (__builtin_expect(m == __builtin_amdgcn_read_exec(), 1)) return atan_sum_f32(s.x);
   asm volatile("" ::);  // keep a real wave-uniform branch
+  Prod3 t1, t2;
+  float m1, m2;
+  strip_triple_rebuild(sm, flags, j, ox, oy, oz, t1, m1);
+  strip_triple_rebuild(sm, flags, j + 3, ox, oy, oz, t2, m2);
   return strip_triple_lds(sm, flags, j, t1, m1, ox, oy, oz, kappa) +
          strip_triple_lds(sm, flags, j + 3, t2, m2, ox, oy, oz, kappa);
+}
+
+// Records j .. j+11 (four turns of the slot rotation): the FP32 sum of their
+// twelve angles, as the argument of the product of their (den + i det) when
+// every lane meets both six-groups' conditions, 1/re positive normal and
+// |x| < 0.4 (one reciprocal and one polynomial for the twelve), else as two
+// groups of six (strip_hexad_of). Only each six's (re, im) and its lane mask
+// stay live across the second six.
+__device__ __forceinline__ float strip_dodecad(const StripSmem& sm,
+                                               const long long* __restrict__ flags, int j,
+                                               int boff, VSlot& A, VSlot& B, VSlot& C,
+                                               float& s_carry, float ox, float oy, float oz,
+                                               float kappa) {
+#pragma clang fp contract(off)  // the two sixes' sums round before they add
+  const Hex32 h1 = strip_hex_product(sm, boff, A, B, C, s_carry, ox, oy, oz);
+  const Hex32 h2 = strip_hex_product(sm, boff + 96, A, B, C, s_carry, ox, oy, oz);
+  const AngleSum s = angle_sum_of(__builtin_fmaf(-h1.im, h2.im, h1.re * h2.re),
+                                  __builtin_fmaf(h1.re, h2.im, h1.im * h2.re));
+  const uint64_t m = h1.ok & h2.ok & lanes_positive_normal(s.rc) &
+                     __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
+  if (__builtin_expect(m == __builtin_amdgcn_read_exec(), 1)) return atan_sum_f32(s.x);
+  asm volatile("" ::);  // keep a real wave-uniform branch
+  return strip_hexad_of(sm, flags, j, h1, ox, oy, oz, kappa) +
+         strip_hexad_of(sm, flags, j + 6, h2, ox, oy, oz, kappa);
 }
 
 // Block-local origin: centre of the bbox of the block's valid queries (the
@@ -352,7 +447,7 @@ __device__ __forceinline__ void strip_block_origin(bool valid, double Ox, double
 }
 
 // Unscaled angle sum of one query over strip tiles [t_begin, t_end); o is
-// the query's block-local FP32 offset in the frame 2^(2 - er)
+// the query's block-local FP32 offset in the frame 2^(-er)
 // (strip_block_origin), rescaled into each tile's frame (strip_tile_store).
 // All threads of the block call it together; wbase = wave_base(). The
 // staging rebuilds the thread index from wbase on every tile (thread_index),
@@ -381,12 +476,13 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     // iteration.)
     int boff = 0;
     asm volatile("" : "+v"(boff));
-    // groups of six records, each group's FP32 angle sum joined to the FP64
-    // sum once; 255 = 42 x 6 + 3, the last three records as a group of three
+    // groups of twelve records, each group's FP32 angle sum joined to the
+    // FP64 sum once; 255 = 21 x 12 + 3, the last three records as a group of
+    // three
     int j = 0;
-    for (; j + 6 <= kStripTileRecords; j += 6) {
-      acc += (double)strip_hexad(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
-      boff += 6 * (int)sizeof(float4);
+    for (; j + 12 <= kStripTileRecords; j += 12) {
+      acc += (double)strip_dodecad(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
+      boff += 12 * (int)sizeof(float4);
       asm volatile("" : "+v"(boff));
     }
     acc += (double)strip_triple(sm, flags, j, boff, A, B, C, s_carry, ox, oy, oz, kappa);
@@ -432,7 +528,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
   double gx, gy, gz;
   int er;
   strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz, er);
-  const double so = pow2_f64(2 - er);
+  const double so = pow2_f64(-er);
   const float ox = valid ? (float)((Ox - gx) * so) : 0.0f;
   const float oy = valid ? (float)((Oy - gy) * so) : 0.0f;
   const float oz = valid ? (float)((Oz - gz) * so) : 0.0f;
@@ -486,7 +582,7 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(8
     double gx, gy, gz;
     int er;
     strip_block_origin(valid, Ox, Oy, Oz, sm, gx, gy, gz, er);
-    const double so = pow2_f64(2 - er);
+    const double so = pow2_f64(-er);
     const float ox = valid ? (float)((Ox - gx) * so) : 0.0f;
     const float oy = valid ? (float)((Oy - gy) * so) : 0.0f;
     const float oz = valid ? (float)((Oz - gz) * so) : 0.0f;

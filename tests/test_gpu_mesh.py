@@ -119,6 +119,36 @@ def test_strip_layout_boundaries(require_gpu, nq, splits, sort):
     assert np.abs(w32[idx] - ref).max() <= 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp32c", "fp64"])
+def test_strip_runs_are_bitwise_repeatable(require_gpu, precision):
+    """Run-to-run bit equality of the strip kernels on an open cap with
+    queries at 1e-6 .. 1e-1 from its faces, which send many groups of faces
+    through every fallback of the group tests (two sixes, triples, single
+    faces, the FP64 re-evaluation) and through the boundary correction: the
+    path a wave takes must depend on the data only (a hazard that let a test
+    read a stale register once made ~70 % of such queries differ between
+    runs by a few 1e-8)."""
+    V, F = W.config5()[:2]
+    V = V.astype(np.float64)
+    cz = V[F].mean(axis=1)[:, 2]
+    Fs = F[np.sort(np.argsort(-cz)[:2000])]
+    rng = np.random.default_rng(77)
+    T = V[Fs[rng.integers(0, len(Fs), 100_000)]]
+    bary = rng.dirichlet(np.ones(3), size=len(T))
+    p = np.einsum("ij,ijk->ik", bary, T)
+    n = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    d = 10.0 ** rng.uniform(-6, -1, size=len(T)) * rng.choice([-1.0, 1.0], size=len(T))
+    Q = np.vstack([p + d[:, None] * n, rng.uniform(V.min(0), V.max(0), size=(162_144, 3))])
+    m = _mesh(V, Fs)
+    try:
+        runs = [m.winding_number(Q, precision=precision) for _ in range(3)]
+    finally:
+        m.close()
+    for w in runs[1:]:
+        assert np.array_equal(w, runs[0]), int((w != runs[0]).sum())
+
+
 def test_mesh_rejects_bad_faces(require_gpu):
     import igl
 
