@@ -54,11 +54,12 @@ __device__ __forceinline__ float atan2_f32(float y, float x, float& mx) {
 //
 // Every FP32 quantity of a pair scales with a power of its coordinates (the
 // lengths l, den ~ l^3, a product of six faces' (den + i det) ~ l^18), and
-// the group products leave the FP32 range unless l stays within about
-// [2^-7, 2^6]. Each LDS tile is therefore staged in a frame scaled by
-// sc = 2^(2 - E), where 2^E bounds both |corner - g| over the tile and
+// the group products leave the FP32 range unless l stays within a few
+// powers of two of 1. Each LDS tile is therefore staged in a frame scaled by
+// sc = 2^(-E), where 2^E bounds both |corner - g| over the tile and
 // |query - g| over the block (g the block origin, infinity norm): every
-// pair's |corner - query| is then below 8 sqrt(3) ~ 13.9. Multiplying by a
+// pair's |corner - query| is then below 2 sqrt(3) ~ 3.5, and a product of
+// twelve far faces' (den + i det) below 2^89. Multiplying by a
 // power of two is exact and FP32 rounding commutes with it (for results in
 // the normal range), so the rounded corners and query offsets are the
 // unscaled ones times sc, every angle is unchanged, and a power-of-two
@@ -301,6 +302,49 @@ __device__ __forceinline__ bool wave_all_sum6(float m1, float m2, const Prod3& t
                                               const Prod3& t2, const AngleSum& s) {
   const uint64_t m = lanes_positive_normal(__builtin_fminf(t1.re, t2.re));
   return (group_lanes(__builtin_fminf(m1, m2), s) & m) == __builtin_amdgcn_read_exec();
+}
+
+// Six faces as two triples t1, t2 (m1, m2 their smallest dp): their
+// product (re, im) and the lanes meeting the six-group conditions other than
+// the quotient's (every dp > 0, both triples' re and the product's re positive
+// normal: no precision lost to underflow), as a lane mask in scalar
+// registers. Twelve faces are the product of two sixes: each six's
+// angle sum then lies within +-pi/2 (its re > 0 and, with every angle within
+// +-pi/4, the sum within +-3pi/2), so the twelve sum within +-pi and a
+// twelve-product argument within +-0.39 is that sum (wave_all_dodecad).
+struct Hex32 {
+  float re, im;
+  uint64_t ok;
+};
+
+__device__ __forceinline__ Hex32 hex_of(const Prod3& t1, float m1, const Prod3& t2, float m2) {
+  Hex32 h;
+  h.re = __builtin_fmaf(-t1.im, t2.im, t1.re * t2.re);
+  h.im = __builtin_fmaf(t1.re, t2.im, t1.im * t2.re);
+  h.ok = __builtin_amdgcn_ballot_w64(__builtin_fminf(m1, m2) > 0.0f) &
+         lanes_positive_normal(__builtin_fminf(t1.re, t2.re)) & lanes_positive_normal(h.re);
+  return h;
+}
+
+// The angle sum of six faces whose Hex32 is h when every lane meets the
+// six-group test (h.ok, 1/re positive normal, |x| < 0.4): true and *x set.
+__device__ __forceinline__ bool wave_all_hex(const Hex32& h, float& x) {
+  const AngleSum s = angle_sum_of(h.re, h.im);
+  x = s.x;
+  const uint64_t m = h.ok & lanes_positive_normal(s.rc) &
+                     __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
+  return m == __builtin_amdgcn_read_exec();
+}
+
+// Twelve faces as two sixes h1, h2: true and *x set when every lane meets
+// both sixes' conditions, 1/re of the product positive normal and |x| < 0.4.
+__device__ __forceinline__ bool wave_all_dodecad(const Hex32& h1, const Hex32& h2, float& x) {
+  const AngleSum s = angle_sum_of(__builtin_fmaf(-h1.im, h2.im, h1.re * h2.re),
+                                  __builtin_fmaf(h1.re, h2.im, h1.im * h2.re));
+  x = s.x;
+  const uint64_t m = h1.ok & h2.ok & lanes_positive_normal(s.rc) &
+                     __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
+  return m == __builtin_amdgcn_read_exec();
 }
 
 // Geometry of one pair in FP32 from a record. Algebra (exact identities,

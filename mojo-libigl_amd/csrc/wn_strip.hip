@@ -338,15 +338,8 @@ __device__ __forceinline__ void strip_triple_rebuild(const StripSmem& sm,
   dp_min = __builtin_fminf(__builtin_fminf(dp[0], dp[1]), dp[2]);
 }
 
-// Six records' product of their faces' (den + i det), and the lanes meeting
-// the six-group conditions other than the quotient's (every dp > 0, both
-// triples' re and the product's re positive normal: no precision lost to
-// underflow), as a lane mask in scalar registers.
-struct Hex32 {
-  float re, im;
-  uint64_t ok;
-};
-
+// Six records' product of their faces' (den + i det) and its lane mask
+// (hex_of).
 __device__ __forceinline__ Hex32 strip_hex_product(const StripSmem& sm, int boff, VSlot& A,
                                                    VSlot& B, VSlot& C, float& s_carry, float ox,
                                                    float oy, float oz) {
@@ -360,12 +353,7 @@ __device__ __forceinline__ Hex32 strip_hex_product(const StripSmem& sm, int boff
   const StripGeom g6 = strip_record(sm, boff + 80, C, A, B, s_carry, ox, oy, oz);
   const Prod3 t2 = prod3_f32(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
   const float m2 = __builtin_fminf(__builtin_fminf(g4.dp, g5.dp), g6.dp);
-  Hex32 h;
-  h.re = __builtin_fmaf(-t1.im, t2.im, t1.re * t2.re);
-  h.im = __builtin_fmaf(t1.re, t2.im, t1.im * t2.re);
-  h.ok = __builtin_amdgcn_ballot_w64(__builtin_fminf(m1, m2) > 0.0f) &
-         lanes_positive_normal(__builtin_fminf(t1.re, t2.re)) & lanes_positive_normal(h.re);
-  return h;
+  return hex_of(t1, m1, t2, m2);
 }
 
 // Six records j .. j+5 whose product h the sweep formed: their FP32 angle sum
@@ -376,10 +364,8 @@ __device__ __forceinline__ float strip_hexad_of(const StripSmem& sm,
                                                 const Hex32& h, float ox, float oy, float oz,
                                                 float kappa) {
 #pragma clang fp contract(off)  // the two triples' sums round before they add
-  const AngleSum s = angle_sum_of(h.re, h.im);
-  const uint64_t m = h.ok & lanes_positive_normal(s.rc) &
-                     __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
-  if (__builtin_expect(m == __builtin_amdgcn_read_exec(), 1)) return atan_sum_f32(s.x);
+  float x;
+  if (__builtin_expect(wave_all_hex(h, x), 1)) return atan_sum_f32(x);
   asm volatile("" ::);  // keep a real wave-uniform branch
   Prod3 t1, t2;
   float m1, m2;
@@ -403,11 +389,8 @@ __device__ __forceinline__ float strip_dodecad(const StripSmem& sm,
 #pragma clang fp contract(off)  // the two sixes' sums round before they add
   const Hex32 h1 = strip_hex_product(sm, boff, A, B, C, s_carry, ox, oy, oz);
   const Hex32 h2 = strip_hex_product(sm, boff + 96, A, B, C, s_carry, ox, oy, oz);
-  const AngleSum s = angle_sum_of(__builtin_fmaf(-h1.im, h2.im, h1.re * h2.re),
-                                  __builtin_fmaf(h1.re, h2.im, h1.im * h2.re));
-  const uint64_t m = h1.ok & h2.ok & lanes_positive_normal(s.rc) &
-                     __builtin_amdgcn_ballot_w64(__builtin_fabsf(s.x) < 0.4f);
-  if (__builtin_expect(m == __builtin_amdgcn_read_exec(), 1)) return atan_sum_f32(s.x);
+  float x;
+  if (__builtin_expect(wave_all_dodecad(h1, h2, x), 1)) return atan_sum_f32(x);
   asm volatile("" ::);  // keep a real wave-uniform branch
   return strip_hexad_of(sm, flags, j, h1, ox, oy, oz, kappa) +
          strip_hexad_of(sm, flags, j + 6, h2, ox, oy, oz, kappa);

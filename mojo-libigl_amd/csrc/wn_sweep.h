@@ -210,11 +210,82 @@ __device__ __forceinline__ void face_hexad_f32c(const FaceRec32* rec, const floa
   }
 }
 
+// Six faces' product of their (den + i det) and its lane mask (hex_of), the
+// geometry computed from the records for one query offset.
+__device__ __forceinline__ Hex32 face_hex_product(const FaceRec32* rec, float ox, float oy,
+                                                  float oz) {
+  const PairGeom32 g1 = pair_geom32(rec[0].r0, rec[0].r1, rec[0].r2, rec[0].r3, ox, oy, oz);
+  const PairGeom32 g2 = pair_geom32(rec[1].r0, rec[1].r1, rec[1].r2, rec[1].r3, ox, oy, oz);
+  const PairGeom32 g3 = pair_geom32(rec[2].r0, rec[2].r1, rec[2].r2, rec[2].r3, ox, oy, oz);
+  const Prod3 t1 = prod3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+  const float m1 = __builtin_fminf(__builtin_fminf(g1.dp, g2.dp), g3.dp);
+  const PairGeom32 g4 = pair_geom32(rec[3].r0, rec[3].r1, rec[3].r2, rec[3].r3, ox, oy, oz);
+  const PairGeom32 g5 = pair_geom32(rec[4].r0, rec[4].r1, rec[4].r2, rec[4].r3, ox, oy, oz);
+  const PairGeom32 g6 = pair_geom32(rec[5].r0, rec[5].r1, rec[5].r2, rec[5].r3, ox, oy, oz);
+  const Prod3 t2 = prod3_f32(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
+  const float m2 = __builtin_fminf(__builtin_fminf(g4.dp, g5.dp), g6.dp);
+  return hex_of(t1, m1, t2, m2);
+}
+
+// Three faces rebuilt from the records: their product and smallest dp, then
+// face_triple_of's test and fallback.
+__device__ __forceinline__ float face_triple_rebuilt(const FaceRec32* rec, float ox, float oy,
+                                                     float oz, float kappa) {
+  const PairGeom32 g1 = pair_geom32(rec[0].r0, rec[0].r1, rec[0].r2, rec[0].r3, ox, oy, oz);
+  const PairGeom32 g2 = pair_geom32(rec[1].r0, rec[1].r1, rec[1].r2, rec[1].r3, ox, oy, oz);
+  const PairGeom32 g3 = pair_geom32(rec[2].r0, rec[2].r1, rec[2].r2, rec[2].r3, ox, oy, oz);
+  const Prod3 t = prod3_f32(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+  const float m = __builtin_fminf(__builtin_fminf(g1.dp, g2.dp), g3.dp);
+  return face_triple_of(rec[0], rec[1], rec[2], t, m, ox, oy, oz, kappa);
+}
+
+// Six faces whose product h was formed: their FP32 angle sum when every lane
+// meets the six-group test, else as two triples rebuilt from the records.
+__device__ __forceinline__ float face_hexad_of(const FaceRec32* rec, const Hex32& h, float ox,
+                                               float oy, float oz, float kappa) {
+#pragma clang fp contract(off)  // the two triples' sums round before they add
+  float x;
+  if (__builtin_expect(wave_all_hex(h, x), 1)) return atan_sum_f32(x);
+  asm volatile("" ::);  // keep a real wave-uniform branch
+  return face_triple_rebuilt(rec, ox, oy, oz, kappa) +
+         face_triple_rebuilt(rec + 3, ox, oy, oz, kappa);
+}
+
+// Twelve faces for QPL queries: the FP32 sum of their angles into sum[k], as
+// the argument of the product of their (den + i det) when every lane meets
+// both six-groups' conditions, 1/re positive normal and |x| < 0.4 (one
+// reciprocal and one polynomial for the twelve), otherwise as two groups of
+// six (face_hexad_of). Only each six's (re, im) and lane mask stay live
+// across the second six.
+template <int QPL>
+__device__ __forceinline__ void face_dodecad_f32c(const FaceRec32* rec, const float (&ox)[QPL],
+                                                  const float (&oy)[QPL], const float (&oz)[QPL],
+                                                  float kappa, float (&sum)[QPL]) {
+#pragma clang fp contract(off)  // the two sixes' sums round before they add
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const Hex32 h1 = face_hex_product(rec, ox[k], oy[k], oz[k]);
+    const Hex32 h2 = face_hex_product(rec + 6, ox[k], oy[k], oz[k]);
+    float x, part;
+    if (__builtin_expect(wave_all_dodecad(h1, h2, x), 1)) {
+      part = atan_sum_f32(x);
+    } else {
+      asm volatile("" ::);  // keep a real wave-uniform branch
+      part = face_hexad_of(rec, h1, ox[k], oy[k], oz[k], kappa) +
+             face_hexad_of(rec + 6, h2, ox[k], oy[k], oz[k], kappa);
+    }
+    sum[k] = part;
+  }
+}
+
 // Faces [f0, f1) for QPL queries per lane; o (ox0 ..) the queries' block-local
 // FP32 offsets in the frame 2^(2 - er) (block_origin). Each tile is staged in
-// its own power-of-two frame sc = 2^(2 - E), 2^E bounding |corner - g| over
-// the tile and |query - g| over the block, and the offsets are rescaled into
-// it by the exact factor 2^(er - E) <= 1 (see frame_exponent).
+// its own power-of-two frame sc = 2^(-E), 2^E bounding |corner - g| over the
+// tile and |query - g| over the block (so that |corner - query| < 2 sqrt(3)
+// ~ 3.5 in it, and a product of twelve faces' (den + i det), each
+// den <= 4 la lb lc < 2^7.4, stays within the FP32 range), and the offsets
+// are rescaled into it by the exact factor 2^(er - E - 2) <= 1/4 (see
+// frame_exponent).
 template <int QPL>
 __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64_t stride,
                                            int64_t f0, int64_t f1, double gx, double gy,
@@ -244,12 +315,12 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
     mt = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt);
     const int ef = frame_exponent_hi(mt) + 1;
     const int E = ef > er ? ef : er;
-    const double sc = pow2_f64(2 - E);
+    const double sc = pow2_f64(-E);
     if (tid < cnt)
       sm.face[tid] = make_face_rec32(d[0] * sc, d[1] * sc, d[2] * sc, d[3] * sc, d[4] * sc,
                                      d[5] * sc, d[6] * sc, d[7] * sc, d[8] * sc, 0.0, 0.0, 0.0);
     __syncthreads();
-    const float fs = pow2_f32(er - E);  // exact rescaling: a power of two <= 1
+    const float fs = pow2_f32(er - E - 2);  // exact rescaling: a power of two <= 1/4
     float ox[QPL], oy[QPL], oz[QPL];
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
@@ -257,12 +328,18 @@ __device__ __forceinline__ void sweep_f32c(const double* __restrict__ tri, int64
       oy[k] = oy0[k] * fs;
       oz[k] = oz0[k] * fs;
     }
-    // the angles of six consecutive faces are summed in FP32 (the argument
-    // of their product, or two groups of three: the argument of a triple's
-    // product or three FP32 angles added; at most a few ulps of their
-    // magnitudes, the bound of a compensated sum) and joined to the FP64 sum
-    // once
+    // the angles of twelve consecutive faces are summed in FP32 (the
+    // argument of their product, or two groups of six, each the argument of
+    // its product or two groups of three: a triple's product or three FP32
+    // angles added; at most a few ulps of their magnitudes, the bound of a
+    // compensated sum) and joined to the FP64 sum once
     int j = 0;
+    for (; j + 12 <= cnt; j += 12) {
+      float s12[QPL];
+      face_dodecad_f32c<QPL>(&sm.face[j], ox, oy, oz, kappa, s12);
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) acc[k] += (double)s12[k];
+    }
     for (; j + 6 <= cnt; j += 6) {
       float s6[QPL];
       face_hexad_f32c<QPL>(&sm.face[j], ox, oy, oz, kappa, s6);
