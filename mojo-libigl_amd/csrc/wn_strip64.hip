@@ -166,95 +166,14 @@ __device__ __noinline__ double2 strip64_triple_of(const StripRec64* __restrict__
   return make_double2(acc, acc_slow);
 }
 
-// Records j, j+1, j+2 rebuilt from the records (strip64_face_geom): their
-// product, smallest dp, then strip64_triple_of's test and fallback. A record
-// that closes no face (flag 0) takes the factor 1 and dp 1; strip64_triple_of
-// then goes face by face and skips it.
-__device__ __forceinline__ double2 strip64_triple_rebuilt(const StripRec64* __restrict__ r,
-                                                          int j, long long f1, long long f2,
-                                                          long long f3, double ox, double oy,
-                                                          double oz) {
-  const long long f[3] = {f1, f2, f3};
-  double det[3], den[3], dp[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    det[i] = 0.0;
-    den[i] = 1.0;
-    dp[i] = 1.0;
-    if (f[i] != 0) {
-      const Geom64 g = strip64_face_geom(r, j + i, ox, oy, oz);
-      det[i] = g.det;
-      den[i] = g.den;
-      dp[i] = g.dp;
-    }
-  }
-  const Prod3F64 t = prod3_f64(det[0], den[0], det[1], den[1], det[2], den[2]);
-  return strip64_triple_of(r, j, f1, f2, f3, t.re, t.im, fmin(fmin(dp[0], dp[1]), dp[2]), ox,
-                           oy, oz);
-}
-
-// Six records j .. j+5 whose product (re, im) and lane mask ok (every dp > 0,
-// both triples' re and re positive normal) the sweep formed: their angle sum
-// when all six close a face and every lane meets the group test, else as two
-// triples rebuilt from the records. Inlined: a call here would make the
-// compiler load the sweep's records as vectors instead of through the scalar
-// cache (the triples' fallback, strip64_triple_of, stays out of line).
-// Returns (fast-path sum, plain-formula sum).
-__device__ __forceinline__ double2 strip64_hexad_of(const StripRec64* __restrict__ r,
-                                                 const long long* __restrict__ flags, int j,
-                                                 double re, double im, uint64_t ok, double ox,
-                                                 double oy, double oz) {
-  const long long f1 = flags[4 * j], f2 = flags[4 * (j + 1)], f3 = flags[4 * (j + 2)];
-  const long long f4 = flags[4 * (j + 3)], f5 = flags[4 * (j + 4)], f6 = flags[4 * (j + 5)];
-  if (f1 != 0 && f2 != 0 && f3 != 0 && f4 != 0 && f5 != 0 && f6 != 0) {
-    const double x = div_f64(im, re);  // used only where re is positive normal
-    const uint64_t m = ok & __builtin_amdgcn_ballot_w64(__builtin_fabs(x) < 0.4);
-    if (m == __builtin_amdgcn_read_exec()) return make_double2(atan_sum_f64(x), 0.0);
-  }
-  const double2 a = strip64_triple_rebuilt(r, j, f1, f2, f3, ox, oy, oz);
-  const double2 b = strip64_triple_rebuilt(r, j + 3, f4, f5, f6, ox, oy, oz);
-  return make_double2(a.x + b.x, a.y + b.y);
-}
-
-// Six records' geometry in the sweep (slots rotating as in strip64_sweep):
-// the product of their faces' (den + i det) as two triples, and the lanes
-// meeting the six-group conditions other than the quotient's.
-struct Hex64 {
-  double re, im;
-  uint64_t ok;
-};
-
-__device__ __forceinline__ Hex64 strip64_hex_product(const StripRec64* __restrict__ r, Slot64& A,
-                                                     Slot64& B, Slot64& C, double& s_carry,
-                                                     double ox, double oy, double oz) {
-  const Geom64 g1 = strip64_geom(r[0], A, B, C, s_carry, ox, oy, oz);
-  const Geom64 g2 = strip64_geom(r[1], B, C, A, s_carry, ox, oy, oz);
-  const Geom64 g3 = strip64_geom(r[2], C, A, B, s_carry, ox, oy, oz);
-  const Prod3F64 t1 = prod3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
-  const double m1 = fmin(fmin(g1.dp, g2.dp), g3.dp);
-  const Geom64 g4 = strip64_geom(r[3], A, B, C, s_carry, ox, oy, oz);
-  const Geom64 g5 = strip64_geom(r[4], B, C, A, s_carry, ox, oy, oz);
-  const Geom64 g6 = strip64_geom(r[5], C, A, B, s_carry, ox, oy, oz);
-  const Prod3F64 t2 = prod3_f64(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
-  const double m2 = fmin(fmin(g4.dp, g5.dp), g6.dp);
-  Hex64 h;
-  h.re = __builtin_fma(-t1.im, t2.im, t1.re * t2.re);
-  h.im = __builtin_fma(t1.re, t2.im, t1.im * t2.re);
-  h.ok = __builtin_amdgcn_ballot_w64(fmin(m1, m2) > 0.0) &
-         lanes_positive_normal_f64(fmin(t1.re, t2.re)) & lanes_positive_normal_f64(h.re);
-  return h;
-}
-
 // Unscaled FP64 angle sum of one query over strip tiles [t_begin, t_end).
-// Records go in groups of twelve (four turns of the three register slots; a
-// tile's 255 = 21 x 12 + 3). When all twelve close a face (flags: scalar) and
-// every lane meets the group test (both sixes' conditions, the product's re
-// positive normal, |x| < 0.4: each six's angle sum within +-pi/2, the
-// twelve's within +-pi), the group's angle is the argument of the product of
-// its faces' (den + i det): one quotient and one 11-term polynomial for the
-// twelve; otherwise each six by its own test (strip64_hexad_of), a six that
-// fails as two groups of three (strip64_triple_of), and a triple that fails
-// face by face. The last three records of a tile go as a group of three.
+// Records go in groups of six (two turns of the three register slots; a
+// tile's 255 = 42 x 6 + 3). When all six close a face (flags: scalar) and
+// every lane meets the group test, the group's angle is the argument of the
+// product of its faces' (den + i det): one quotient and one 11-term
+// polynomial for the six; otherwise as two groups of three
+// (strip64_triple_of), and a triple that fails face by face. The last three
+// records of a tile go as a group of three.
 __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ rec64,
                                                 const double4* __restrict__ rec, int64_t t_begin,
                                                 int64_t t_end, double ox, double oy, double oz) {
@@ -264,26 +183,31 @@ __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ r
     const StripRec64* r = rec64 + t * kStripTileRecords;
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     int j = 0;
-    for (; j + 12 <= kStripTileRecords; j += 12) {
-      const Hex64 h1 = strip64_hex_product(r + j, A, B, C, s_carry, ox, oy, oz);
-      const Hex64 h2 = strip64_hex_product(r + j + 6, A, B, C, s_carry, ox, oy, oz);
-      bool all_faces = true;
-#pragma unroll
-      for (int i = 0; i < 12; ++i) all_faces = all_faces && flags[4 * (j + i)] != 0;
-      if (__builtin_expect(all_faces, 1)) {
-        const double re = __builtin_fma(-h1.im, h2.im, h1.re * h2.re);
-        const double im = __builtin_fma(h1.re, h2.im, h1.im * h2.re);
+    for (; j + 6 <= kStripTileRecords; j += 6) {
+      const long long f1 = flags[4 * j], f2 = flags[4 * (j + 1)], f3 = flags[4 * (j + 2)];
+      const long long f4 = flags[4 * (j + 3)], f5 = flags[4 * (j + 4)], f6 = flags[4 * (j + 5)];
+      const Geom64 g1 = strip64_geom(r[j], A, B, C, s_carry, ox, oy, oz);
+      const Geom64 g2 = strip64_geom(r[j + 1], B, C, A, s_carry, ox, oy, oz);
+      const Geom64 g3 = strip64_geom(r[j + 2], C, A, B, s_carry, ox, oy, oz);
+      const Prod3F64 t1 = prod3_f64(g1.det, g1.den, g2.det, g2.den, g3.det, g3.den);
+      const double m1 = fmin(fmin(g1.dp, g2.dp), g3.dp);
+      const Geom64 g4 = strip64_geom(r[j + 3], A, B, C, s_carry, ox, oy, oz);
+      const Geom64 g5 = strip64_geom(r[j + 4], B, C, A, s_carry, ox, oy, oz);
+      const Geom64 g6 = strip64_geom(r[j + 5], C, A, B, s_carry, ox, oy, oz);
+      const Prod3F64 t2 = prod3_f64(g4.det, g4.den, g5.det, g5.den, g6.det, g6.den);
+      const double m2 = fmin(fmin(g4.dp, g5.dp), g6.dp);
+      if (__builtin_expect(f1 != 0 && f2 != 0 && f3 != 0 && f4 != 0 && f5 != 0 && f6 != 0, 1)) {
+        const double re = __builtin_fma(-t1.im, t2.im, t1.re * t2.re);
+        const double im = __builtin_fma(t1.re, t2.im, t1.im * t2.re);
         const double x = div_f64(im, re);  // used only where re is positive normal
-        const uint64_t m = h1.ok & h2.ok & lanes_positive_normal_f64(re) &
-                           __builtin_amdgcn_ballot_w64(__builtin_fabs(x) < 0.4);
-        if (__builtin_expect(m == __builtin_amdgcn_read_exec(), 1)) {
+        if (__builtin_expect(wave_all_group_f64(fmin(m1, m2), fmin(t1.re, t2.re), re, x), 1)) {
           acc += atan_sum_f64(x);
           continue;
         }
       }
       asm volatile("" ::);  // keep a real wave-uniform branch
-      const double2 a = strip64_hexad_of(r, flags, j, h1.re, h1.im, h1.ok, ox, oy, oz);
-      const double2 b = strip64_hexad_of(r, flags, j + 6, h2.re, h2.im, h2.ok, ox, oy, oz);
+      const double2 a = strip64_triple_of(r, j, f1, f2, f3, t1.re, t1.im, m1, ox, oy, oz);
+      const double2 b = strip64_triple_of(r, j + 3, f4, f5, f6, t2.re, t2.im, m2, ox, oy, oz);
       acc += a.x;
       acc += b.x;
       acc_slow += a.y;
@@ -303,10 +227,9 @@ __device__ __forceinline__ double strip64_sweep(const StripRec64* __restrict__ r
   return acc + acc_slow;
 }
 
-// 5 waves per SIMD: 96 VGPRs, no spills on the fast path (the inlined
-// fallback of a failed group of twelve spills, and registers are saved around
-// the triples' calls; at 6 waves and 80 VGPRs groups of six spilled in the
-// loop, 6.5 % slower)
+// 5 waves per SIMD: 96 VGPRs, no spills on the fast path (the slow paths
+// are calls, around which registers are saved; at 6 waves and 80 VGPRs the
+// groups of six spilled in the loop, 6.5 % slower)
 __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5))) void fwd_strip64_kernel(
     const StripRec64* __restrict__ rec64, const double4* __restrict__ rec, int64_t ntiles,
     const double* __restrict__ q, const int32_t* __restrict__ perm, double* __restrict__ out,
