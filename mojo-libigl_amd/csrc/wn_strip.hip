@@ -4,9 +4,10 @@
 // Same arithmetic as fwd_f32c_kernel (block-local origin, FP32 pairs, FP64
 // accumulation, far-pair fast path, FP64 re-evaluation of ill-conditioned
 // pairs, the fast path on |det / den| < 1/8 with a 3-term arctangent),
-// except that the angles of six consecutive records are summed in FP32
-// before each FP64 add (error a few ulps of the six angles' magnitudes, the
-// bound of a compensated sum, with a sixth of the FP64 instructions), and
+// except that the angles of twelve consecutive records are summed in FP32
+// before each FP64 add (error a few ulps of the twelve angles' magnitudes,
+// the bound of a compensated sum, with a twelfth of the FP64 instructions),
+// and
 // the re-evaluated angle is rounded to FP32 once before that sum. Faces
 // arrive as a strip: record k brings vertex w_k and closes
 // face (w_{k-2}, w_{k-1}, w_k). Per face the lane computes one offset, one
@@ -29,12 +30,14 @@
 // staged with N = 0 and R1.x = +2^20, which makes its angle 0 without a
 // branch. The flag itself is read (from the records, at a wave-uniform
 // address) only on the rare slow path.
-// Fast path: for each group of six records, the sum of their angles is the
-// argument of the product of the faces' (den + i det) (sum6_f32): one
-// reciprocal and one 5-term polynomial per six faces instead of six of each.
-// A group that fails its test is taken as two groups of three (sum3_f32),
-// and a group of three that fails as single faces; the single faces are
-// rebuilt from the LDS tile, so the fast path keeps no per-face values live.
+// Fast path: for each group of twelve records, the sum of their angles is the
+// argument of the product of the faces' (den + i det) (two sixes, each two
+// triple products): one reciprocal and one 5-term polynomial per twelve faces
+// instead of twelve of each. A group that fails its test takes each six by its
+// own (strip_hexad_of), a six that fails is rebuilt from the LDS tile as two
+// groups of three (sum3_f32), and a group of three that fails goes face by
+// face; the rebuilt faces come from the LDS tile, so the fast path keeps no
+// per-face values live.
 // Squared lengths and N are computed in FP64 from the rounded FP32 corners.
 
 #include <algorithm>
@@ -453,7 +456,7 @@ __device__ __forceinline__ double strip_sweep(const double4* __restrict__ rec, i
     const float ox = ox0 * f, oy = oy0 * f, oz = oz0 * f;
     const long long* flags = reinterpret_cast<const long long*>(rec + t * kStripTileRecords) + 3;
     // The record offset is wave-uniform, but an LDS read takes its address
-    // from a VGPR: a VGPR byte offset advanced once per six records and read
+    // from a VGPR: a VGPR byte offset advanced once per twelve records and read
     // with immediate offsets. (The empty asm after the increment keeps it one
     // VGPR updated in place: placed before the use, it cost a copy per
     // iteration.)
