@@ -203,6 +203,48 @@ def test_fp32c_pure_fp32_is_worse_near_surface(require_gpu):
     assert err_fp32 >= err_default
 
 
+@pytest.mark.parametrize("qpl", [1, 2])
+def test_face_records_near_surface(require_gpu, qpl):
+    """The face-record kernels on an open cap (the 2,000 highest faces of a
+    displaced geodesic sphere) with 100,000 queries at 1e-6 .. 1e-1 from its
+    faces and 62,144 uniform in its box: many groups of twelve faces fail
+    their test and go through the sixes, triples, single faces and the FP64
+    re-evaluation, and the open rim takes the boundary correction. Against
+    the plain FP64 kernel: the FP32-compensated kernel within 1e-6 where a
+    query is at least 1e-3 from the surface and within 1e-4 (the near-surface
+    bar) closer, the FP64 face kernel within 1e-10; both bit for bit the same
+    over three runs."""
+    V, F = W.config5()[:2]
+    V = V.astype(np.float64)
+    cz = V[F].mean(axis=1)[:, 2]
+    Fs = F[np.sort(np.argsort(-cz)[:2000])]
+    rng = np.random.default_rng(78)
+    T = V[Fs[rng.integers(0, len(Fs), 100_000)]]
+    bary = rng.dirichlet(np.ones(3), size=len(T))
+    p = np.einsum("ij,ijk->ik", bary, T)
+    n = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    d = 10.0 ** rng.uniform(-6, -1, size=len(T)) * rng.choice([-1.0, 1.0], size=len(T))
+    box = V[np.unique(Fs)]
+    Q = np.vstack([p + d[:, None] * n, rng.uniform(box.min(0), box.max(0), size=(62_144, 3))])
+    tri = oracle.pack_soa(V, Fs)
+    ref = host_call(tri, Q, 2)
+    dist = np.sqrt(igl.point_mesh_squared_distance(Q, V, Fs)[0])
+    far = dist >= 1e-3
+    assert far.sum() > 50_000 and (~far).sum() > 50_000
+    w32 = [host_call(tri, Q, 1, qpl) for _ in range(3)]
+    w64 = [host_call(tri, Q, 0, qpl) for _ in range(3)]
+    err = np.abs(w32[0] - ref)
+    print(f"qpl {qpl}: fp32c max |dw| {err[far].max():.3e} (>= 1e-3 from the surface), "
+          f"{err[~far].max():.3e} (closer); fp64 face {np.abs(w64[0] - ref).max():.3e}", flush=True)
+    assert err[far].max() <= FP32C_TOL
+    assert err[~far].max() <= 1e-4
+    assert np.abs(w64[0] - ref).max() <= FP64_TOL
+    for runs in (w32, w64):
+        for w in runs[1:]:
+            assert np.array_equal(w, runs[0]), int((w != runs[0]).sum())
+
+
 @pytest.mark.parametrize("nf_take,nq", [(1, 1), (5, 7), (257, 255), (300, 513), (5120, 1)])
 def test_ragged_sizes(require_gpu, c1, nf_take, nq):
     """Odd face and query counts (tails of the LDS tile and of the query
