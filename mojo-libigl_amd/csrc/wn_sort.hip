@@ -5,7 +5,7 @@
 // those queries, and ill-conditioned (FP64) pairs cluster in few blocks.
 //
 // bbox (order-independent atomic min/max on order-preserving integer keys, so
-// deterministic) -> 30-bit codes -> rocPRIM radix sort of (code, index).
+// deterministic) -> 30-bit codes -> rocPRIM/hipCUB radix sort of (code, index).
 
 #include <hipcub/hipcub.hpp>
 
@@ -107,19 +107,11 @@ __global__ __launch_bounds__(kBlockThreads) void morton_kernel(
   vals[i] = (int32_t)i;
 }
 
-// The codes' radix sort takes rocPRIM's Onesweep from 131,072 keys up: its
-// default configuration keeps a block sort and ~10 merge passes up to
-// 1,048,576 keys; on the 1M-query step Onesweep's four 8-bit passes took
-// ~0.02 ms less (0.236 -> 0.214 ms of work around the strip kernel). Both
-// are stable, so the order is the same.
-using MortonSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                                    rocprim::default_config, 131072>;
-
 size_t sort_temp_bytes(int64_t nq) {
   size_t temp = 0;
-  (void)rocprim::radix_sort_pairs<MortonSortConfig>(nullptr, temp, (const uint32_t*)nullptr,
-                                                    (uint32_t*)nullptr, (const int32_t*)nullptr,
-                                                    (int32_t*)nullptr, (size_t)nq, 0u, 30u);
+  hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const uint32_t*)nullptr,
+                                     (uint32_t*)nullptr, (const int32_t*)nullptr,
+                                     (int32_t*)nullptr, (int)nq, 0, 30);
   return temp;
 }
 
@@ -172,8 +164,8 @@ hipError_t morton_order(const double* q, int64_t nq, int32_t* perm, void* ws,
                   stream>>>(q, nq, box, keys_in, vals_in);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return rocprim::radix_sort_pairs<MortonSortConfig>(cur, temp, keys_in, keys_out, vals_in, perm,
-                                                     (size_t)nq, 0u, 30u, stream);
+  return hipcub::DeviceRadixSort::SortPairs(cur, temp, keys_in, keys_out, vals_in, perm,
+                                            (int)nq, 0, 30, stream);
 }
 
 // ---------------------------------------------------------------------------
