@@ -49,6 +49,21 @@ __device__ __forceinline__ float atan2_f32(float y, float x, float& mx) {
   return __builtin_copysignf(a, y);
 }
 
+// First thread of the calling wave, in a scalar register (all lanes active).
+__device__ __forceinline__ int32_t wave_base() {
+  return __builtin_amdgcn_readfirstlane((int32_t)threadIdx.x) & ~(kWave - 1);
+}
+
+// threadIdx.x rebuilt from the wave base and the lane's position in the
+// wave, computed where it is called: an output index formed from it after a
+// sweep needs no vector register kept live across the sweep. (The mbcnt pair
+// sits in a volatile asm so the compiler cannot hoist it above the sweep.)
+__device__ __forceinline__ int32_t thread_index(int32_t base) {
+  int32_t lane;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
+  return base + lane;
+}
+
 // ---------------------------------------------------------------------------
 // Power-of-two frames of the FP32-compensated kernels.
 //

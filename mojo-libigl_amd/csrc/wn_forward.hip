@@ -375,14 +375,12 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
   const int tid = threadIdx.x;
   const int64_t qbase = (int64_t)blockIdx.x * (kBlockThreads * QPL);
 
-  int64_t qidx[QPL];
   bool valid[QPL];
   double Ox[QPL], Oy[QPL], Oz[QPL];
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
     const int64_t slot = qbase + k * kBlockThreads + tid;
     valid[k] = slot < nq;
-    qidx[k] = valid[k] ? (perm ? (int64_t)perm[slot] : slot) : 0;
     // q is in launch order (gathered through perm beforehand when sorted)
     Ox[k] = valid[k] ? q[3 * slot] : 0.0;
     Oy[k] = valid[k] ? q[3 * slot + 1] : 0.0;
@@ -406,13 +404,17 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(Q
   const int64_t f1 = f0 + faces_per_split < nf ? f0 + faces_per_split : nf;
   sweep_f32c<QPL>(tri, nf, f0, f1, gx, gy, gz, er, ox, oy, oz, kappa, acc, sm);
 
+  // output slots rebuilt after the sweep (kept live across it, the 64-bit
+  // output index was spilled to scratch at every block start)
+  const int32_t t = thread_index(wave_base());
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
-    if (valid[k]) {
+    const int64_t slot = (int64_t)blockIdx.x * (kBlockThreads * QPL) + k * kBlockThreads + t;
+    if (slot < nq) {
       if (splits == 1)
-        out[qidx[k]] = acc[k] * kInv2Pi;
+        out[perm ? (int64_t)perm[slot] : slot] = acc[k] * kInv2Pi;
       else
-        out[(int64_t)blockIdx.y * nq + qbase + k * kBlockThreads + tid] = acc[k];
+        out[(int64_t)blockIdx.y * nq + slot] = acc[k];
     }
   }
 }

@@ -63,21 +63,6 @@ struct VSlot {
   float ax, ay, az, l2, l;
 };
 
-// First thread of the calling wave, in a scalar register (all lanes active).
-__device__ __forceinline__ int32_t wave_base() {
-  return __builtin_amdgcn_readfirstlane((int32_t)threadIdx.x) & ~(kWave - 1);
-}
-
-// threadIdx.x rebuilt from the wave base and the lane's position in the
-// wave, computed where it is called: an output index formed from it after a
-// sweep needs no vector register kept live across the sweep. (The mbcnt pair
-// sits in a volatile asm so the compiler cannot hoist it above the sweep.)
-__device__ __forceinline__ int32_t thread_index(int32_t base) {
-  int32_t lane;
-  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lane));
-  return base + lane;
-}
-
 
 // One tile's staging, in two steps around a block barrier. strip_tile_extent:
 // every thread of the block (all lanes: the wave reduction needs them) reads
