@@ -243,11 +243,14 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(5
   const int64_t t_begin = (int64_t)blockIdx.y * tiles_per_split;
   const int64_t t_end = t_begin + tiles_per_split < ntiles ? t_begin + tiles_per_split : ntiles;
   const double acc = strip64_sweep(rec64, rec, t_begin, t_end, ox, oy, oz);
-  if (valid) {
+  // the output slot rebuilt after the sweep (kept live across it, it was
+  // spilled to scratch at every block start)
+  const int64_t s = (int64_t)blockIdx.x * kBlockThreads + thread_index(wave_base());
+  if (s < nq) {
     if (splits == 1)
-      out[perm ? (int64_t)perm[slot] : slot] = acc * kInv2Pi;
+      out[perm ? (int64_t)perm[s] : s] = acc * kInv2Pi;
     else
-      out[(int64_t)blockIdx.y * nq + slot] = acc;
+      out[(int64_t)blockIdx.y * nq + s] = acc;
   }
 }
 
